@@ -1,0 +1,740 @@
+// Host side of the C ABI declared in include/lbc.h: context and device
+// buffer management, launch sequencing on the context's stream, HIP-graph
+// capture of a whole cloning step, and event timing of the step kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lbc.h"
+#include "lbc_internal.h"
+
+using lbc::Counters;
+using lbc::InstDev;
+using lbc::WorkItem;
+
+namespace {
+
+thread_local std::string g_err;
+
+constexpr double kW[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,
+                          1.0 / 9.0,  1.0 / 9.0,  1.0 / 36.0,
+                          1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+constexpr int kCX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+constexpr int kCY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+constexpr int kMaxClones = 8192;
+constexpr int kMaxIntervals = 1 << 16;
+
+size_t elem_bytes(int storage) {
+  switch (storage) {
+    case LBC_STORE_F64: return 8;
+    case LBC_STORE_F32: return 4;
+    default: return 2;
+  }
+}
+
+}  // namespace
+
+struct lbc_ctx {
+  lbc_config cfg{};
+  std::vector<lbc_instance> inst;
+  hipStream_t stream = nullptr;
+  std::vector<InstDev> idh;           // host copies of the descriptors
+  InstDev* idd = nullptr;             // device descriptors
+  Counters* counters = nullptr;
+  WorkItem* items = nullptr;
+  int n_items = 0;
+  int* flip_mask = nullptr;
+  std::vector<int> parity;            // host mirror of counters->parity
+  std::vector<int> selected;          // selection done since last gather
+  std::vector<void*> allocs;
+  double* staging = nullptr;          // AoS FP64 transfer buffer (device)
+  size_t staging_bytes = 0;
+  int threads = 256;
+  int ring_max = 0;
+  int n_max = 0;
+  std::string err;
+  std::vector<hipEvent_t> ev;         // pairs around step-kernel launches
+  int ev_used = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+};
+
+namespace {
+
+int fail(lbc_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  g_err = msg;
+  return code;
+}
+
+#define HIPC(ctx, expr)                                                      \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail((ctx), LBC_E_HIP,                                          \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+void feq_ref_order(double rho, double ux, double uy, double out[9]) {
+  double u2 = -1.5 * (ux * ux + uy * uy);
+  for (int k = 0; k < 9; k++) {
+    double eu = kCX[k] * ux + kCY[k] * uy;
+    double eueu = 4.5 * eu * eu;
+    out[k] = kW[k] * rho * (1.0 + 3.0 * eu + eueu + u2);
+  }
+}
+
+int validate(const lbc_config* cfg, std::string& why) {
+  if (!cfg || !cfg->instances) return why = "null config", -1;
+  if (cfg->mode != LBC_MODE_REF && cfg->mode != LBC_MODE_CHANNEL)
+    return why = "mode must be LBC_MODE_REF or LBC_MODE_CHANNEL", -1;
+  if (cfg->storage < LBC_STORE_F64 || cfg->storage > LBC_STORE_BF16)
+    return why = "unknown storage", -1;
+  if (cfg->n_instances < 1 || cfg->n_instances > lbc::kMaxInstances)
+    return why = "n_instances must be in [1, 64]", -1;
+  if (cfg->threads != 0 &&
+      (cfg->threads < 64 || cfg->threads > 1024 || cfg->threads % 64))
+    return why = "threads must be 0 or a multiple of 64 in [64, 1024]", -1;
+  for (int i = 0; i < cfg->n_instances; i++) {
+    const lbc_instance& s = cfg->instances[i];
+    char buf[160];
+    auto bad = [&](const char* m) {
+      snprintf(buf, sizeof buf, "instance %d: %s", i, m);
+      why = buf;
+      return -1;
+    };
+    if (s.Lx < 8 || s.Ly < 8) return bad("lattice must be at least 8x8");
+    if ((long long)s.Lx * s.Ly * 9 > (1LL << 30)) return bad("lattice too large");
+    if (s.xmin < 2 || s.xmax > s.Lx - 3 || s.ymin < 2 || s.ymax > s.Ly - 3)
+      return bad("square must keep two fluid nodes from every boundary");
+    if (s.xmax - s.xmin < 2 || s.ymax - s.ymin < 2)
+      return bad("square side must be at least 2");
+    if (s.n_clones < 1 || s.n_clones > kMaxClones)
+      return bad("n_clones must be in [1, 8192]");
+    if (s.steps < 1) return bad("steps must be >= 1");
+    if (!(s.tau > 0.5)) return bad("tau must exceed 0.5");
+    if (s.rank_blocks < 1 || s.n_clones % s.rank_blocks)
+      return bad("rank_blocks must divide n_clones");
+    if (s.n_snapshots < 0 || s.n_snapshots > 64)
+      return bad("n_snapshots must be in [0, 64]");
+    if (!std::isfinite(s.force_scale) || !std::isfinite(s.force_shift) ||
+        !std::isfinite(s.dt) || !std::isfinite(s.bias) ||
+        !std::isfinite(s.beta) || !std::isfinite(s.u_in))
+      return bad("non-finite parameter");
+  }
+  return 0;
+}
+
+template <class T>
+int dalloc(lbc_ctx* c, T** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(c, LBC_E_NOMEM,
+                std::string("hipMalloc(") + std::to_string(bytes) +
+                    "): " + hipGetErrorString(e));
+  }
+  c->allocs.push_back((void*)*p);
+  return LBC_OK;
+}
+
+lbc::Launch make_launch(lbc_ctx* c) {
+  lbc::Launch L{};
+  L.insts = c->idd;
+  L.items = c->items;
+  L.counters = c->counters;
+  L.n_items = c->n_items;
+  L.threads = c->threads;
+  L.ring_max = c->ring_max;
+  L.mode = c->cfg.mode;
+  L.storage = c->cfg.storage;
+  L.vec = 1;
+  L.stream = c->stream;
+  return L;
+}
+
+uint32_t seed_lo(const lbc_ctx* c) { return (uint32_t)(c->cfg.seed & 0xFFFFFFFFu); }
+uint32_t seed_hi(const lbc_ctx* c) { return (uint32_t)(c->cfg.seed >> 32); }
+
+int ensure_staging(lbc_ctx* c, size_t bytes) {
+  if (bytes <= c->staging_bytes) return LBC_OK;
+  if (c->staging) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->staging);
+    c->staging = nullptr;
+    c->staging_bytes = 0;
+  }
+  hipError_t e = hipMalloc((void**)&c->staging, bytes);
+  if (e != hipSuccess) return fail(c, LBC_E_NOMEM, "staging allocation failed");
+  c->staging_bytes = bytes;
+  return LBC_OK;
+}
+
+int check_inst(lbc_ctx* c, int inst) {
+  if (!c) return fail(nullptr, LBC_E_ARG, "null context");
+  if (inst < 0 || inst >= c->cfg.n_instances)
+    return fail(c, LBC_E_ARG, "instance index out of range");
+  return LBC_OK;
+}
+
+// Step kernel + parity flip for every instance, bracketed by events.
+int enqueue_interval(lbc_ctx* c, bool timed) {
+  lbc::Launch L = make_launch(c);
+  if (timed) {
+    if (c->ev_used + 2 > (int)c->ev.size()) {
+      for (int i = 0; i < 64; i++) {
+        hipEvent_t e;
+        HIPC(c, hipEventCreate(&e));
+        c->ev.push_back(e);
+      }
+    }
+    HIPC(c, hipEventRecord(c->ev[c->ev_used], c->stream));
+  }
+  HIPC(c, lbc::launch_interval(L));
+  if (timed) {
+    HIPC(c, hipEventRecord(c->ev[c->ev_used + 1], c->stream));
+    c->ev_used += 2;
+  }
+  HIPC(c, lbc::launch_flip(L, c->cfg.n_instances, c->flip_mask));
+  for (int i = 0; i < c->cfg.n_instances; i++)
+    c->parity[i] ^= (c->inst[i].steps + 1) & 1;
+  return LBC_OK;
+}
+
+int enqueue_perturb(lbc_ctx* c, bool philox) {
+  if (c->cfg.mode != LBC_MODE_REF) return LBC_OK;
+  bool any = false;
+  for (auto& s : c->inst) any |= s.n_snapshots > 0;
+  if (!any) return LBC_OK;
+  lbc::Launch L = make_launch(c);
+  HIPC(c, lbc::launch_perturb(L, philox ? 1 : 0, seed_lo(c), seed_hi(c)));
+  return LBC_OK;
+}
+
+int autotune(lbc_ctx* c) {
+  // Time short intervals at each workgroup size on the freshly initialised
+  // state; the state is re-initialised afterwards by the caller.
+  std::vector<InstDev> tmp = c->idh;
+  for (auto& d : tmp) d.steps = std::min(d.steps, 8);
+  HIPC(c, hipMemcpyAsync(c->idd, tmp.data(), sizeof(InstDev) * tmp.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  const int cands[3] = {256, 512, 1024};
+  float best = 1e30f;
+  int best_t = 256;
+  hipEvent_t e0, e1;
+  HIPC(c, hipEventCreate(&e0));
+  HIPC(c, hipEventCreate(&e1));
+  for (int t : cands) {
+    lbc::Launch L = make_launch(c);
+    L.threads = t;
+    HIPC(c, lbc::launch_interval(L));  // warm-up
+    HIPC(c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < 2; r++) HIPC(c, lbc::launch_interval(L));
+    HIPC(c, hipEventRecord(e1, c->stream));
+    HIPC(c, hipEventSynchronize(e1));
+    float ms = 0;
+    HIPC(c, hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      best_t = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->threads = best_t;
+  HIPC(c, hipMemcpyAsync(c->idd, c->idh.data(), sizeof(InstDev) * c->idh.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lbc_last_error(const lbc_ctx* ctx) {
+  if (ctx && !ctx->err.empty()) return ctx->err.c_str();
+  return g_err.c_str();
+}
+
+int lbc_threads(const lbc_ctx* ctx) { return ctx ? ctx->threads : 0; }
+
+void lbc_destroy(lbc_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (auto e : c->ev) (void)hipEventDestroy(e);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->staging) (void)hipFree(c->staging);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
+  if (!out) return fail(nullptr, LBC_E_ARG, "null output pointer");
+  *out = nullptr;
+  std::string why;
+  if (validate(cfg, why)) return fail(nullptr, LBC_E_ARG, why);
+  lbc_ctx* c = new lbc_ctx();
+  c->cfg = *cfg;
+  c->inst.assign(cfg->instances, cfg->instances + cfg->n_instances);
+  c->cfg.instances = c->inst.data();
+  auto bail = [&](int code) {
+    lbc_destroy(c);
+    return code;
+  };
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return bail(fail(nullptr, LBC_E_HIP, "no HIP device available"));
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return bail(fail(nullptr, LBC_E_ARG, "device ordinal out of range"));
+  if (hipSetDevice(cfg->device) != hipSuccess)
+    return bail(fail(nullptr, LBC_E_HIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(nullptr, LBC_E_HIP, "hipStreamCreate failed"));
+
+  const int ni = cfg->n_instances;
+  const size_t eb = elem_bytes(cfg->storage);
+  c->idh.resize(ni);
+  c->parity.assign(ni, 0);
+  c->selected.assign(ni, 0);
+  int clone_base = 0;
+  for (int i = 0; i < ni; i++) {
+    const lbc_instance& s = c->inst[i];
+    InstDev& d = c->idh[i];
+    std::memset(&d, 0, sizeof d);
+    d.Lx = s.Lx;
+    d.Ly = s.Ly;
+    d.xmin = s.xmin;
+    d.xmax = s.xmax;
+    d.ymin = s.ymin;
+    d.ymax = s.ymax;
+    d.plane = s.Lx * s.Ly;
+    // clones start on 128-element boundaries
+    d.clone_stride = ((long long)9 * d.plane + 127) / 128 * 128;
+    d.n_clones = s.n_clones;
+    d.steps = s.steps;
+    d.ring_n = 2 * (s.xmax - s.xmin + 1) + 2 * (s.ymax - s.ymin + 1);
+    d.n_snapshots = cfg->mode == LBC_MODE_REF ? s.n_snapshots : 0;
+    d.rank_blocks = s.rank_blocks;
+    d.clone_base = clone_base;
+    clone_base += s.n_clones;
+    d.tau = s.tau;
+    d.beta = cfg->mode == LBC_MODE_REF ? s.beta : 0.0;
+    d.u_in = cfg->mode == LBC_MODE_CHANNEL ? s.u_in : 0.0;
+    d.scale = s.force_scale;
+    d.shift = s.force_shift;
+    d.dt = s.dt;
+    d.bias = s.bias;
+    d.eps = s.eps;
+    d.sigma = s.noise_sigma;
+    feq_ref_order(1.0, d.u_in, 0.0, d.feq_in);
+    for (int k = 0; k < 9; k++) d.feq_in_s[k] = (float)(d.feq_in[k] - kW[k]);
+    d.max_intervals = kMaxIntervals;
+    c->ring_max = std::max(c->ring_max, d.ring_n);
+    c->n_max = std::max(c->n_max, s.n_clones);
+    const size_t n = (size_t)s.n_clones;
+    int rc = LBC_OK;
+    for (int b = 0; b < 2 && rc == LBC_OK; b++)
+      rc = dalloc(c, &d.buf[b], (size_t)d.clone_stride * n * eb);
+    if (rc == LBC_OK) rc = dalloc(c, &d.drag, sizeof(double) * n * s.steps);
+    if (rc == LBC_OK) rc = dalloc(c, &d.obs, sizeof(double) * n);
+    if (rc == LBC_OK) rc = dalloc(c, &d.logw, sizeof(double) * n);
+    if (rc == LBC_OK) rc = dalloc(c, &d.weights, sizeof(double) * n);
+    if (rc == LBC_OK) rc = dalloc(c, &d.u, sizeof(double) * n);
+    if (rc == LBC_OK) rc = dalloc(c, &d.rint, sizeof(int32_t) * n);
+    if (rc == LBC_OK) rc = dalloc(c, &d.nb_before, sizeof(int32_t) * n);
+    if (rc == LBC_OK) rc = dalloc(c, &d.nb_after, sizeof(int32_t) * n);
+    if (rc == LBC_OK) rc = dalloc(c, &d.pairs, sizeof(int32_t) * 2 * n);
+    if (rc == LBC_OK) rc = dalloc(c, &d.labels, sizeof(int32_t) * n);
+    if (rc == LBC_OK) rc = dalloc(c, &d.sel_info, sizeof(int32_t) * 4);
+    if (rc == LBC_OK) rc = dalloc(c, &d.sel_R, sizeof(double) * 2);
+    if (rc == LBC_OK)
+      rc = dalloc(c, &d.R_hist, sizeof(double) * 2 * (size_t)kMaxIntervals);
+    if (rc == LBC_OK) rc = dalloc(c, &d.marks, n);
+    if (rc == LBC_OK)
+      rc = dalloc(c, &d.prefactors,
+                  sizeof(double) * n * std::max(1, d.n_snapshots));
+    if (rc == LBC_OK)
+      rc = dalloc(c, &d.snapshots,
+                  sizeof(double) * 9 * (size_t)d.plane * d.n_snapshots);
+    if (rc == LBC_OK) rc = dalloc(c, &d.norms, sizeof(double) * 2 * n);
+    if (rc != LBC_OK) return bail(rc);
+    if (hipMemsetAsync(d.marks, 0, n, c->stream) != hipSuccess ||
+        hipMemsetAsync(d.drag, 0, sizeof(double) * n * s.steps, c->stream) !=
+            hipSuccess)
+      return bail(fail(c, LBC_E_HIP, "hipMemsetAsync failed"));
+    std::vector<int32_t> ident(n);
+    for (size_t j = 0; j < n; j++) ident[j] = (int32_t)j;
+    if (hipMemcpy(d.labels, ident.data(), sizeof(int32_t) * n,
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(c, LBC_E_HIP, "label init failed"));
+  }
+  if (dalloc(c, &c->idd, sizeof(InstDev) * ni) ||
+      dalloc(c, &c->counters, sizeof(Counters)) ||
+      dalloc(c, &c->flip_mask, sizeof(int) * lbc::kMaxInstances))
+    return bail(LBC_E_NOMEM);
+  // work items: biggest lattices first so the tail of the launch is short
+  std::vector<int> order(ni);
+  for (int i = 0; i < ni; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (long long)c->idh[a].plane * c->idh[a].steps >
+           (long long)c->idh[b].plane * c->idh[b].steps;
+  });
+  std::vector<WorkItem> items;
+  for (int i : order)
+    for (int j = 0; j < c->inst[i].n_clones; j++) items.push_back({i, j});
+  c->n_items = (int)items.size();
+  if (dalloc(c, &c->items, sizeof(WorkItem) * items.size())) return bail(LBC_E_NOMEM);
+  std::vector<int> flips(lbc::kMaxInstances, 0);
+  for (int i = 0; i < ni; i++) flips[i] = (c->inst[i].steps + 1) & 1;
+  Counters zero{};
+  if (hipMemcpy(c->idd, c->idh.data(), sizeof(InstDev) * ni,
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->items, items.data(), sizeof(WorkItem) * items.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->flip_mask, flips.data(), sizeof(int) * flips.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->counters, &zero, sizeof zero, hipMemcpyHostToDevice) !=
+          hipSuccess)
+    return bail(fail(c, LBC_E_HIP, "descriptor upload failed"));
+
+  // valid initial state everywhere: equilibrium of the mode's inflow
+  lbc::Launch L = make_launch(c);
+  for (int i = 0; i < ni; i++) {
+    if (lbc::launch_init_eq(L, i, c->inst[i].n_clones, 0.0, 0, 0) != hipSuccess)
+      return bail(fail(c, LBC_E_HIP, "init kernel launch failed"));
+  }
+  if (cfg->threads == 0) {
+    int rc = autotune(c);
+    if (rc) return bail(rc);
+    L = make_launch(c);
+    for (int i = 0; i < ni; i++)
+      if (lbc::launch_init_eq(L, i, c->inst[i].n_clones, 0.0, 0, 0) != hipSuccess)
+        return bail(fail(c, LBC_E_HIP, "init kernel launch failed"));
+  } else {
+    c->threads = cfg->threads;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess)
+    return bail(fail(c, LBC_E_HIP, "initialisation failed"));
+  *out = c;
+  return LBC_OK;
+}
+
+int lbc_load_states(lbc_ctx* c, int inst, int clone0, int n, const double* aos) {
+  if (int rc = check_inst(c, inst)) return rc;
+  const lbc_instance& s = c->inst[inst];
+  if (!aos || n < 1 || clone0 < 0 || clone0 + n > s.n_clones)
+    return fail(c, LBC_E_ARG, "clone range out of bounds");
+  const size_t bytes = sizeof(double) * 9 * (size_t)s.Lx * s.Ly * n;
+  if (int rc = ensure_staging(c, bytes)) return rc;
+  HIPC(c, hipMemcpyAsync(c->staging, aos, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, lbc::launch_aos_to_soa(c->cfg.storage, c->idd, c->counters, inst, 0,
+                                 clone0, n, c->staging, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+static int store_which(lbc_ctx* c, int inst, int which, int clone0, int n,
+                       double* aos) {
+  if (int rc = check_inst(c, inst)) return rc;
+  const lbc_instance& s = c->inst[inst];
+  if (!aos || n < 1 || clone0 < 0 || clone0 + n > s.n_clones)
+    return fail(c, LBC_E_ARG, "clone range out of bounds");
+  const size_t bytes = sizeof(double) * 9 * (size_t)s.Lx * s.Ly * n;
+  if (int rc = ensure_staging(c, bytes)) return rc;
+  HIPC(c, lbc::launch_soa_to_aos(c->cfg.storage, c->idd, c->counters, inst,
+                                 which, clone0, n, c->staging, c->stream));
+  HIPC(c, hipMemcpyAsync(aos, c->staging, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+int lbc_store_states(lbc_ctx* c, int inst, int clone0, int n, double* aos) {
+  return store_which(c, inst, 0, clone0, n, aos);
+}
+
+int lbc_init_equilibrium(lbc_ctx* c, int inst, double amp) {
+  if (int rc = check_inst(c, inst)) return rc;
+  if (!std::isfinite(amp) || amp < 0) return fail(c, LBC_E_ARG, "bad amplitude");
+  lbc::Launch L = make_launch(c);
+  HIPC(c, lbc::launch_init_eq(L, inst, c->inst[inst].n_clones, amp, seed_lo(c),
+                              seed_hi(c)));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+int lbc_set_snapshots(lbc_ctx* c, int inst, const double* aos) {
+  if (int rc = check_inst(c, inst)) return rc;
+  const InstDev& d = c->idh[inst];
+  if (d.n_snapshots <= 0)
+    return fail(c, LBC_E_STATE, "instance has no snapshot slots (REF mode, n_snapshots > 0)");
+  if (!aos) return fail(c, LBC_E_ARG, "null snapshots");
+  const size_t bytes = sizeof(double) * 9 * (size_t)d.plane * d.n_snapshots;
+  if (int rc = ensure_staging(c, bytes)) return rc;
+  HIPC(c, hipMemcpyAsync(c->staging, aos, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, lbc::launch_snap_to_soa(c->idd, inst, d.n_snapshots, c->staging, d.Lx,
+                                  d.Ly, d.snapshots, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+int lbc_set_marks(lbc_ctx* c, int inst, const uint8_t* marks) {
+  if (int rc = check_inst(c, inst)) return rc;
+  if (!marks) return fail(c, LBC_E_ARG, "null marks");
+  HIPC(c, hipMemcpyAsync(c->idh[inst].marks, marks, c->inst[inst].n_clones,
+                         hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+int lbc_get_marks(lbc_ctx* c, int inst, uint8_t* marks) {
+  if (int rc = check_inst(c, inst)) return rc;
+  if (!marks) return fail(c, LBC_E_ARG, "null marks");
+  HIPC(c, hipMemcpyAsync(marks, c->idh[inst].marks, c->inst[inst].n_clones,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+int lbc_run_interval(lbc_ctx* c, const double* prefactors) {
+  if (!c) return fail(nullptr, LBC_E_ARG, "null context");
+  bool philox = true;
+  if (prefactors && c->cfg.mode == LBC_MODE_REF) {
+    philox = false;
+    size_t off = 0;
+    for (int i = 0; i < c->cfg.n_instances; i++) {
+      const InstDev& d = c->idh[i];
+      const size_t cnt = (size_t)d.n_clones * d.n_snapshots;
+      if (cnt)
+        HIPC(c, hipMemcpyAsync(d.prefactors, prefactors + off,
+                               sizeof(double) * cnt, hipMemcpyHostToDevice,
+                               c->stream));
+      off += cnt;
+    }
+  }
+  c->ev_used = 0;
+  if (int rc = enqueue_perturb(c, philox)) return rc;
+  if (int rc = enqueue_interval(c, true)) return rc;
+  return LBC_OK;
+}
+
+int lbc_get_drag(lbc_ctx* c, int inst, double* A) {
+  if (int rc = check_inst(c, inst)) return rc;
+  if (!A) return fail(c, LBC_E_ARG, "null output");
+  const InstDev& d = c->idh[inst];
+  HIPC(c, hipMemcpyAsync(A, d.drag, sizeof(double) * d.n_clones * d.steps,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+int lbc_get_logweights(lbc_ctx* c, int inst, double* obs, double* logw) {
+  if (int rc = check_inst(c, inst)) return rc;
+  const InstDev& d = c->idh[inst];
+  if (obs)
+    HIPC(c, hipMemcpyAsync(obs, d.obs, sizeof(double) * d.n_clones,
+                           hipMemcpyDeviceToHost, c->stream));
+  if (logw)
+    HIPC(c, hipMemcpyAsync(logw, d.logw, sizeof(double) * d.n_clones,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+int lbc_select(lbc_ctx* c, int inst, const double* weights, const double* u,
+               const int32_t* rint, lbc_selection* out) {
+  if (int rc = check_inst(c, inst)) return rc;
+  const InstDev& d = c->idh[inst];
+  const size_t n = (size_t)d.n_clones;
+  if ((u == nullptr) != (rint == nullptr))
+    return fail(c, LBC_E_ARG, "u and rint must both be given or both be NULL");
+  if (weights)
+    HIPC(c, hipMemcpyAsync(d.weights, weights, sizeof(double) * n,
+                           hipMemcpyHostToDevice, c->stream));
+  if (u) {
+    for (size_t j = 0; j < n; j++)
+      if (rint[j] < 0) return fail(c, LBC_E_ARG, "rint must be non-negative");
+    HIPC(c, hipMemcpyAsync(d.u, u, sizeof(double) * n, hipMemcpyHostToDevice,
+                           c->stream));
+    HIPC(c, hipMemcpyAsync(d.rint, rint, sizeof(int32_t) * n,
+                           hipMemcpyHostToDevice, c->stream));
+  }
+  // one-instance launch through a descriptor pointer offset
+  HIPC(c, lbc::launch_select(c->idd, c->counters, inst, 1, d.n_clones,
+                             weights ? 1 : 0, u ? 1 : 0, seed_lo(c), seed_hi(c),
+                             c->stream));
+  c->selected[inst] = 1;
+  int32_t info[4];
+  double R[2];
+  HIPC(c, hipMemcpyAsync(info, d.sel_info, sizeof info, hipMemcpyDeviceToHost,
+                         c->stream));
+  HIPC(c, hipMemcpyAsync(R, d.sel_R, sizeof R, hipMemcpyDeviceToHost, c->stream));
+  if (out) {
+    if (out->nb_before)
+      HIPC(c, hipMemcpyAsync(out->nb_before, d.nb_before, 4 * n,
+                             hipMemcpyDeviceToHost, c->stream));
+    if (out->nb_after)
+      HIPC(c, hipMemcpyAsync(out->nb_after, d.nb_after, 4 * n,
+                             hipMemcpyDeviceToHost, c->stream));
+    if (out->labels)
+      HIPC(c, hipMemcpyAsync(out->labels, d.labels, 4 * n,
+                             hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (out) {
+    out->n_pairs = info[0];
+    out->n_rint_used = info[1];
+    out->total_R = R[0];
+    out->log_total_R = R[1];
+    if (out->pairs && info[0] > 0) {
+      HIPC(c, hipMemcpy(out->pairs, d.pairs, sizeof(int32_t) * 2 * info[0],
+                        hipMemcpyDeviceToHost));
+    }
+  }
+  if (info[2] != 0)
+    return fail(c, LBC_E_NUMERIC,
+                "selection: non-finite or out-of-range weights");
+  return LBC_OK;
+}
+
+int lbc_gather(lbc_ctx* c) {
+  if (!c) return fail(nullptr, LBC_E_ARG, "null context");
+  for (int i = 0; i < c->cfg.n_instances; i++)
+    if (!c->selected[i])
+      return fail(c, LBC_E_STATE, "lbc_gather: every instance needs lbc_select first");
+  lbc::Launch L = make_launch(c);
+  HIPC(c, lbc::launch_gather(L, seed_lo(c), seed_hi(c)));
+  for (int i = 0; i < c->cfg.n_instances; i++) c->selected[i] = 0;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+int lbc_advance(lbc_ctx* c, int n_intervals) {
+  if (!c) return fail(nullptr, LBC_E_ARG, "null context");
+  if (n_intervals < 0) return fail(c, LBC_E_ARG, "negative interval count");
+  if (!c->graph_exec) {
+    // capture one cloning step: perturb (REF), step kernel, flip, selection
+    // of every instance, gather
+    HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    lbc::Launch L = make_launch(c);
+    int rc = LBC_OK;
+    if (c->cfg.mode == LBC_MODE_REF)
+      if (lbc::launch_perturb(L, 1, seed_lo(c), seed_hi(c)) != hipSuccess)
+        rc = LBC_E_HIP;
+    if (!rc && lbc::launch_interval(L) != hipSuccess) rc = LBC_E_HIP;
+    if (!rc && lbc::launch_flip(L, c->cfg.n_instances, c->flip_mask) != hipSuccess)
+      rc = LBC_E_HIP;
+    if (!rc && lbc::launch_select(c->idd, c->counters, 0, c->cfg.n_instances,
+                                  c->n_max, 0, 0, seed_lo(c), seed_hi(c),
+                                  c->stream) != hipSuccess)
+      rc = LBC_E_HIP;
+    if (!rc && lbc::launch_gather(L, seed_lo(c), seed_hi(c)) != hipSuccess)
+      rc = LBC_E_HIP;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(c, LBC_E_HIP, "graph capture failed");
+    }
+    c->graph = g;
+    HIPC(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+  }
+  c->ev_used = 0;
+  for (int k = 0; k < n_intervals; k++) {
+    if (c->ev_used + 2 > (int)c->ev.size()) {
+      for (int i = 0; i < 64; i++) {
+        hipEvent_t e;
+        HIPC(c, hipEventCreate(&e));
+        c->ev.push_back(e);
+      }
+    }
+    HIPC(c, hipEventRecord(c->ev[c->ev_used], c->stream));
+    HIPC(c, hipGraphLaunch(c->graph_exec, c->stream));
+    HIPC(c, hipEventRecord(c->ev[c->ev_used + 1], c->stream));
+    c->ev_used += 2;
+    for (int i = 0; i < c->cfg.n_instances; i++)
+      c->parity[i] ^= (c->inst[i].steps + 1) & 1;
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+int lbc_get_R_history(lbc_ctx* c, int inst, double* R, int max_n, int* n_out) {
+  if (int rc = check_inst(c, inst)) return rc;
+  Counters h;
+  HIPC(c, hipMemcpyAsync(&h, c->counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  const int n = std::min(h.interval[inst], kMaxIntervals);
+  if (n_out) *n_out = n;
+  if (R && max_n > 0) {
+    std::vector<double> tmp(2 * (size_t)std::min(n, max_n));
+    if (!tmp.empty()) {
+      HIPC(c, hipMemcpy(tmp.data(), c->idh[inst].R_hist,
+                        sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+      for (size_t t = 0; t < tmp.size() / 2; t++) R[t] = tmp[2 * t];
+    }
+  }
+  return LBC_OK;
+}
+
+int lbc_sync(lbc_ctx* c) {
+  if (!c) return fail(nullptr, LBC_E_ARG, "null context");
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+int lbc_step_kernel_ms(lbc_ctx* c, double* avg_ms, int* n_launches) {
+  if (!c || !avg_ms) return fail(c, LBC_E_ARG, "null argument");
+  HIPC(c, hipStreamSynchronize(c->stream));
+  double tot = 0;
+  int n = 0;
+  for (int i = 0; i + 1 < c->ev_used; i += 2) {
+    float ms = 0;
+    HIPC(c, hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+    tot += ms;
+    n++;
+  }
+  *avg_ms = n ? tot / n : 0.0;
+  if (n_launches) *n_launches = n;
+  return LBC_OK;
+}
+
+int lbc_step_host(int mode, const lbc_instance* geom, double* fin, double* fout,
+                  double* force) {
+  if (!geom || !fin || !fout) return fail(nullptr, LBC_E_ARG, "null argument");
+  lbc_instance s = *geom;
+  s.n_clones = 1;
+  s.steps = 1;
+  s.force_scale = 1.0;
+  s.force_shift = 0.0;
+  s.dt = 1.0;
+  s.rank_blocks = 1;
+  s.n_snapshots = 0;
+  lbc_config cfg{};
+  cfg.mode = mode;
+  cfg.storage = LBC_STORE_F64;
+  cfg.n_instances = 1;
+  cfg.threads = 256;
+  cfg.instances = &s;
+  lbc_ctx* c = nullptr;
+  int rc = lbc_create(&cfg, &c);
+  if (rc) return rc;
+  rc = lbc_load_states(c, 0, 0, 1, fin);
+  if (!rc) rc = lbc_run_interval(c, nullptr);
+  if (!rc) rc = store_which(c, 0, 0, 0, 1, fout);
+  if (!rc) rc = store_which(c, 0, 1, 0, 1, fin);
+  if (!rc && force) rc = lbc_get_drag(c, 0, force);
+  if (rc) g_err = c->err;
+  lbc_destroy(c);
+  return rc;
+}
+
+}  // extern "C"

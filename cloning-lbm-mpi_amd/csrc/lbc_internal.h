@@ -1,0 +1,102 @@
+// Internal definitions shared by the HIP kernels and the host side of the
+// C ABI.  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbc {
+
+// Instance descriptor as the kernels see it (device memory, one per
+// instance).  Populations of clone c, direction q, site (x,y) live at
+// buf[par][c*clone_stride + q*plane + y*Lx + x]   (SoA [clone][q][y][x]).
+struct InstDev {
+  void* buf[2];            // ping-pong population buffers
+  long long clone_stride;  // elements per clone = 9*plane
+  int plane;               // elements per direction plane = Lx*Ly
+  int Lx, Ly, xmin, xmax, ymin, ymax;
+  int n_clones, steps;
+  int ring_n;              // drag slots: 2*(sx+1) + 2*(sy+1)
+  int n_snapshots;
+  int rank_blocks;
+  int clone_base;          // first global clone index of this instance
+  double tau, beta, u_in;
+  double scale, shift, dt, bias;
+  double eps, sigma;
+  double feq_in[9];        // inlet ghost populations feq(1,(u_in,0)), FP64
+  float feq_in_s[9];       // the same minus w_q, FP32
+  double* drag;            // [n_clones][steps]
+  double* obs;             // [n_clones]  dt*sum A
+  double* logw;            // [n_clones]
+  double* weights;         // [n_clones]  selection input
+  double* u;               // [n_clones]  floor-step uniforms
+  int32_t* rint;           // [n_clones]  prune/add draws
+  int32_t* nb_before;      // [n_clones]
+  int32_t* nb_after;       // [n_clones]
+  int32_t* pairs;          // [2*n_clones]
+  int32_t* labels;         // [n_clones]  parent of each slot
+  int32_t* sel_info;       // [4]: n_pairs, n_rint_used, status, spare
+  double* sel_R;           // [2]: total_R, log total_R
+  double* R_hist;          // [max_intervals]
+  uint8_t* marks;          // [n_clones]
+  double* prefactors;      // [n_clones][n_snapshots]
+  double* snapshots;       // [n_snapshots][9*plane] FP64 SoA
+  double* norms;           // [n_clones][2] REF perturbation norms
+  int max_intervals;
+  int pad_;
+};
+
+// Device-resident counters shared by all instances.
+struct Counters {
+  int parity[64];          // which buffer holds the post-stream state
+  int interval[64];        // cloning steps completed per instance
+};
+
+constexpr int kMaxInstances = 64;
+
+// Work item of the step kernel: one clone of one instance.
+struct WorkItem {
+  int inst;
+  int clone;
+};
+
+enum Storage { kF64 = 0, kF32 = 1, kF16 = 2, kBF16 = 3 };
+
+struct Launch {
+  const InstDev* insts;
+  const WorkItem* items;
+  Counters* counters;
+  int n_items;
+  int threads;
+  int ring_max;
+  int mode;
+  int storage;
+  int vec;           // sites per thread group (1 or 2)
+  hipStream_t stream;
+};
+
+// Kernel launchers (lbc_kernels.hip).
+hipError_t launch_interval(const Launch& L);
+hipError_t launch_flip(const Launch& L, int n_inst, const int* flip_mask_dev);
+hipError_t launch_perturb(const Launch& L, int philox_prefactors,
+                          uint32_t seed_lo, uint32_t seed_hi);
+hipError_t launch_select(const InstDev* insts, Counters* counters, int inst0,
+                         int n_inst, int n_max, int weights_from_host, int draws_from_host,
+                         uint32_t seed_lo, uint32_t seed_hi,
+                         hipStream_t stream);
+hipError_t launch_gather(const Launch& L, uint32_t seed_lo, uint32_t seed_hi);
+hipError_t launch_init_eq(const Launch& L, int inst, int n_clones, double amp,
+                          uint32_t seed_lo, uint32_t seed_hi);
+hipError_t launch_aos_to_soa(int storage, const InstDev* inst_dev,
+                             const Counters* counters, int inst, int which,
+                             int clone0, int n, const double* aos_dev,
+                             hipStream_t stream);
+hipError_t launch_soa_to_aos(int storage, const InstDev* inst_dev,
+                             const Counters* counters, int inst, int which,
+                             int clone0, int n, double* aos_dev,
+                             hipStream_t stream);
+hipError_t launch_snap_to_soa(const InstDev* inst_dev, int inst, int n,
+                              const double* aos_dev, int Lx, int Ly,
+                              double* snaps_dev, hipStream_t stream);
+
+}  // namespace lbc

@@ -1,0 +1,1197 @@
+// HIP kernels of the cloning-sampler inner loop for gfx950 (MI355X).
+//
+//  k_interval  one persistent workgroup per clone runs a whole cloning
+//              interval: pull-scheme stream + BGK collide in SoA
+//              [clone][q][y][x] layout, bounce-back walls and square, the
+//              mode's x boundaries, fused per-clone drag reduced in LDS in a
+//              fixed order, FP64 time integration of the drag.
+//  k_select    one workgroup per instance: weights, mean weight, floor
+//              step, prune/add, copy table by two prefix sums.
+//  k_gather    copies selected parents into their destination slots, with
+//              Philox Gaussian population noise (channel mode).
+//  k_perturb   snapshot-mix perturbation of marked clones (reference mode).
+//  k_init_eq, k_aos_to_soa, k_soa_to_aos, k_snap_to_soa: state set-up and
+//              exchange with the reference's AoS FP64 layout.
+//
+// FP64 storage reproduces the reference's arithmetic order exactly
+// (floating-point contraction is off in those paths), so its lattice fields
+// and drag match the reference bit for bit.  Shifted storages hold f - w_q
+// and collide in FP32.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lbc_internal.h"
+
+namespace lbc {
+namespace {
+
+// D2Q9 velocity set and weights in the reference's ordering
+// (src/mpi_LBM_GK_pops_perturb.cpp:31-32).
+constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+constexpr double WD[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,
+                          1.0 / 9.0,  1.0 / 9.0,  1.0 / 36.0,
+                          1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+constexpr float WF[9] = {4.0f / 9.0f,  1.0f / 9.0f,  1.0f / 9.0f,
+                         1.0f / 9.0f,  1.0f / 9.0f,  1.0f / 36.0f,
+                         1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f};
+
+constexpr int kModeRef = 0;
+constexpr int kModeChannel = 1;
+
+enum Phase { kCollide = 0, kBoth = 1, kStream = 2 };
+
+// ---------------------------------------------------------------- storage
+template <int ST>
+struct Traits;
+
+template <>
+struct Traits<kF64> {
+  using S = double;
+  using C = double;
+  static constexpr bool shifted = false;
+  __device__ static C ld(const S* p) { return *p; }
+  __device__ static void st(S* p, C v) { *p = v; }
+  __device__ static C rest(int q) { return WD[q]; }
+};
+
+template <>
+struct Traits<kF32> {
+  using S = float;
+  using C = float;
+  static constexpr bool shifted = true;
+  __device__ static C ld(const S* p) { return *p; }
+  __device__ static void st(S* p, C v) { *p = v; }
+  __device__ static C rest(int) { return 0.0f; }
+};
+
+template <>
+struct Traits<kF16> {
+  using S = _Float16;
+  using C = float;
+  static constexpr bool shifted = true;
+  __device__ static C ld(const S* p) { return (float)*p; }
+  __device__ static void st(S* p, C v) { *p = (_Float16)v; }
+  __device__ static C rest(int) { return 0.0f; }
+};
+
+// bfloat16 bits, round-to-nearest-even on store.
+__device__ inline uint16_t f32_to_bf16(float v) {
+  uint32_t b = __float_as_uint(v);
+  if ((b & 0x7f800000u) == 0x7f800000u) return (uint16_t)((b >> 16) | ((b & 0xffffu) ? 0x40u : 0u));
+  b += 0x7fffu + ((b >> 16) & 1u);
+  return (uint16_t)(b >> 16);
+}
+
+template <>
+struct Traits<kBF16> {
+  using S = uint16_t;
+  using C = float;
+  static constexpr bool shifted = true;
+  __device__ static C ld(const S* p) { return __uint_as_float((uint32_t)*p << 16); }
+  __device__ static void st(S* p, C v) { *p = f32_to_bf16(v); }
+  __device__ static C rest(int) { return 0.0f; }
+};
+
+// ---------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon et al., SC'11).
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ inline U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                            uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+  return U4{c0, c1, c2, c3};
+}
+
+// 53-bit uniform in [0,1) from two words.
+__device__ inline double u53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) *
+         (1.0 / 9007199254740992.0);
+}
+
+__device__ inline void box_muller_d(U4 r, double n[4]) {
+  const double two32inv = 2.3283064365386963e-10;
+  const double twopi = 6.283185307179586;
+  const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    double u1 = ((double)rr[2 * h] + 1.0) * two32inv;
+    double u2 = (double)rr[2 * h + 1] * two32inv;
+    double rad = sqrt(-2.0 * log(u1));
+    double ang = twopi * u2;
+    n[2 * h] = rad * cos(ang);
+    n[2 * h + 1] = rad * sin(ang);
+  }
+}
+
+__device__ inline void box_muller_f(U4 r, float n[4]) {
+  const float two32inv = 2.3283064365386963e-10f;
+  const float twopi = 6.2831853071795865f;
+  const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    float u1 = ((float)(rr[2 * h] >> 8) + 1.0f) * (two32inv * 256.0f);
+    float u2 = (float)(rr[2 * h + 1] >> 8) * (two32inv * 256.0f);
+    float rad = sqrtf(-2.0f * logf(u1));
+    float s, c;
+    sincosf(twopi * u2, &s, &c);
+    n[2 * h] = rad * c;
+    n[2 * h + 1] = rad * s;
+  }
+}
+
+// ---------------------------------------------------------------- physics
+// BGK collision with Guo forcing in the reference's operation order
+// (src/lbm_src/streamCollCompute.cpp:34-52), contraction off.
+__device__ inline void collide_f64(double (&f)[9], double beta, double tau) {
+#pragma clang fp contract(off)
+  const double omega = 1. / tau;
+  const double omega1 = 1.0 - omega;
+  const double cforce = 1.0 - 0.5 * omega;
+  double rho = 0.0, ux = 0.0, uy = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const double ft = f[k];
+    rho += ft;
+    ux += ft * (double)CX[k];
+    uy += ft * (double)CY[k];
+  }
+  ux = ux / rho + 0.5 * beta / rho;
+  uy /= rho;
+  const double u2 = -1.5 * (ux * ux + uy * uy);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const double eu = (double)CX[k] * ux + (double)CY[k] * uy;
+    const double eueu = 4.5 * eu * eu;
+    const double feq = WD[k] * rho * (1.0 + 3.0 * eu + eueu + u2);
+    const double fd = WD[k] * cforce * beta *
+                      (3. * ((double)CX[k] - ux) + 9. * (double)CX[k] * eu);
+    f[k] = f[k] * omega1 + feq * omega + fd;
+  }
+}
+
+// Same collision on shifted populations d = f - w, FP32.
+template <int MODE>
+__device__ inline void collide_shifted(float (&d)[9], float omega, float beta) {
+#pragma clang fp contract(fast)
+  const float drho = ((d[0] + d[1]) + (d[2] + d[3])) +
+                     ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
+  const float jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
+  const float jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
+  const float rho = 1.0f + drho;
+  const float inv = __builtin_amdgcn_rcpf(rho);
+  float ux = jx * inv;
+  if (MODE == kModeRef) ux += 0.5f * beta * inv;
+  const float uy = jy * inv;
+  const float usq = 1.5f * (ux * ux + uy * uy);
+  const float omega1 = 1.0f - omega;
+  const float cforce = 1.0f - 0.5f * omega;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const float eu = (float)CX[k] * ux + (float)CY[k] * uy;
+    const float deq = WF[k] * (drho + rho * (3.0f * eu + 4.5f * eu * eu - usq));
+    float v = omega1 * d[k] + omega * deq;
+    if (MODE == kModeRef)
+      v += WF[k] * cforce * beta *
+           (3.0f * ((float)CX[k] - ux) + 9.0f * (float)CX[k] * eu);
+    d[k] = v;
+  }
+}
+
+// Moments and non-equilibrium stress of one node in the order of
+// src/lbm_src/force.cpp:26-46 (FP64, contraction off).
+__device__ inline void node_stress_f64(const double (&f)[9], double& rho_o,
+                                       double& pxx_o, double& pxy_o) {
+#pragma clang fp contract(off)
+  double rho = 0.0, ux = 0.0, uy = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    rho += f[k];
+    ux += f[k] * (double)CX[k];
+    uy += f[k] * (double)CY[k];
+  }
+  uy /= rho;
+  ux /= rho;
+  double pxx = 0.0, pxy = 0.0;
+  const double u2 = -1.5 * (ux * ux + uy * uy);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const double eu = (double)CX[k] * ux + (double)CY[k] * uy;
+    const double eueu = 4.5 * eu * eu;
+    const double feq = WD[k] * rho * (1.0 + 3.0 * eu + eueu + u2);
+    const double fneq = f[k] - feq;
+    pxx += fneq * (double)CX[k] * (double)CX[k];
+    pxy += fneq * (double)CX[k] * (double)CY[k];
+  }
+  rho_o = rho;
+  pxx_o = pxx;
+  pxy_o = pxy;
+}
+
+// Shifted FP32 version: returns drho and the non-equilibrium stresses.
+__device__ inline void node_stress_shifted(const float (&d)[9], float& drho_o,
+                                           float& pxx_o, float& pxy_o) {
+  const float drho = ((d[0] + d[1]) + (d[2] + d[3])) +
+                     ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
+  const float jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
+  const float jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
+  const float rho = 1.0f + drho;
+  const float inv = 1.0f / rho;
+  const float ux = jx * inv, uy = jy * inv;
+  const float usq = 1.5f * (ux * ux + uy * uy);
+  float pxx = 0.0f, pxy = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const float eu = (float)CX[k] * ux + (float)CY[k] * uy;
+    const float deq = WF[k] * (drho + rho * (3.0f * eu + 4.5f * eu * eu - usq));
+    const float ne = d[k] - deq;
+    pxx += ne * (float)(CX[k] * CX[k]);
+    pxy += ne * (float)(CX[k] * CY[k]);
+  }
+  drho_o = drho;
+  pxx_o = pxx;
+  pxy_o = pxy;
+}
+
+// Bounce-back directions of the square ring at (x,y) (the writes of
+// computeSquareBounceBack_TEST, src/lbm_src/square.cpp:24-50).
+__device__ inline int square_mask(int x, int y, int x0, int x1, int y0, int y1) {
+  if (x < x0 || x > x1 || y < y0 || y > y1) return 0;
+  const bool inx = (x > x0 && x < x1), iny = (y > y0 && y < y1);
+  if (inx && iny) return 0;
+  if (y == y0 && inx) return (1 << 4) | (1 << 7) | (1 << 8);
+  if (y == y1 && inx) return (1 << 2) | (1 << 5) | (1 << 6);
+  if (x == x0 && iny) return (1 << 3) | (1 << 6) | (1 << 7);
+  if (x == x1 && iny) return (1 << 1) | (1 << 5) | (1 << 8);
+  if (x == x0) return (y == y0) ? (1 << 7) : (1 << 6);
+  return (y == y1) ? (1 << 5) : (1 << 8);
+}
+
+// Wall bounce-back directions (src/lbm_src/domain_noSlipWalls.cpp:10-21).
+__device__ inline int wall_mask(int y, int Ly) {
+  int m = 0;
+  if (y == 0) m |= (1 << 2) | (1 << 5) | (1 << 6);
+  if (y == Ly - 1) m |= (1 << 4) | (1 << 7) | (1 << 8);
+  return m;
+}
+
+// Per-instance scalars held in registers by the step kernel.
+struct Geo {
+  int Lx, Ly, x0, x1, y0, y1, plane;
+  int sx, sy;
+};
+
+// One site of one pass.  src/dst point at this clone's populations.
+template <int MODE, int ST, int PH>
+__device__ inline void do_site(const Geo& g, int x, int y,
+                               const typename Traits<ST>::S* __restrict__ src,
+                               typename Traits<ST>::S* __restrict__ dst,
+                               double* __restrict__ ring, double beta,
+                               double tau, const InstDev* __restrict__ I) {
+  using T = Traits<ST>;
+  using C = typename T::C;
+  const int site = y * g.Lx + x;
+  const bool in_box = (x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1);
+  if (PH != kCollide && in_box && x > g.x0 && x < g.x1 && y > g.y0 &&
+      y < g.y1) {
+    // solid interior: rest state (the reference resets it to w every step;
+    // its post-collision values never reach a fluid node).  The collide-only
+    // pass collides it like the reference does.
+#pragma unroll
+    for (int q = 0; q < 9; q++) T::st(dst + q * g.plane + site, T::rest(q));
+    return;
+  }
+  C f[9];
+  if (PH == kCollide) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) f[q] = T::ld(src + q * g.plane + site);
+  } else {
+    const bool plain = !in_box && y >= 1 && y <= g.Ly - 2 && x >= 1 &&
+                       x <= g.Lx - 2;
+    if (plain) {
+#pragma unroll
+      for (int q = 0; q < 9; q++)
+        f[q] = T::ld(src + q * g.plane + site - CY[q] * g.Lx - CX[q]);
+    } else {
+      const int m = wall_mask(y, g.Ly) | square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
+#pragma unroll
+      for (int q = 0; q < 9; q++) {
+        if ((m >> q) & 1) {
+          f[q] = T::ld(src + OPP[q] * g.plane + site);
+          continue;
+        }
+        int xs = x - CX[q];
+        const int ys = y - CY[q];
+        if (MODE == kModeRef) {
+          if (xs < 0) xs += g.Lx;
+          else if (xs >= g.Lx) xs -= g.Lx;
+        } else {
+          if (xs < 0) {
+            if (T::shifted) f[q] = (C)I->feq_in_s[q];
+            else f[q] = (C)I->feq_in[q];
+            continue;
+          }
+          if (xs >= g.Lx) xs = g.Lx - 1;
+        }
+        f[q] = T::ld(src + q * g.plane + ys * g.Lx + xs);
+      }
+      if (in_box) {
+        // drag contribution of this ring node
+        if (MODE == kModeRef) {
+          double tW, tE, tN, tS;
+          if (!T::shifted) {
+#pragma clang fp contract(off)
+            double rho, pxx, pxy;
+            double ff[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) ff[q] = (double)f[q];
+            node_stress_f64(ff, rho, pxx, pxy);
+            const double ot = 1. / 3.;
+            const double cf = 1.0 - .5 * (1. / tau);
+            tW = rho * ot + cf * pxx;
+            tE = -rho * ot - cf * pxx;
+            tN = -cf * pxy;
+            tS = +cf * pxy;
+          } else {
+            float dr, pxx, pxy;
+            float dd[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) dd[q] = (float)f[q];
+            node_stress_shifted(dd, dr, pxx, pxy);
+            const float cf = 1.0f - 0.5f / (float)tau;
+            // the constant 1/3 of each node cancels between the west and
+            // east faces, which have the same number of nodes
+            tW = (double)(dr * (1.0f / 3.0f) + cf * pxx);
+            tE = -tW;
+            tN = (double)(-cf * pxy);
+            tS = -tN;
+          }
+          if (x == g.x0) ring[y - g.y0] = tW;
+          if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = tE;
+          if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = tN;
+          if (y == g.y0) ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = tS;
+        } else {
+          const int sm = square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
+          double t = 0.0;
+          if (!T::shifted) {
+#pragma clang fp contract(off)
+#pragma unroll
+            for (int q = 0; q < 9; q++)
+              if ((sm >> q) & 1) t += (double)(-2 * CX[q]) * (double)f[q];
+          } else {
+            float tf = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 9; q++)
+              if ((sm >> q) & 1) tf += (float)(-2 * CX[q]) * (float)f[q];
+            t = (double)tf;
+          }
+          if (x == g.x0) ring[y - g.y0] = t;
+          else if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = t;
+          else if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = t;
+          else ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = t;
+        }
+      }
+    }
+  }
+  if (PH != kStream) {
+    if (!T::shifted) {
+      double ff[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) ff[q] = (double)f[q];
+      collide_f64(ff, MODE == kModeRef ? beta : 0.0, tau);
+#pragma unroll
+      for (int q = 0; q < 9; q++) f[q] = (C)ff[q];
+    } else {
+      float dd[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) dd[q] = (float)f[q];
+      collide_shifted<MODE>(dd, (float)(1.0 / tau), (float)beta);
+#pragma unroll
+      for (int q = 0; q < 9; q++) f[q] = (C)dd[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; q++) T::st(dst + q * g.plane + site, f[q]);
+}
+
+template <int MODE, int ST, int PH>
+__device__ inline void site_pass(const Geo& g, int x, int y, int dx, int dy,
+                                 const typename Traits<ST>::S* __restrict__ src,
+                                 typename Traits<ST>::S* __restrict__ dst,
+                                 double* __restrict__ ring, double beta,
+                                 double tau, const InstDev* __restrict__ I) {
+  while (y < g.Ly) {
+    do_site<MODE, ST, PH>(g, x, y, src, dst, ring, beta, tau, I);
+    x += dx;
+    y += dy;
+    if (x >= g.Lx) {
+      x -= g.Lx;
+      y += 1;
+    }
+  }
+}
+
+// Sum of the drag slots in the reference's order: west face (y ascending),
+// east face, north face (x ascending), south face
+// (src/lbm_src/force.cpp:23,54,84,114,141).
+template <int MODE>
+__device__ inline double ring_sum(const double* ring, int sx, int sy) {
+#pragma clang fp contract(off)
+  double fW = 0.0, fE = 0.0, fN = 0.0, fS = 0.0;
+  for (int i = 0; i <= sy; i++) fW += ring[i];
+  for (int i = 0; i <= sy; i++) fE += ring[(sy + 1) + i];
+  const int lo = (MODE == kModeRef) ? 0 : 1;
+  const int hi = (MODE == kModeRef) ? sx : sx - 1;
+  for (int i = lo; i <= hi; i++) fN += ring[2 * (sy + 1) + i];
+  for (int i = lo; i <= hi; i++) fS += ring[2 * (sy + 1) + (sx + 1) + i];
+  return fW + fE + fN + fS;
+}
+
+// One cloning interval of one clone per workgroup.  The post-stream state
+// S_0 is in buf[parity]; pass 0 collides it, passes 1..M-1 stream and
+// collide, pass M streams only, so the interval ends with S_M in
+// buf[parity ^ ((M+1)&1)].  Drag F_n is taken on S_n in pass n.
+template <int MODE, int ST>
+__global__ void __launch_bounds__(1024)
+    k_interval(const InstDev* __restrict__ insts,
+               const WorkItem* __restrict__ items,
+               const Counters* __restrict__ counters, int ring_max) {
+  using S = typename Traits<ST>::S;
+  extern __shared__ double ring_lds[];
+  const WorkItem wi = items[blockIdx.x];
+  const InstDev* __restrict__ I = insts + wi.inst;
+  Geo g;
+  g.Lx = I->Lx;
+  g.Ly = I->Ly;
+  g.x0 = I->xmin;
+  g.x1 = I->xmax;
+  g.y0 = I->ymin;
+  g.y1 = I->ymax;
+  g.plane = I->plane;
+  g.sx = g.x1 - g.x0;
+  g.sy = g.y1 - g.y0;
+  const int M = I->steps;
+  const double beta = I->beta, tau = I->tau;
+  const double scale = I->scale, shift = I->shift;
+  const int par = counters->parity[wi.inst];
+  S* A = (S*)I->buf[par] + (long long)wi.clone * I->clone_stride;
+  S* B = (S*)I->buf[par ^ 1] + (long long)wi.clone * I->clone_stride;
+  double* drag = I->drag + (long long)wi.clone * M;
+
+  const int nthr = blockDim.x, tid = threadIdx.x;
+  const int dx = nthr % g.Lx, dy = nthr / g.Lx;
+  const int x_init = tid % g.Lx, y_init = tid / g.Lx;
+  double acc = 0.0;
+
+  for (int n = 0; n <= M; ++n) {
+    const S* src = (n & 1) ? B : A;
+    S* dst = (n & 1) ? A : B;
+    double* ring = ring_lds + (n & 1) * ring_max;
+    if (n == 0)
+      site_pass<MODE, ST, kCollide>(g, x_init, y_init, dx, dy, src, dst, ring,
+                                    beta, tau, I);
+    else if (n == M)
+      site_pass<MODE, ST, kStream>(g, x_init, y_init, dx, dy, src, dst, ring,
+                                   beta, tau, I);
+    else
+      site_pass<MODE, ST, kBoth>(g, x_init, y_init, dx, dy, src, dst, ring,
+                                 beta, tau, I);
+    __syncthreads();
+    if (n >= 1 && tid == 0) {
+#pragma clang fp contract(off)
+      double F = ring_sum<MODE>(ring, g.sx, g.sy);
+      F = F * scale;
+      F = F - shift;
+      drag[n - 1] = F;
+      acc += F;
+    }
+  }
+  if (tid == 0) {
+#pragma clang fp contract(off)
+    const double o = acc * I->dt;
+    I->obs[wi.clone] = o;
+    I->logw[wi.clone] = I->bias * o;
+  }
+}
+
+__global__ void k_flip(const InstDev* __restrict__ insts, Counters* counters,
+                       int n_inst, const int* __restrict__ flip_mask) {
+  const int i = threadIdx.x;
+  if (i < n_inst && flip_mask[i]) counters->parity[i] ^= 1;
+}
+
+// ---------------------------------------------------------------- selection
+// Exclusive scan of v[0..n) in LDS by the whole block; returns the total.
+// part must hold blockDim.x ints.
+__device__ int block_scan_excl(int* v, int n, int* part) {
+  const int nt = blockDim.x, t = threadIdx.x;
+  const int chunk = (n + nt - 1) / nt;
+  const int b = t * chunk, e = min(n, b + chunk);
+  int s = 0;
+  for (int i = b; i < e; i++) s += v[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {
+    int add = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
+  }
+  const int total = part[nt - 1];
+  int run = part[t] - s;
+  for (int i = b; i < e; i++) {
+    const int x = v[i];
+    v[i] = run;
+    run += x;
+  }
+  __syncthreads();
+  return total;
+}
+
+// Selection of the cloning step (src/mpi_LBM_GK_pops_perturb.cpp:489-632).
+// LDS: nb[N], temp[2N], work[N], part[blockDim].
+__global__ void __launch_bounds__(1024)
+    k_select(const InstDev* __restrict__ insts, Counters* counters, int inst0,
+             int weights_from_host, int draws_from_host, uint32_t seed_lo,
+             uint32_t seed_hi) {
+  extern __shared__ int sm[];
+  const int inst = inst0 + blockIdx.x;
+  const InstDev* __restrict__ I = insts + inst;
+  const int N = I->n_clones;
+  const int t = threadIdx.x, nt = blockDim.x;
+  int* nb = sm;
+  int* temp = nb + N;
+  int* work = temp + 2 * N;
+  int* part = work + N;
+  __shared__ double sh_total, sh_shift;
+  __shared__ int sh_bad, sh_used;
+  const int interval = counters->interval[inst];
+
+  // weights and draws
+  if (t == 0) sh_bad = 0;
+  __syncthreads();
+  double lmax = -1e308, lmin = 1e308;
+  if (!weights_from_host) {
+    for (int j = t; j < N; j += nt) {
+      const double l = I->logw[j];
+      lmax = fmax(lmax, l);
+      lmin = fmin(lmin, l);
+    }
+  }
+  // block max/min of log-weights through LDS (reuse part as doubles)
+  double* pd = (double*)(part);
+  __syncthreads();
+  if (!weights_from_host) {
+    if (t == 0) sh_shift = -1e308;
+    __syncthreads();
+    // wave-level reduction then one thread folds the waves
+    for (int off = 32; off > 0; off >>= 1) {
+      lmax = fmax(lmax, __shfl_down(lmax, off, 64));
+      lmin = fmin(lmin, __shfl_down(lmin, off, 64));
+    }
+    if ((t & 63) == 0) {
+      pd[2 * (t >> 6)] = lmax;
+      pd[2 * (t >> 6) + 1] = lmin;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double mx = -1e308, mn = 1e308;
+      for (int w = 0; w < (nt + 63) / 64; w++) {
+        mx = fmax(mx, pd[2 * w]);
+        mn = fmin(mn, pd[2 * w + 1]);
+      }
+      // keep the reference's arithmetic (shift 0) unless exp() would
+      // overflow or underflow
+      sh_shift = (mx < 700.0 && mn > -700.0) ? 0.0 : mx;
+      if (!(mx == mx) || !(mn == mn) || mx > 1e300 || mn < -1e300) sh_bad = 1;
+    }
+    __syncthreads();
+    const double sh = sh_shift;
+    for (int j = t; j < N; j += nt) I->weights[j] = exp(I->logw[j] - sh);
+  } else if (t == 0) {
+    sh_shift = 0.0;
+  }
+  if (!draws_from_host) {
+    for (int j = t; j < N; j += nt) {
+      const U4 r = philox((uint32_t)j, (uint32_t)interval, (uint32_t)inst,
+                          0x5E1EC7u, seed_lo, seed_hi);
+      I->u[j] = u53(r.x, r.y);
+      I->rint[j] = (int32_t)(r.z >> 1);
+    }
+  }
+  __syncthreads();
+
+  // total_R: sequential, in rank blocks (src/mpi_LBM_GK_pops_perturb.cpp:489-497,512)
+  if (t == 0) {
+#pragma clang fp contract(off)
+    const int p = I->rank_blocks > 0 ? I->rank_blocks : 1;
+    const int local = N / p;
+    double total = 0.0;
+    for (int r = 0; r < p; r++) {
+      double R = 0.0;
+      for (int j = r * local; j < (r + 1) * local; j++) R += I->weights[j];
+      if (r == 0) total = R;
+      else total += R;
+    }
+    total /= N;
+    sh_total = total;
+    if (!(total > 0.0) || !(total < 1e308)) sh_bad = 1;
+  }
+  __syncthreads();
+  const double total = sh_total;
+
+  // floor step (:514-520)
+  for (int j = t; j < N; j += nt) {
+    const double v = floor(I->weights[j] / total + I->u[j]);
+    int c = 0;
+    if (v >= 0.0 && v <= 2.0 * N) c = (int)v;
+    else sh_bad = 1;
+    nb[j] = c;
+    I->nb_before[j] = c;
+    work[j] = c;
+  }
+  __syncthreads();
+  // multiset of parents (:524-532)
+  const int ncp = sh_bad ? 0 : block_scan_excl(work, N, part);
+  if (sh_bad || ncp < 1 || ncp > 2 * N) {
+    // non-finite or inconsistent weights: keep every clone, report the error
+    for (int j = t; j < N; j += nt) {
+      I->nb_after[j] = 1;
+      I->labels[j] = j;
+      I->marks[j] = 0;
+    }
+    if (t == 0) {
+      I->sel_info[0] = 0;
+      I->sel_info[1] = 0;
+      I->sel_info[2] = -5;
+      I->sel_R[0] = total;
+      I->sel_R[1] = log(total) + sh_shift;
+    }
+    return;
+  }
+  for (int j = t; j < N; j += nt) {
+    const int o = work[j];
+    for (int i = 0; i < nb[j]; i++) temp[o + i] = j;
+  }
+  __syncthreads();
+  // prune / add (:540-571), sequential by construction
+  if (t == 0) {
+    const int dN = ncp - N;
+    int used = 0;
+    if (dN > 0) {
+      for (int i = 0; i < dN; i++) {
+        const int idx = I->rint[used++] % (ncp - i);
+        const int c = temp[idx];
+        temp[idx] = temp[ncp - i - 1];
+        nb[c]--;
+      }
+    } else if (dN < 0) {
+      for (int i = 0; i < -dN; i++) {
+        const int idx = I->rint[used++] % ncp;
+        const int c = temp[idx];
+        temp[ncp + i] = c;
+        nb[c]++;
+      }
+    }
+    sh_used = used;
+  }
+  __syncthreads();
+  // copy table (:590-619): the j-th extra copy (parents ascending) goes to
+  // the j-th empty slot (ascending)
+  for (int j = t; j < N; j += nt) {
+    I->nb_after[j] = nb[j];
+    work[j] = nb[j] > 1 ? nb[j] - 1 : 0;
+    temp[j] = nb[j] == 0 ? 1 : 0;
+  }
+  __syncthreads();
+  const int n_pairs = block_scan_excl(work, N, part);
+  block_scan_excl(temp, N, part);
+  // empty-slot list into temp[N..2N)
+  for (int j = t; j < N; j += nt)
+    if (nb[j] == 0) temp[N + temp[j]] = j;
+  __syncthreads();
+  for (int j = t; j < N; j += nt) I->labels[j] = j;
+  __syncthreads();
+  for (int j = t; j < N; j += nt) {
+    const int e = nb[j] > 1 ? nb[j] - 1 : 0;
+    for (int i = 0; i < e; i++) {
+      const int pidx = work[j] + i;
+      const int dst = temp[N + pidx];
+      I->pairs[2 * pidx] = j;
+      I->pairs[2 * pidx + 1] = dst;
+      I->labels[dst] = j;
+    }
+  }
+  __syncthreads();
+  for (int j = t; j < N; j += nt) I->marks[j] = (I->labels[j] != j) ? 1 : 0;
+  if (t == 0) {
+    const double lr = log(total) + sh_shift;
+    I->sel_info[0] = n_pairs;
+    I->sel_info[1] = sh_used;
+    I->sel_info[2] = 0;
+    I->sel_R[0] = sh_shift == 0.0 ? total : exp(lr);
+    I->sel_R[1] = lr;
+    if (interval < I->max_intervals) {
+      I->R_hist[2 * interval] = I->sel_R[0];
+      I->R_hist[2 * interval + 1] = lr;
+    }
+    counters->interval[inst] = interval + 1;
+  }
+}
+
+// ---------------------------------------------------------------- gather
+// Destination slot k <- parent labels[k], site by site, with Gaussian
+// noise on the copy in channel mode.  The interval counter has already been
+// advanced by the selection.
+template <int MODE, int ST>
+__global__ void __launch_bounds__(256)
+    k_gather(const InstDev* __restrict__ insts,
+             const WorkItem* __restrict__ items,
+             const Counters* __restrict__ counters, uint32_t seed_lo,
+             uint32_t seed_hi) {
+  using T = Traits<ST>;
+  using S = typename T::S;
+  const WorkItem wi = items[blockIdx.x];
+  const InstDev* __restrict__ I = insts + wi.inst;
+  const int parent = I->labels[wi.clone];
+  if (parent == wi.clone) return;
+  const int par = counters->parity[wi.inst];
+  const int interval = counters->interval[wi.inst];
+  const S* src = (const S*)I->buf[par] + (long long)parent * I->clone_stride;
+  S* dst = (S*)I->buf[par] + (long long)wi.clone * I->clone_stride;
+  const int Lx = I->Lx, plane = I->plane;
+  const double sigma = I->sigma;
+  const bool noisy = (MODE == kModeChannel) && sigma > 0.0;
+  for (int s = threadIdx.x; s < plane; s += blockDim.x) {
+    typename T::C f[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) f[q] = T::ld(src + q * plane + s);
+    if (noisy) {
+      const int x = s % Lx, y = s / Lx;
+      const bool interior = x > I->xmin && x < I->xmax && y > I->ymin && y < I->ymax;
+      if (!interior) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          const U4 r = philox(3u * (uint32_t)s + (uint32_t)j, (uint32_t)wi.clone,
+                              (uint32_t)interval, (uint32_t)wi.inst, seed_lo,
+                              seed_hi);
+          if (!T::shifted) {
+            double nrm[4];
+            box_muller_d(r, nrm);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+              if (4 * j + i < 9) f[4 * j + i] += sigma * nrm[i];
+          } else {
+            float nrm[4];
+            box_muller_f(r, nrm);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+              if (4 * j + i < 9) f[4 * j + i] += (float)sigma * nrm[i];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) T::st(dst + q * plane + s, f[q]);
+  }
+}
+
+// ---------------------------------------------------------------- perturb
+// Snapshot-mix perturbation of marked clones
+// (src/perturbation_routines/generatePerturbedState.cpp:7-36):
+// f' = sum_n a_n P_n, r = |f|/|f'|, f <- (f + eps*r*f') / (1 + eps*r*sum a).
+template <int ST>
+__global__ void __launch_bounds__(256)
+    k_perturb(const InstDev* __restrict__ insts,
+              const WorkItem* __restrict__ items,
+              const Counters* __restrict__ counters, int philox_prefactors,
+              uint32_t seed_lo, uint32_t seed_hi) {
+  using T = Traits<ST>;
+  using S = typename T::S;
+  const WorkItem wi = items[blockIdx.x];
+  const InstDev* __restrict__ I = insts + wi.inst;
+  if (!I->marks[wi.clone]) return;
+  const int NN = I->n_snapshots;
+  const int plane = I->plane;
+  const int par = counters->parity[wi.inst];
+  S* f = (S*)I->buf[par] + (long long)wi.clone * I->clone_stride;
+  __shared__ double a[64];
+  __shared__ double red[2][256];
+  __shared__ double sh_w, sh_inv, sh_eps;
+  const int t = threadIdx.x;
+  if (t < NN) {
+    if (philox_prefactors) {
+      const int interval = counters->interval[wi.inst];
+      const U4 r = philox((uint32_t)t, (uint32_t)wi.clone, (uint32_t)interval,
+                          (uint32_t)wi.inst | 0xB0000000u, seed_lo, seed_hi);
+      a[t] = u53(r.x, r.y);
+    } else {
+      a[t] = I->prefactors[(long long)wi.clone * NN + t];
+    }
+  }
+  __syncthreads();
+  double n0 = 0.0, np = 0.0;
+  const int total = 9 * plane;
+  for (int i = t; i < total; i += blockDim.x) {
+    const int q = i / plane;
+    double v = (double)T::ld(f + i);
+    if (T::shifted) v += WD[q];
+    double fp = 0.0;
+    for (int nn = 0; nn < NN; nn++)
+      fp += a[nn] * I->snapshots[(long long)nn * total + i];
+    n0 += v * v;
+    np += fp * fp;
+  }
+  red[0][t] = n0;
+  red[1][t] = np;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (t < off) {
+      red[0][t] += red[0][t + off];
+      red[1][t] += red[1][t + off];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma clang fp contract(off)
+    double sum = 0.0;
+    for (int nn = 0; nn < NN; nn++) sum += a[nn];
+    const double eps = I->eps;
+    const double weight = sqrt(red[0][0]) / sqrt(red[1][0]);
+    const double A = 1.0 + sum * eps * weight;
+    sh_w = weight;
+    sh_inv = 1. / A;
+    sh_eps = eps;
+    I->norms[2 * wi.clone] = red[0][0];
+    I->norms[2 * wi.clone + 1] = red[1][0];
+  }
+  __syncthreads();
+  const double weight = sh_w, inv = sh_inv, eps = sh_eps;
+  for (int i = t; i < total; i += blockDim.x) {
+#pragma clang fp contract(off)
+    const int q = i / plane;
+    double v = (double)T::ld(f + i);
+    if (T::shifted) v += WD[q];
+    double fp = 0.0;
+    for (int nn = 0; nn < NN; nn++)
+      fp += a[nn] * I->snapshots[(long long)nn * total + i];
+    v += eps * fp * weight;
+    v = v * inv;
+    if (T::shifted) v -= WD[q];
+    T::st(f + i, (typename T::C)v);
+  }
+  __syncthreads();
+  if (t == 0) I->marks[wi.clone] = 0;
+}
+
+// ---------------------------------------------------------------- set-up
+template <int ST>
+__global__ void k_init_eq(const InstDev* __restrict__ insts,
+                          const Counters* __restrict__ counters, int inst,
+                          double amp, uint32_t seed_lo, uint32_t seed_hi) {
+  using T = Traits<ST>;
+  using S = typename T::S;
+  const InstDev* __restrict__ I = insts + inst;
+  const int clone = blockIdx.x;
+  const int par = counters->parity[inst];
+  S* f = (S*)I->buf[par] + (long long)clone * I->clone_stride;
+  const int Lx = I->Lx, plane = I->plane;
+  for (int s = threadIdx.x; s < plane; s += blockDim.x) {
+    const int x = s % Lx, y = s / Lx;
+    double feq[9];
+    if (x > I->xmin && x < I->xmax && y > I->ymin && y < I->ymax) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) feq[k] = WD[k];
+    } else {
+#pragma clang fp contract(off)
+      const U4 r = philox((uint32_t)s, (uint32_t)clone, 0xFFFFFFFFu,
+                          (uint32_t)inst, seed_lo, seed_hi);
+      const double two32inv = 2.3283064365386963e-10;
+      const double ux = I->u_in + amp * (2.0 * ((double)r.x * two32inv) - 1.0);
+      const double uy = amp * (2.0 * ((double)r.y * two32inv) - 1.0);
+      const double u2 = -1.5 * (ux * ux + uy * uy);
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const double eu = (double)CX[k] * ux + (double)CY[k] * uy;
+        const double eueu = 4.5 * eu * eu;
+        feq[k] = WD[k] * 1.0 * (1.0 + 3.0 * eu + eueu + u2);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const double v = T::shifted ? feq[k] - WD[k] : feq[k];
+      T::st(f + k * plane + s, (typename T::C)v);
+    }
+  }
+}
+
+// AoS FP64 [clone][x][y][k] <-> SoA storage of clones clone0..clone0+n-1 in
+// buf[which ^ parity] (which = 0: current state, 1: the other buffer).
+template <int ST>
+__global__ void k_aos_to_soa(const InstDev* __restrict__ insts,
+                             const Counters* __restrict__ counters, int inst,
+                             int which, int clone0, const double* __restrict__ aos) {
+  using T = Traits<ST>;
+  using S = typename T::S;
+  const InstDev* __restrict__ I = insts + inst;
+  const int c = blockIdx.y;
+  const int Lx = I->Lx, Ly = I->Ly, plane = I->plane;
+  const int par = counters->parity[inst] ^ which;
+  S* f = (S*)I->buf[par] + (long long)(clone0 + c) * I->clone_stride;
+  const double* src = aos + (long long)c * 9 * plane;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < plane;
+       s += gridDim.x * blockDim.x) {
+    const int x = s % Lx, y = s / Lx;
+    const double* p = src + 9LL * (y + (long long)Ly * x);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const double v = T::shifted ? p[k] - WD[k] : p[k];
+      T::st(f + k * plane + s, (typename T::C)v);
+    }
+  }
+}
+
+template <int ST>
+__global__ void k_soa_to_aos(const InstDev* __restrict__ insts,
+                             const Counters* __restrict__ counters, int inst,
+                             int which, int clone0, double* __restrict__ aos) {
+  using T = Traits<ST>;
+  using S = typename T::S;
+  const InstDev* __restrict__ I = insts + inst;
+  const int c = blockIdx.y;
+  const int Lx = I->Lx, Ly = I->Ly, plane = I->plane;
+  const int par = counters->parity[inst] ^ which;
+  const S* f = (const S*)I->buf[par] + (long long)(clone0 + c) * I->clone_stride;
+  double* dst = aos + (long long)c * 9 * plane;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < plane;
+       s += gridDim.x * blockDim.x) {
+    const int x = s % Lx, y = s / Lx;
+    double* p = dst + 9LL * (y + (long long)Ly * x);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      double v = (double)T::ld(f + k * plane + s);
+      if (T::shifted) v += WD[k];
+      p[k] = v;
+    }
+  }
+}
+
+__global__ void k_snap_to_soa(int n, const double* __restrict__ aos, int Lx,
+                              int Ly, double* __restrict__ snaps) {
+  const int c = blockIdx.y;
+  const int plane = Lx * Ly;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < plane;
+       s += gridDim.x * blockDim.x) {
+    const int x = s % Lx, y = s / Lx;
+    const double* p = aos + (long long)c * 9 * plane + 9LL * (y + (long long)Ly * x);
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+      snaps[(long long)c * 9 * plane + (long long)k * plane + s] = p[k];
+  }
+}
+
+template <int MODE, int ST>
+hipError_t launch_interval_t(const Launch& L) {
+  const size_t lds = sizeof(double) * 2 * (size_t)L.ring_max;
+  hipLaunchKernelGGL((k_interval<MODE, ST>), dim3(L.n_items), dim3(L.threads),
+                     lds, L.stream, L.insts, L.items, L.counters, L.ring_max);
+  return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_interval_m(const Launch& L) {
+  switch (L.storage) {
+    case kF64: return launch_interval_t<MODE, kF64>(L);
+    case kF32: return launch_interval_t<MODE, kF32>(L);
+    case kF16: return launch_interval_t<MODE, kF16>(L);
+    case kBF16: return launch_interval_t<MODE, kBF16>(L);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int MODE, int ST>
+hipError_t launch_gather_t(const Launch& L, uint32_t lo, uint32_t hi) {
+  hipLaunchKernelGGL((k_gather<MODE, ST>), dim3(L.n_items), dim3(256), 0,
+                     L.stream, L.insts, L.items, L.counters, lo, hi);
+  return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_gather_m(const Launch& L, uint32_t lo, uint32_t hi) {
+  switch (L.storage) {
+    case kF64: return launch_gather_t<MODE, kF64>(L, lo, hi);
+    case kF32: return launch_gather_t<MODE, kF32>(L, lo, hi);
+    case kF16: return launch_gather_t<MODE, kF16>(L, lo, hi);
+    case kBF16: return launch_gather_t<MODE, kBF16>(L, lo, hi);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_interval(const Launch& L) {
+  if (L.n_items <= 0) return hipSuccess;
+  return L.mode == kModeRef ? launch_interval_m<kModeRef>(L)
+                            : launch_interval_m<kModeChannel>(L);
+}
+
+hipError_t launch_flip(const Launch& L, int n_inst, const int* flip_mask_dev) {
+  hipLaunchKernelGGL(k_flip, dim3(1), dim3(64), 0, L.stream, L.insts,
+                     L.counters, n_inst, flip_mask_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_perturb(const Launch& L, int philox_prefactors,
+                          uint32_t lo, uint32_t hi) {
+  if (L.n_items <= 0) return hipSuccess;
+  switch (L.storage) {
+    case kF64:
+      hipLaunchKernelGGL(k_perturb<kF64>, dim3(L.n_items), dim3(256), 0,
+                         L.stream, L.insts, L.items, L.counters,
+                         philox_prefactors, lo, hi);
+      break;
+    case kF32:
+      hipLaunchKernelGGL(k_perturb<kF32>, dim3(L.n_items), dim3(256), 0,
+                         L.stream, L.insts, L.items, L.counters,
+                         philox_prefactors, lo, hi);
+      break;
+    case kF16:
+      hipLaunchKernelGGL(k_perturb<kF16>, dim3(L.n_items), dim3(256), 0,
+                         L.stream, L.insts, L.items, L.counters,
+                         philox_prefactors, lo, hi);
+      break;
+    case kBF16:
+      hipLaunchKernelGGL(k_perturb<kBF16>, dim3(L.n_items), dim3(256), 0,
+                         L.stream, L.insts, L.items, L.counters,
+                         philox_prefactors, lo, hi);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_select(const InstDev* insts, Counters* counters, int inst0,
+                         int n_inst, int n_max, int weights_from_host,
+                         int draws_from_host, uint32_t lo, uint32_t hi,
+                         hipStream_t stream) {
+  const int threads = 1024;
+  const size_t lds = sizeof(int) * (4 * (size_t)n_max + threads);
+  static size_t lds_set = 0;
+  if (lds > 65536 && lds > lds_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_select,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    lds_set = lds;
+  }
+  hipLaunchKernelGGL(k_select, dim3(n_inst), dim3(threads), lds, stream, insts,
+                     counters, inst0, weights_from_host, draws_from_host, lo,
+                     hi);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const Launch& L, uint32_t lo, uint32_t hi) {
+  if (L.n_items <= 0) return hipSuccess;
+  return L.mode == kModeRef ? launch_gather_m<kModeRef>(L, lo, hi)
+                            : launch_gather_m<kModeChannel>(L, lo, hi);
+}
+
+hipError_t launch_init_eq(const Launch& L, int inst, int n_clones, double amp,
+                          uint32_t lo, uint32_t hi) {
+  switch (L.storage) {
+    case kF64:
+      hipLaunchKernelGGL(k_init_eq<kF64>, dim3(n_clones), dim3(256), 0,
+                         L.stream, L.insts, L.counters, inst, amp, lo, hi);
+      break;
+    case kF32:
+      hipLaunchKernelGGL(k_init_eq<kF32>, dim3(n_clones), dim3(256), 0,
+                         L.stream, L.insts, L.counters, inst, amp, lo, hi);
+      break;
+    case kF16:
+      hipLaunchKernelGGL(k_init_eq<kF16>, dim3(n_clones), dim3(256), 0,
+                         L.stream, L.insts, L.counters, inst, amp, lo, hi);
+      break;
+    case kBF16:
+      hipLaunchKernelGGL(k_init_eq<kBF16>, dim3(n_clones), dim3(256), 0,
+                         L.stream, L.insts, L.counters, inst, amp, lo, hi);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_aos_to_soa(int storage, const InstDev* inst_dev,
+                             const Counters* counters, int inst, int which,
+                             int clone0, int n, const double* aos_dev,
+                             hipStream_t stream) {
+  dim3 grid(64, n);
+  switch (storage) {
+    case kF64:
+      hipLaunchKernelGGL(k_aos_to_soa<kF64>, grid, dim3(256), 0, stream,
+                         inst_dev, counters, inst, which, clone0, aos_dev);
+      break;
+    case kF32:
+      hipLaunchKernelGGL(k_aos_to_soa<kF32>, grid, dim3(256), 0, stream,
+                         inst_dev, counters, inst, which, clone0, aos_dev);
+      break;
+    case kF16:
+      hipLaunchKernelGGL(k_aos_to_soa<kF16>, grid, dim3(256), 0, stream,
+                         inst_dev, counters, inst, which, clone0, aos_dev);
+      break;
+    case kBF16:
+      hipLaunchKernelGGL(k_aos_to_soa<kBF16>, grid, dim3(256), 0, stream,
+                         inst_dev, counters, inst, which, clone0, aos_dev);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(int storage, const InstDev* inst_dev,
+                             const Counters* counters, int inst, int which,
+                             int clone0, int n, double* aos_dev,
+                             hipStream_t stream) {
+  dim3 grid(64, n);
+  switch (storage) {
+    case kF64:
+      hipLaunchKernelGGL(k_soa_to_aos<kF64>, grid, dim3(256), 0, stream,
+                         inst_dev, counters, inst, which, clone0, aos_dev);
+      break;
+    case kF32:
+      hipLaunchKernelGGL(k_soa_to_aos<kF32>, grid, dim3(256), 0, stream,
+                         inst_dev, counters, inst, which, clone0, aos_dev);
+      break;
+    case kF16:
+      hipLaunchKernelGGL(k_soa_to_aos<kF16>, grid, dim3(256), 0, stream,
+                         inst_dev, counters, inst, which, clone0, aos_dev);
+      break;
+    case kBF16:
+      hipLaunchKernelGGL(k_soa_to_aos<kBF16>, grid, dim3(256), 0, stream,
+                         inst_dev, counters, inst, which, clone0, aos_dev);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_snap_to_soa(const InstDev*, int, int n, const double* aos_dev,
+                              int Lx, int Ly, double* snaps_dev,
+                              hipStream_t stream) {
+  hipLaunchKernelGGL(k_snap_to_soa, dim3(64, n), dim3(256), 0, stream, n,
+                     aos_dev, Lx, Ly, snaps_dev);
+  return hipGetLastError();
+}
+
+}  // namespace lbc

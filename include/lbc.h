@@ -1,0 +1,196 @@
+/*
+ * lbc.h -- C ABI of the MI355X cloning-sampler inner loop.
+ *
+ * N D2Q9 lattice-Boltzmann clones of flow past a square obstacle advance
+ * together on one GPU; their drag is time-integrated into FP64 log-weights
+ * and the population is resampled, copied and perturbed on the device at
+ * every cloning step.  One context may hold several independent sampler
+ * instances (lattice size, bias, clone count, steps per interval) that share
+ * one physics mode and one storage precision; all of them advance in one
+ * persistent kernel launch per cloning interval.
+ *
+ * In the reference (tlestang/cloning-lbm-MPI) this path is a set of free C++
+ * functions over caller-owned double arrays plus inline code in the MPI
+ * driver's main(); each entry point below names what it replaces.
+ *
+ * Conventions
+ *  - Every int-returning function returns LBC_OK (0) or a negative LBC_E_*
+ *    code; lbc_last_error() gives the message of the last failure.
+ *  - Host arrays are caller-owned and only read/written during the call.
+ *    Device arrays are library-owned.
+ *  - Population arrays crossing this boundary are FP64 in the reference's
+ *    AoS layout IDX(x,y,k) = k + 9*(y + Ly*x) (src/lbm_src/global.h:4),
+ *    one lattice after another.
+ *  - A context is single-threaded.  All device work is ordered on the
+ *    context's own stream; functions that return host data synchronise it.
+ */
+#ifndef LBC_H
+#define LBC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBC_OK 0
+#define LBC_E_ARG -1     /* invalid argument or configuration */
+#define LBC_E_HIP -2     /* HIP runtime error */
+#define LBC_E_NOMEM -3   /* device allocation failed */
+#define LBC_E_STATE -4   /* call not valid in the current state */
+#define LBC_E_NUMERIC -5 /* non-finite weight or population detected */
+
+/* Physics modes. */
+#define LBC_MODE_REF 0     /* periodic x, body force, stress-integral drag */
+#define LBC_MODE_CHANNEL 1 /* equilibrium inlet, zero-gradient outlet,
+                              momentum-exchange drag */
+
+/* Population storage.  F64 stores f; the others store f - w_q (shifted by
+ * the rest weights) and collide in FP32. */
+#define LBC_STORE_F64 0
+#define LBC_STORE_F32 1
+#define LBC_STORE_F16 2
+#define LBC_STORE_BF16 3
+
+typedef struct lbc_instance {
+  int Lx, Ly;                 /* lattice (Dx, Dy in the reference) */
+  int xmin, xmax, ymin, ymax; /* square ring nodes; solid = open interior */
+  int n_clones;               /* Nc */
+  int steps;                  /* LBM steps per cloning interval (M) */
+  double tau;                 /* BGK relaxation time */
+  double beta;                /* REF: uniform body force (beta0) */
+  double u_in;                /* CHANNEL: inlet velocity */
+  double force_scale;         /* A_n = F_n*force_scale - force_shift */
+  double force_shift;
+  double dt;                  /* interval observable = dt * sum_n A_n */
+  double bias;                /* log-weight = bias * observable (alpha, k) */
+  double eps;                 /* REF: snapshot-perturbation amplitude */
+  double noise_sigma;         /* CHANNEL: population noise after a copy */
+  int rank_blocks;            /* total_R summed in this many equal blocks (>=1) */
+  int n_snapshots;            /* REF: snapshots mixed by the perturbation (NN) */
+} lbc_instance;
+
+typedef struct lbc_config {
+  int mode;                   /* LBC_MODE_* */
+  int storage;                /* LBC_STORE_* */
+  int n_instances;
+  int device;                 /* HIP device ordinal */
+  uint64_t seed;              /* Philox key */
+  int threads;                /* workgroup size of the step kernel; 0 = tuned */
+  int flags;                  /* reserved, 0 */
+  const lbc_instance* instances;
+} lbc_config;
+
+/* Host-visible result of one selection (all arrays caller-owned, sized for
+ * the instance's n_clones; any pointer may be NULL). */
+typedef struct lbc_selection {
+  int32_t* nb_before;  /* copies per clone after the floor step */
+  int32_t* nb_after;   /* after prune/add (sums to n_clones) */
+  int32_t* pairs;      /* [2*n_clones]: (source, destination) per copy */
+  int32_t* labels;     /* parent of every slot */
+  int32_t n_pairs;
+  int32_t n_rint_used; /* prune/add draws consumed */
+  double total_R;      /* mean weight of the interval */
+  double log_total_R;  /* log of the mean weight */
+} lbc_selection;
+
+typedef struct lbc_ctx lbc_ctx;
+
+/* Create a context: allocates every device buffer and, if threads == 0,
+ * picks the workgroup size by timing.  Replaces the per-rank allocations of
+ * src/mpi_LBM_GK_pops_perturb.cpp:180-199 and the geometry globals of
+ * src/lbm_src/global.h:11-15. */
+int lbc_create(const lbc_config* cfg, lbc_ctx** out);
+void lbc_destroy(lbc_ctx* ctx);
+const char* lbc_last_error(const lbc_ctx* ctx);
+
+/* Copy n lattices (AoS FP64) into clones clone0..clone0+n-1.
+ * Replaces the pops-file reads of src/mpi_LBM_GK_pops_perturb.cpp:291,305. */
+int lbc_load_states(lbc_ctx* ctx, int inst, int clone0, int n,
+                    const double* aos);
+/* Read n lattices back (AoS FP64, unshifted). */
+int lbc_store_states(lbc_ctx* ctx, int inst, int clone0, int n, double* aos);
+
+/* Equilibrium initial state feq(rho=1, u=(u_in + a(2U-1), a(2U-1))) per
+ * fluid site, Philox-seeded, rest weights on the solid interior.  No
+ * reference counterpart (channel configuration); the rest-state case a=0,
+ * u_in=0 is src/lbm_src/initialize_lattice_arrays.cpp:7-20. */
+int lbc_init_equilibrium(lbc_ctx* ctx, int inst, double amp);
+
+/* Snapshot states mixed by the REF perturbation (n_snapshots AoS lattices).
+ * Replaces popsForPerturb (src/mpi_LBM_GK_pops_perturb.cpp:321-364). */
+int lbc_set_snapshots(lbc_ctx* ctx, int inst, const double* aos);
+
+/* Clones marked for perturbation at the start of the next interval
+ * (mark_perturb, src/mpi_LBM_GK_pops_perturb.cpp:204-211,642-648,676,693).
+ * Selection sets them for every overwritten slot. */
+int lbc_set_marks(lbc_ctx* ctx, int inst, const uint8_t* marks);
+int lbc_get_marks(lbc_ctx* ctx, int inst, uint8_t* marks);
+
+/* One cloning interval of every instance: perturb marked clones (REF), then
+ * `steps` LBM steps of every clone with the drag integrated into FP64
+ * log-weights.  prefactors: NULL (device Philox draws) or, per instance in
+ * order, n_clones*n_snapshots uniforms (rows of unmarked clones unused).
+ * Replaces src/mpi_LBM_GK_pops_perturb.cpp:399-470: generatePerturbedState
+ * (src/perturbation_routines/generatePerturbedState.cpp:4-38),
+ * streamingAndCollisionComputeMacroBodyForce
+ * (src/lbm_src/streamCollCompute.cpp:7-48), computeDomainNoSlipWalls_BB
+ * (src/lbm_src/domain_noSlipWalls.cpp:6-22), computeSquareBounceBack_TEST
+ * (src/lbm_src/square.cpp:24-75), the interior reset (:430-439),
+ * computeForceOnSquare (src/lbm_src/force.cpp:8-144) and the weight
+ * accumulation (:445-469). */
+int lbc_run_interval(lbc_ctx* ctx, const double* prefactors);
+
+/* Per-step observable A_n = F_n*scale - shift of the last interval,
+ * [n_clones][steps] (what the reference appends to data_force.datout,
+ * src/mpi_LBM_GK_pops_perturb.cpp:456). */
+int lbc_get_drag(lbc_ctx* ctx, int inst, double* A);
+/* Interval observable dt*sum A (s_ at :465) and log-weight bias*that. */
+int lbc_get_logweights(lbc_ctx* ctx, int inst, double* obs, double* logw);
+
+/* Selection and copy table for one instance.
+ * weights: NULL (exp(logw) on device) or n_clones host weights.
+ * u: NULL (Philox) or n_clones uniforms in [0,1) for the floor step.
+ * rint: NULL (Philox) or n_clones non-negative ints for prune/add.
+ * Replaces the master's gather and selection,
+ * src/mpi_LBM_GK_pops_perturb.cpp:487-632, and the copy-table broadcast
+ * :636-640. */
+int lbc_select(lbc_ctx* ctx, int inst, const double* weights, const double* u,
+               const int32_t* rint, lbc_selection* out);
+
+/* Copy every selected parent into its destination slot (with CHANNEL noise)
+ * and mark the destinations.  Replaces the memcpy / MPI_Send / MPI_Recv
+ * loop of src/mpi_LBM_GK_pops_perturb.cpp:658-701. */
+int lbc_gather(lbc_ctx* ctx);
+
+/* n full cloning steps (interval, selection, copy) of every instance with
+ * device Philox draws, replayed from one captured HIP graph. */
+int lbc_advance(lbc_ctx* ctx, int n_intervals);
+
+/* Mean weight of every interval so far (R_record,
+ * src/mpi_LBM_GK_pops_perturb.cpp:621, written to rvalues.dat at :705-714). */
+int lbc_get_R_history(lbc_ctx* ctx, int inst, double* R, int max_n,
+                      int* n_out);
+
+/* Wait for the context's stream. */
+int lbc_sync(lbc_ctx* ctx);
+
+/* Average device time (ms) of the step kernel over the last
+ * lbc_run_interval / lbc_advance call, from HIP events on the context's
+ * stream, and the number of launches it averaged. */
+int lbc_step_kernel_ms(lbc_ctx* ctx, double* avg_ms, int* n_launches);
+
+/* Workgroup size the context uses for the step kernel. */
+int lbc_threads(const lbc_ctx* ctx);
+
+/* One reference LBM step on one lattice held in host memory (FP64):
+ * fin (post-stream state) becomes the post-collision state, fout receives
+ * the next post-stream state, *force the raw drag on it.  Parity hook for
+ * the sequence at src/mpi_LBM_GK_pops_perturb.cpp:426-445. */
+int lbc_step_host(int mode, const lbc_instance* geom, double* fin,
+                  double* fout, double* force);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

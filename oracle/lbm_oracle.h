@@ -1,0 +1,125 @@
+/*
+ * CPU FP64 oracle for the cloning-sampler inner loop (D2Q9 LBM clones, drag,
+ * weights, selection, perturbation).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library or driver links,
+ * loads or calls this code; it is the checker used by tests/, by the smoke
+ * test and by bench.py's CPU-baseline leg.
+ *
+ * Layout: the reference's AoS population layout, IDX(x,y,k) = k + 9*(y + Dy*x)
+ * (src/lbm_src/global.h:4), FP64.
+ *
+ * Two physics modes share geometry and the cloning machinery:
+ *   ORC_MODE_REF     periodic x, uniform body force beta with Guo forcing,
+ *                    half-way bounce-back walls and square, interior reset,
+ *                    stress-integral drag (the reference's production path);
+ *   ORC_MODE_CHANNEL equilibrium inlet (ghost column at x=-1 holding
+ *                    feq(rho=1, u=(u_in,0))), zero-gradient outlet (ghost
+ *                    column x=Lx copies column Lx-1), same walls and square,
+ *                    momentum-exchange drag.  No reference counterpart.
+ */
+#ifndef LBM_ORACLE_H
+#define LBM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MODE_REF 0
+#define ORC_MODE_CHANNEL 1
+
+typedef struct {
+  int mode;
+  int Lx, Ly;                    /* Dx, Dy of the reference */
+  int xmin, xmax, ymin, ymax;    /* square ring; interior is open box */
+  double tau;
+  double beta;                   /* REF body force (Guo) */
+  double u_in;                   /* CHANNEL inlet velocity */
+} orc_geom;
+
+/* D2Q9 velocity set and weights, in the reference's ordering. */
+extern const int orc_c[9][2];
+extern const double orc_w[9];
+extern const int orc_opp[9];
+
+/* Bounce-back mask of site (x,y): bit q set when population q arriving at
+ * (x,y) is reflected from (x,y)'s own opposite population. */
+int orc_bb_mask(const orc_geom* g, int x, int y);
+
+/* One LBM step in push form with explicit boundary passes.  fin holds the
+ * post-stream state S_n on entry and the post-collision state P_n on exit;
+ * fout receives S_{n+1}.  Returns the raw drag on S_{n+1} (stress integral in
+ * REF mode, momentum exchange in CHANNEL mode). */
+double orc_step(const orc_geom* g, double* fin, double* fout);
+
+/* Drag functionals on a post-stream state. */
+double orc_force_stress(const orc_geom* g, const double* f);
+double orc_force_mea(const orc_geom* g, const double* f);
+
+/* M steps on `state` (S form in and out; scratch of the same size).
+ * A_out[n] = F_{n+1}*scale - shift (may be NULL); returns
+ * dt * sum_n A_n, summed sequentially. */
+double orc_run_steps(const orc_geom* g, double* state, double* scratch, int M,
+                     double scale, double shift, double dt, double* A_out);
+
+/* Snapshot-mix perturbation with injected prefactors a[NN]:
+ * f' = sum_n a_n P_n; r = |fin|/|f'|; fin <- (fin + eps*r*f') / (1 + eps*r*sum a).
+ * snaps is NN contiguous states of N doubles. */
+void orc_perturb(double* fin, const double* snaps, int NN, long N, double eps,
+                 const double* a);
+
+/* Selection with injected draws.  s[Nc] weights, u[Nc] uniforms for the floor
+ * step, rint[] non-negative ints consumed one per prune/add iteration
+ * (at most Nc are needed).  p_blocks: number of equal rank blocks for the
+ * total_R summation order (1 = sequential).  Outputs: nb_before[Nc] (after
+ * the floor step), nb_after[Nc] (after prune/add), pairs[2*Nc] (src,dst),
+ * labels[Nc], *total_R.  Returns the number of pairs, or -1 on bad input.
+ * *n_rint_used receives the number of rint entries consumed. */
+int orc_select(int Nc, const double* s, int p_blocks, const double* u,
+               const int32_t* rint, int32_t* nb_before, int32_t* nb_after,
+               int32_t* pairs, int32_t* labels, double* total_R,
+               int* n_rint_used);
+
+/* Philox4x32-10 block (Salmon et al., SC'11). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
+                       uint32_t out[4]);
+
+/* Standard normals from one Philox block via Box-Muller:
+ * u1 = (r0+1)*2^-32, u2 = r1*2^-32 -> (n0, n1); same for (r2, r3). */
+void orc_box_muller4(const uint32_t r[4], double n[4]);
+
+/* CHANNEL population noise of one clone: every fluid population gets
+ * sigma * N(0,1); counters (3*site + j, clone, interval, instance),
+ * key (seed_lo, seed_hi).  site = y*Lx + x. */
+void orc_noise(const orc_geom* g, double* f, double sigma, uint32_t seed_lo,
+               uint32_t seed_hi, uint32_t clone, uint32_t interval,
+               uint32_t instance);
+
+/* CHANNEL initial state of one clone: feq(1, (u_in + a*(2U-1), a*(2U-1)))
+ * on fluid sites, w on the solid interior; counter (site, clone, 0xFFFFFFFF,
+ * instance). */
+void orc_init_channel(const orc_geom* g, double* f, double amp, uint32_t seed_lo,
+                      uint32_t seed_hi, uint32_t clone, uint32_t instance);
+
+/* Equilibrium of (rho, ux, uy) in the reference's arithmetic order. */
+void orc_feq(double rho, double ux, double uy, double feq[9]);
+
+/* glibc RNG access (the reference's own generators). */
+void orc_glibc_srand(unsigned seed);
+void orc_glibc_rand_n(int n, int32_t* out);
+void orc_glibc_seed48_default(void);
+void orc_glibc_drand48_n(int n, double* out);
+
+/* Time of an OpenMP-free, single-thread AoS run: clone-LUPS measurement
+ * helper for the CPU baseline (runs `nclones` independent clones for M
+ * steps each, returns the accumulated sum of A over all clones). */
+double orc_run_clones(const orc_geom* g, double* states, double* scratch,
+                      int nclones, int M, double scale, double shift,
+                      double dt, double* acc_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,241 @@
+"""GPU parity of the HIP path against the CPU FP64 oracle, through the C ABI.
+
+FP64 storage must match the oracle bit for bit (lattice fields, per-step
+drag, interval observable, selection, copies).  Quantities that go through
+libm transcendental functions on the device (noise, perturbation norms,
+exp of log-weights) are compared at 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_state(O, g, rng, amp=1e-3, u0=0.05):
+    f = np.empty((g.Lx, g.Ly, 9))
+    for x in range(g.Lx):
+        for y in range(g.Ly):
+            f[x, y] = O.feq(1 + amp * rng.standard_normal(),
+                            u0 + amp * rng.standard_normal(),
+                            amp * rng.standard_normal())
+    f += 1e-4 * amp * rng.standard_normal(f.shape)
+    # solid interior at rest, as the reference's reset leaves it
+    f[g.xmin + 1:g.xmax, g.ymin + 1:g.ymax, :] = O.W
+    return f
+
+
+def geom_of(O, spec, mode):
+    return O.Geom(mode, spec.Lx, spec.Ly, spec.xmin, spec.xmax, spec.ymin,
+                  spec.ymax, spec.tau, spec.beta if mode == O.MODE_REF else 0.0,
+                  spec.u_in if mode == O.MODE_CHANNEL else 0.0)
+
+
+@pytest.mark.parametrize("L", [4, 8])
+def test_ref_step_host_bitexact(O, lbc, L):
+    spec = lbc.ref_box(L, 0.53, 0.05, 1, 1, 1.0)
+    g = O.ref_box_geom(L, 0.53, 0.05)
+    print(f"case: lbc_step_host REF L={L}", flush=True)
+    fin = rand_state(O, g, np.random.default_rng(L))
+    a, b = fin.copy(), np.zeros_like(fin)
+    F = O.step(g, a, b)
+    pc, nxt, Fg = lbc.step_host(lbc.MODE_REF, spec, fin)
+    assert np.array_equal(nxt, b)
+    assert np.array_equal(pc, a)
+    assert Fg == F
+
+
+@pytest.mark.parametrize("storage_threads", [256, 1024])
+def test_ref_interval_f64_bitexact(O, lbc, storage_threads):
+    L, M, N = 8, 37, 5
+    spec = lbc.ref_box(L, 0.515, 0.05, N, M, alpha=3.0)
+    g = O.ref_box_geom(L, 0.515, 0.05)
+    rng = np.random.default_rng(11)
+    states = np.stack([rand_state(O, g, rng) for _ in range(N)])
+    print(f"case: REF interval F64 L={L} M={M} N={N} threads={storage_threads}",
+          flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_REF, threads=storage_threads) as s:
+        s.load_states(0, 0, states)
+        s.run_interval()
+        got = s.store_states(0)
+        A = s.drag(0)
+        obs, lw = s.logweights(0)
+    for j in range(N):
+        ref = states[j].copy()
+        acc, Aref = O.run_steps(g, ref, M, spec.force_scale, spec.force_shift,
+                                spec.dt)
+        assert np.array_equal(got[j], ref), f"clone {j} fields"
+        assert np.array_equal(A[j], Aref), f"clone {j} drag"
+        assert obs[j] == acc
+        assert lw[j] == spec.bias * acc
+
+
+@pytest.mark.parametrize("M", [1, 2, 24])
+def test_channel_interval_f64_bitexact(O, lbc, M):
+    spec = lbc.channel(80, 40, 8, 20, 20, 0.53, 0.05, 3, M, k=0.01)
+    g = geom_of(O, spec, O.MODE_CHANNEL)
+    print(f"case: CHANNEL interval F64 80x40 M={M}", flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, seed=5, threads=512) as s:
+        s.init_equilibrium(0, 1e-3)
+        init = s.store_states(0)
+        for j in range(3):
+            ref0 = O.init_channel(g, 1e-3, 5, j)
+            assert np.array_equal(init[j], ref0), f"init clone {j}"
+        s.run_interval()
+        got = s.store_states(0)
+        A = s.drag(0)
+        obs, lw = s.logweights(0)
+    for j in range(3):
+        ref = init[j].copy()
+        acc, Aref = O.run_steps(g, ref, M)
+        assert np.array_equal(got[j], ref), f"clone {j} fields"
+        assert np.array_equal(A[j], Aref), f"clone {j} drag"
+        assert obs[j] == acc
+        assert lw[j] == 0.01 * acc
+
+
+@pytest.mark.parametrize("N,p", [(1, 1), (2, 1), (7, 1), (64, 4), (1000, 8),
+                                 (4096, 1), (8192, 2)])
+def test_selection_matches_oracle(O, lbc, N, p):
+    rng = np.random.default_rng(N)
+    spec = lbc.channel(32, 16, 4, 8, 8, 0.6, 0.05, N, 1, k=0.0)
+    print(f"case: selection N={N} p={p}", flush=True)
+    spec.rank_blocks = p
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, threads=256) as s:
+        for trial in range(3):
+            w = np.exp(rng.normal(0, 0.5 + trial, N))
+            u = rng.random(N)
+            r = rng.integers(0, 2**31 - 1, N, dtype=np.int64).astype(np.int32)
+            ref = O.select(w, u, r, p_blocks=p)
+            got = s.select(0, weights=w, u=u, rint=r)
+            assert np.array_equal(got.nb_before, ref["nb_before"])
+            assert np.array_equal(got.nb_after, ref["nb_after"])
+            assert np.array_equal(got.pairs, ref["pairs"])
+            assert np.array_equal(got.labels, ref["labels"])
+            assert got.total_R == ref["total_R"]
+            assert got.n_rint_used == ref["n_rint"]
+
+
+def test_gather_copies_parents_exactly(O, lbc):
+    L, N = 4, 8
+    spec = lbc.ref_box(L, 0.53, 0.05, N, 3, alpha=1.0)
+    g = O.ref_box_geom(L, 0.53, 0.05)
+    rng = np.random.default_rng(3)
+    states = np.stack([rand_state(O, g, rng) for _ in range(N)])
+    print("case: gather REF F64", flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_REF, threads=256) as s:
+        s.load_states(0, 0, states)
+        w = np.array([5.0, 0.1, 0.1, 3.0, 0.1, 1.0, 0.1, 0.1])
+        u = rng.random(N)
+        r = rng.integers(0, 2**31 - 1, N).astype(np.int32)
+        sel = s.select(0, weights=w, u=u, rint=r)
+        ref = O.select(w, u, r)
+        assert np.array_equal(sel.labels, ref["labels"])
+        assert len(sel.pairs) > 0
+        s.gather()
+        got = s.store_states(0)
+        marks = s.get_marks(0)
+    for k in range(N):
+        assert np.array_equal(got[k], states[sel.labels[k]])
+        assert marks[k] == (sel.labels[k] != k)
+
+
+def test_channel_noise_on_copies(O, lbc):
+    spec = lbc.channel(48, 24, 6, 12, 12, 0.55, 0.05, 6, 2, k=0.0, sigma=1e-4)
+    g = geom_of(O, spec, O.MODE_CHANNEL)
+    print("case: gather CHANNEL F64 with noise", flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, seed=123, threads=256) as s:
+        s.init_equilibrium(0, 1e-3)
+        before = s.store_states(0)
+        w = np.array([4.0, 0.01, 0.01, 1.0, 0.5, 0.5])
+        u = np.full(6, 0.5)
+        r = np.arange(6, dtype=np.int32)
+        sel = s.select(0, weights=w, u=u, rint=r)
+        s.gather()
+        after = s.store_states(0)
+    assert len(sel.pairs) > 0
+    for k in range(6):
+        exp = before[sel.labels[k]].copy()
+        if sel.labels[k] != k:
+            O.noise(g, exp, 1e-4, 123, k, 1, 0)
+            np.testing.assert_allclose(after[k], exp, rtol=0, atol=1e-17)
+            assert not np.array_equal(after[k], before[sel.labels[k]])
+        else:
+            assert np.array_equal(after[k], exp)
+
+
+def test_ref_perturbation(O, lbc):
+    L, N, NN, M = 4, 4, 3, 5
+    spec = lbc.ref_box(L, 0.53, 0.05, N, M, alpha=1.0, eps=0.3, n_snapshots=NN)
+    g = O.ref_box_geom(L, 0.53, 0.05)
+    rng = np.random.default_rng(17)
+    states = np.stack([rand_state(O, g, rng) for _ in range(N)])
+    snaps = np.stack([rand_state(O, g, rng, amp=1e-2) for _ in range(NN)])
+    pref = rng.random((N, NN))
+    marks = np.array([0, 1, 1, 0], np.uint8)
+    print("case: REF snapshot perturbation F64", flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_REF, threads=256) as s:
+        s.load_states(0, 0, states)
+        s.set_snapshots(0, snaps)
+        s.set_marks(0, marks)
+        s.run_interval(prefactors=pref)
+        got = s.store_states(0)
+        A = s.drag(0)
+        assert not s.get_marks(0).any()
+    for j in range(N):
+        ref = states[j].copy()
+        if marks[j]:
+            O.perturb(ref, snaps, 0.3, pref[j])
+        acc, Aref = O.run_steps(g, ref, M, spec.force_scale, spec.force_shift,
+                                spec.dt)
+        np.testing.assert_allclose(got[j], ref, rtol=1e-12, atol=0)
+        np.testing.assert_allclose(A[j], Aref, rtol=1e-9, atol=1e-12)
+        if not marks[j]:
+            assert np.array_equal(got[j], ref)
+
+
+def test_advance_graph_matches_stepwise(lbc):
+    """lbc_advance (captured graph, device draws) equals the same sequence
+    issued call by call."""
+    spec = lbc.channel(64, 32, 6, 16, 16, 0.55, 0.05, 16, 10, k=2.0,
+                       sigma=1e-4)
+    print("case: advance graph vs stepwise, CHANNEL F64", flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, seed=9, threads=256) as a, \
+            lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, seed=9, threads=256) as b:
+        a.init_equilibrium(0, 1e-3)
+        b.init_equilibrium(0, 1e-3)
+        a.advance(3)
+        for _ in range(3):
+            b.run_interval()
+            b.select(0)
+            b.gather()
+        assert np.array_equal(a.store_states(0), b.store_states(0))
+        Ra, Rb = a.R_history(0), b.R_history(0)
+        assert len(Ra) == 3 and np.array_equal(Ra, Rb)
+        assert np.all(np.isfinite(Ra)) and np.all(Ra > 0)
+
+
+@pytest.mark.parametrize("storage,tol", [(1, 2e-6), (2, 1e-3), (3, 1e-2)])
+def test_low_precision_channel_interval(O, lbc, storage, tol):
+    """FP32-collide on shifted storage vs the FP64 oracle from identical
+    states over one interval: per-clone interval drag within `tol` relative."""
+    spec = lbc.channel(200, 50, 10, 50, 25, 0.515, 0.05, 4, 100, k=0.01)
+    g = geom_of(O, spec, O.MODE_CHANNEL)
+    print(f"case: low precision storage={storage} 200x50 M=100", flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, seed=0, threads=512) as s64:
+        s64.init_equilibrium(0, 1e-3)
+        s64.run_interval()  # develop a little flow in FP64
+        init = s64.store_states(0)
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=storage,
+                     threads=512) as s:
+        s.load_states(0, 0, init)
+        s.run_interval()
+        obs, _ = s.logweights(0)
+        got = s.store_states(0)
+    for j in range(4):
+        ref = init[j].copy()
+        acc, _ = O.run_steps(g, ref, 100)
+        rel = abs(obs[j] - acc) / abs(acc)
+        print(f"storage {storage} clone {j}: drag sum {obs[j]:.9g} vs "
+              f"{acc:.9g} rel {rel:.2e}")
+        assert rel < tol
+        assert np.max(np.abs(got[j] - ref)) < 50 * tol
