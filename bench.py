@@ -1,0 +1,240 @@
+#!/usr/bin/env python3
+"""Benchmark of the cloning-sampler inner loop on one MI355X.
+
+Workload: channel flow 400x100 past a square of side 20 centred at
+(100, 50), tau 0.53, inlet u 0.05, 4096 clones, 200 LBM steps per cloning
+interval, bias k 0.005, population noise 1e-4 after each copy.  A "step" of
+this benchmark is one full cloning interval of all 4096 clones: the
+persistent step kernel (200 LBM steps with fused drag and FP64 weights), the
+selection and the on-device copy/noise, replayed from one HIP graph.
+
+The headline value is the clone-MLUPS of the FP16-shifted storage path
+(FP32 collide, FP64 drag/weights/selection).  The plain FP64 path and the
+FP32 / BF16-shifted paths are timed on the same workload in the same run,
+and the FP16 path's per-interval drag and log-mean-weight errors are
+measured against FP64 from identical states over one interval.
+
+Prints one JSON line.  Only N=1 is supported (one GPU).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cloning-lbm-mpi_amd"))
+import lbc  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD = dict(Lx=400, Ly=100, side=20, cx=100, cy=50, tau=0.53, u_in=0.05,
+                n_clones=4096, steps=200, k=0.005, sigma=1e-4)
+
+
+def spec_for(n_clones: int, steps: int | None = None) -> lbc.InstanceSpec:
+    w = WORKLOAD
+    return lbc.channel(w["Lx"], w["Ly"], w["side"], w["cx"], w["cy"], w["tau"],
+                       w["u_in"], n_clones, steps or w["steps"], w["k"],
+                       sigma=w["sigma"])
+
+
+def time_storage(storage: int, K: int, W: int, seed: int = 0) -> dict:
+    """W untimed warm-up intervals then K timed ones (graph replays)."""
+    spec = spec_for(WORKLOAD["n_clones"])
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=storage,
+                     seed=seed) as s:
+        s.init_equilibrium(0, 1e-3)
+        if W:
+            s.advance(W)
+        s.sync()
+        t0 = time.perf_counter()
+        s.advance(K)
+        s.sync()
+        t1 = time.perf_counter()
+        kms, nk = s.step_kernel_ms()
+        R = s.R_history(0)
+        threads = s.threads
+    secs = t1 - t0
+    lups = spec.n_clones * spec.sites * spec.steps * K
+    bpl = lbc.bytes_per_lup(storage)
+    alg_bytes = spec.n_clones * spec.sites * spec.steps * bpl
+    achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else float("nan")
+    return dict(
+        storage=lbc.STORAGE_NAMES[storage],
+        clone_mlups=lups / secs / 1e6,
+        ms_per_interval=secs / K * 1e3,
+        kernel_ms=kms,
+        kernel_launches=nk,
+        kernel_clone_mlups=spec.n_clones * spec.sites * spec.steps / (kms * 1e-3) / 1e6,
+        bytes_per_lup=bpl,
+        alg_bytes_per_launch=alg_bytes,
+        achieved_gbs=achieved,
+        roofline_frac=achieved / HBM_PEAK_GBS,
+        threads=threads,
+        logR_last=float(math.log(R[-1])) if len(R) else None,
+    )
+
+
+def accuracy(storage: int, n: int = 256, warm: int = 10, seed: int = 1) -> dict:
+    """Drag and log-mean-weight error of `storage` against FP64 from
+    identical developed states over one interval (no selection between)."""
+    spec = spec_for(n)
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=lbc.STORE_F64,
+                     seed=seed) as s64:
+        s64.init_equilibrium(0, 1e-3)
+        s64.advance(warm)
+        states = s64.store_states(0)
+        s64.load_states(0, 0, states)
+        s64.run_interval()
+        obs64, lw64 = s64.logweights(0)
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=storage,
+                     seed=seed) as slp:
+        slp.load_states(0, 0, states)
+        slp.run_interval()
+        obslp, lwlp = slp.logweights(0)
+    rel = np.abs(obslp - obs64) / np.abs(obs64)
+
+    def log_mean_exp(v):
+        m = v.max()
+        return m + math.log(np.mean(np.exp(v - m)))
+
+    logR64 = log_mean_exp(lw64)
+    logRlp = log_mean_exp(lwlp)
+    return dict(storage=lbc.STORAGE_NAMES[storage], clones=n,
+                warmup_intervals=warm,
+                drag_rel_err_max=float(rel.max()),
+                drag_rel_err_median=float(np.median(rel)),
+                logR_abs_err=float(abs(logRlp - logR64)),
+                logR_fp64=float(logR64))
+
+
+def cpu_baseline(sample_clones_per_core: int = 2, steps: int = 200) -> dict:
+    """The CPU oracle (a port of the reference's AoS push step, serial clones
+    per worker, one worker thread per core, like one MPI rank per core) on a
+    bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    w = WORKLOAD
+    g = O.channel_geom(w["Lx"], w["Ly"], w["side"], w["cx"], w["cy"], w["tau"],
+                       w["u_in"])
+    cores = max(1, min(16, os.cpu_count() or 1))
+    base = O.init_channel(g, 1e-3, 0, 0)
+    states = [np.stack([base] * sample_clones_per_core) for _ in range(cores)]
+    O.lib()
+
+    def work(i):
+        O.run_clones(g, states[i], steps)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(cores)]
+    t0 = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    secs = time.perf_counter() - t0
+    lups = cores * sample_clones_per_core * g.Lx * g.Ly * steps
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(value=lups / secs / 1e6, unit="clone-MLUPS", cores=cores,
+                kind="port",
+                sample=(f"{cores} threads x {sample_clones_per_core} clones x "
+                        f"{steps} steps of 400x100 (FP64 AoS push oracle), "
+                        f"{secs:.1f} s on {cpu}"))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5,
+                    help="timed cloning intervals")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="",
+                    help="comma list of storages to time (f64,f32,f16,bf16)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (N=1); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        return 2
+    K, W = args.steps, args.warmup
+    want = [x for x in args.only.split(",") if x] or ["f16", "f64", "f32",
+                                                       "bf16"]
+    codes = {"f64": lbc.STORE_F64, "f32": lbc.STORE_F32, "f16": lbc.STORE_F16,
+             "bf16": lbc.STORE_BF16}
+    runs = {}
+    for name in want:
+        print(f"bench: timing {name} ({K} intervals after {W} warm-up)",
+              file=sys.stderr, flush=True)
+        runs[name] = time_storage(codes[name], K, W)
+    acc = {}
+    for name in want:
+        if name == "f64":
+            continue
+        print(f"bench: accuracy {name} vs f64", file=sys.stderr, flush=True)
+        acc[name] = accuracy(codes[name])
+    cpu = None
+    if not args.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline()
+    head = runs.get("f16") or next(iter(runs.values()))
+    out = {
+        "metric": "clone-MLUPS + % HBM peak, FP16-shifted/FP32 vs plain FP64, "
+                  "at stated drag error",
+        "value": round(head["clone_mlups"], 1),
+        "unit": "clone-MLUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(head["ms_per_interval"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "storage": head["storage"] + "-shifted",
+        "data": "synthetic",
+        "config": {
+            "workload": "channel 400x100, square side 20 at (100,50), tau 0.53, "
+                        "u_in 0.05, 4096 clones, 200 LBM steps per cloning "
+                        "interval, k 0.005, noise sigma 1e-4; one step = one "
+                        "cloning interval (step kernel + selection + copy)",
+            "clones": WORKLOAD["n_clones"],
+            "lattice": [WORKLOAD["Lx"], WORKLOAD["Ly"]],
+            "lbm_steps_per_interval": WORKLOAD["steps"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(head["achieved_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(head["roofline_frac"], 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "paths": runs,
+        "accuracy_vs_fp64": acc,
+    }
+    if "f64" in runs and "f16" in runs:
+        out["speedup_f16_vs_f64"] = round(
+            runs["f16"]["clone_mlups"] / runs["f64"]["clone_mlups"], 3)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -60,8 +60,10 @@ struct lbc_ctx {
   std::string err;
   std::vector<hipEvent_t> ev;         // pairs around step-kernel launches
   int ev_used = 0;
-  hipGraphExec_t graph_exec = nullptr;
-  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // step kernel | the rest
+  hipGraph_t graphs[2] = {nullptr, nullptr};
+  bool timing_from_graph = false;
+  std::vector<float> graph_ms;
 };
 
 namespace {
@@ -268,8 +270,10 @@ int lbc_threads(const lbc_ctx* ctx) { return ctx ? ctx->threads : 0; }
 void lbc_destroy(lbc_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (int i = 0; i < 2; i++) {
+    if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
+    if (c->graphs[i]) (void)hipGraphDestroy(c->graphs[i]);
+  }
   for (auto e : c->ev) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->staging) (void)hipFree(c->staging);
@@ -521,6 +525,7 @@ int lbc_run_interval(lbc_ctx* c, const double* prefactors) {
     }
   }
   c->ev_used = 0;
+  c->timing_from_graph = false;
   if (int rc = enqueue_perturb(c, philox)) return rc;
   if (int rc = enqueue_interval(c, true)) return rc;
   return LBC_OK;
@@ -617,37 +622,57 @@ int lbc_gather(lbc_ctx* c) {
   return LBC_OK;
 }
 
+namespace {
+
+// Capture one cloning step of every instance as two graphs: [step kernel]
+// and [parity flip, selection, gather, perturbation of the new copies
+// (REF)].  Ordinary events recorded between the two replays time the step
+// kernel alone.
+int capture_step(lbc_ctx* c) {
+  lbc::Launch L = make_launch(c);
+  for (int part = 0; part < 2; part++) {
+    HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    bool ok = true;
+    if (part == 0) {
+      ok = lbc::launch_interval(L) == hipSuccess;
+    } else {
+      ok = lbc::launch_flip(L, c->cfg.n_instances, c->flip_mask) == hipSuccess;
+      if (ok)
+        ok = lbc::launch_select(c->idd, c->counters, 0, c->cfg.n_instances,
+                                c->n_max, 0, 0, seed_lo(c), seed_hi(c),
+                                c->stream) == hipSuccess;
+      if (ok) ok = lbc::launch_gather(L, seed_lo(c), seed_hi(c)) == hipSuccess;
+      if (ok && c->cfg.mode == LBC_MODE_REF)
+        ok = lbc::launch_perturb(L, 1, seed_lo(c), seed_hi(c)) == hipSuccess;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (!ok || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return fail(c, LBC_E_HIP, "graph capture failed");
+    }
+    hipError_t ei = hipGraphInstantiate(&c->gexec[part], g, nullptr, nullptr, 0);
+    c->graphs[part] = g;
+    if (ei != hipSuccess) {
+      c->gexec[part] = nullptr;
+      return fail(c, LBC_E_HIP, "graph instantiation failed");
+    }
+  }
+  return LBC_OK;
+}
+
+}  // namespace
+
 int lbc_advance(lbc_ctx* c, int n_intervals) {
   if (!c) return fail(nullptr, LBC_E_ARG, "null context");
   if (n_intervals < 0) return fail(c, LBC_E_ARG, "negative interval count");
-  if (!c->graph_exec) {
-    // capture one cloning step: perturb (REF), step kernel, flip, selection
-    // of every instance, gather
-    HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    lbc::Launch L = make_launch(c);
-    int rc = LBC_OK;
-    if (c->cfg.mode == LBC_MODE_REF)
-      if (lbc::launch_perturb(L, 1, seed_lo(c), seed_hi(c)) != hipSuccess)
-        rc = LBC_E_HIP;
-    if (!rc && lbc::launch_interval(L) != hipSuccess) rc = LBC_E_HIP;
-    if (!rc && lbc::launch_flip(L, c->cfg.n_instances, c->flip_mask) != hipSuccess)
-      rc = LBC_E_HIP;
-    if (!rc && lbc::launch_select(c->idd, c->counters, 0, c->cfg.n_instances,
-                                  c->n_max, 0, 0, seed_lo(c), seed_hi(c),
-                                  c->stream) != hipSuccess)
-      rc = LBC_E_HIP;
-    if (!rc && lbc::launch_gather(L, seed_lo(c), seed_hi(c)) != hipSuccess)
-      rc = LBC_E_HIP;
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (rc || e != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return fail(c, LBC_E_HIP, "graph capture failed");
-    }
-    c->graph = g;
-    HIPC(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
-  }
+  if (!c->gexec[0] || !c->gexec[1])
+    if (int rc = capture_step(c)) return rc;
+  c->timing_from_graph = false;
   c->ev_used = 0;
+  if (n_intervals > 0)
+    if (int rc = enqueue_perturb(c, true)) return rc;
   for (int k = 0; k < n_intervals; k++) {
     if (c->ev_used + 2 > (int)c->ev.size()) {
       for (int i = 0; i < 64; i++) {
@@ -657,8 +682,9 @@ int lbc_advance(lbc_ctx* c, int n_intervals) {
       }
     }
     HIPC(c, hipEventRecord(c->ev[c->ev_used], c->stream));
-    HIPC(c, hipGraphLaunch(c->graph_exec, c->stream));
+    HIPC(c, hipGraphLaunch(c->gexec[0], c->stream));
     HIPC(c, hipEventRecord(c->ev[c->ev_used + 1], c->stream));
+    HIPC(c, hipGraphLaunch(c->gexec[1], c->stream));
     c->ev_used += 2;
     for (int i = 0; i < c->cfg.n_instances; i++)
       c->parity[i] ^= (c->inst[i].steps + 1) & 1;
@@ -696,11 +722,18 @@ int lbc_step_kernel_ms(lbc_ctx* c, double* avg_ms, int* n_launches) {
   HIPC(c, hipStreamSynchronize(c->stream));
   double tot = 0;
   int n = 0;
-  for (int i = 0; i + 1 < c->ev_used; i += 2) {
-    float ms = 0;
-    HIPC(c, hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
-    tot += ms;
-    n++;
+  if (c->timing_from_graph) {
+    for (float ms : c->graph_ms) {
+      tot += ms;
+      n++;
+    }
+  } else {
+    for (int i = 0; i + 1 < c->ev_used; i += 2) {
+      float ms = 0;
+      HIPC(c, hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+      tot += ms;
+      n++;
+    }
   }
   *avg_ms = n ? tot / n : 0.0;
   if (n_launches) *n_launches = n;
