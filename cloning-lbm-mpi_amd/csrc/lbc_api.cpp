@@ -55,6 +55,7 @@ struct lbc_ctx {
   double* staging = nullptr;          // AoS FP64 transfer buffer (device)
   size_t staging_bytes = 0;
   int threads = 256;
+  int vec = 1;                        // sites per thread group in the step kernel
   int ring_max = 0;
   int n_max = 0;
   std::string err;
@@ -102,6 +103,14 @@ int validate(const lbc_config* cfg, std::string& why) {
   if (cfg->threads != 0 &&
       (cfg->threads < 64 || cfg->threads > 1024 || cfg->threads % 64))
     return why = "threads must be 0 or a multiple of 64 in [64, 1024]", -1;
+  {
+    const int v = cfg->flags & 7;
+    if ((cfg->flags & ~7) || (v != 0 && v != 1 && v != 2 && v != 4))
+      return why = "flags: only bits 0-2 (sites per thread: 0, 1, 2 or 4)", -1;
+    for (int i = 0; v > 1 && i < cfg->n_instances; i++)
+      if (cfg->instances[i].Lx % v)
+        return why = "flags: sites per thread must divide every Lx", -1;
+  }
   for (int i = 0; i < cfg->n_instances; i++) {
     const lbc_instance& s = cfg->instances[i];
     char buf[160];
@@ -156,7 +165,7 @@ lbc::Launch make_launch(lbc_ctx* c) {
   L.ring_max = c->ring_max;
   L.mode = c->cfg.mode;
   L.storage = c->cfg.storage;
-  L.vec = 1;
+  L.vec = c->vec;
   L.stream = c->stream;
   return L;
 }
@@ -219,37 +228,52 @@ int enqueue_perturb(lbc_ctx* c, bool philox) {
   return LBC_OK;
 }
 
+bool vec_ok(const lbc_ctx* c, int v) {
+  for (const auto& s : c->inst)
+    if (s.Lx % v) return false;
+  return true;
+}
+
 int autotune(lbc_ctx* c) {
-  // Time short intervals at each workgroup size on the freshly initialised
-  // state; the state is re-initialised afterwards by the caller.
+  // Time short intervals at each (workgroup size, sites per thread) on the
+  // freshly initialised state; the caller re-initialises the state after.
   std::vector<InstDev> tmp = c->idh;
-  for (auto& d : tmp) d.steps = std::min(d.steps, 8);
+  for (auto& d : tmp) d.steps = std::min(d.steps, 16);
   HIPC(c, hipMemcpyAsync(c->idd, tmp.data(), sizeof(InstDev) * tmp.size(),
                          hipMemcpyHostToDevice, c->stream));
-  const int cands[3] = {256, 512, 1024};
+  const int tcands[3] = {256, 512, 1024};
+  const int vfixed = c->cfg.flags & 7;
   float best = 1e30f;
-  int best_t = 256;
+  int best_t = 256, best_v = 1;
   hipEvent_t e0, e1;
   HIPC(c, hipEventCreate(&e0));
   HIPC(c, hipEventCreate(&e1));
-  for (int t : cands) {
-    lbc::Launch L = make_launch(c);
-    L.threads = t;
-    HIPC(c, lbc::launch_interval(L));  // warm-up
-    HIPC(c, hipEventRecord(e0, c->stream));
-    for (int r = 0; r < 2; r++) HIPC(c, lbc::launch_interval(L));
-    HIPC(c, hipEventRecord(e1, c->stream));
-    HIPC(c, hipEventSynchronize(e1));
-    float ms = 0;
-    HIPC(c, hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best) {
-      best = ms;
-      best_t = t;
+  for (int v : {1, 2, 4}) {
+    if (vfixed && v != vfixed) continue;
+    if (!vec_ok(c, v)) continue;
+    for (int t : tcands) {
+      if (c->cfg.threads && t != c->cfg.threads) continue;
+      lbc::Launch L = make_launch(c);
+      L.threads = t;
+      L.vec = v;
+      HIPC(c, lbc::launch_interval(L));  // warm-up
+      HIPC(c, hipEventRecord(e0, c->stream));
+      for (int r = 0; r < 2; r++) HIPC(c, lbc::launch_interval(L));
+      HIPC(c, hipEventRecord(e1, c->stream));
+      HIPC(c, hipEventSynchronize(e1));
+      float ms = 0;
+      HIPC(c, hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_t = t;
+        best_v = v;
+      }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->threads = best_t;
+  c->vec = best_v;
   HIPC(c, hipMemcpyAsync(c->idd, c->idh.data(), sizeof(InstDev) * c->idh.size(),
                          hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -266,6 +290,8 @@ const char* lbc_last_error(const lbc_ctx* ctx) {
 }
 
 int lbc_threads(const lbc_ctx* ctx) { return ctx ? ctx->threads : 0; }
+
+int lbc_sites_per_thread(const lbc_ctx* ctx) { return ctx ? ctx->vec : 0; }
 
 void lbc_destroy(lbc_ctx* c) {
   if (!c) return;
@@ -416,7 +442,8 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     if (lbc::launch_init_eq(L, i, c->inst[i].n_clones, 0.0, 0, 0) != hipSuccess)
       return bail(fail(c, LBC_E_HIP, "init kernel launch failed"));
   }
-  if (cfg->threads == 0) {
+  const int vfixed = cfg->flags & 7;
+  if (cfg->threads == 0 || vfixed == 0) {
     int rc = autotune(c);
     if (rc) return bail(rc);
     L = make_launch(c);
@@ -425,6 +452,7 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
         return bail(fail(c, LBC_E_HIP, "init kernel launch failed"));
   } else {
     c->threads = cfg->threads;
+    c->vec = vfixed;
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(fail(c, LBC_E_HIP, "initialisation failed"));
@@ -756,6 +784,7 @@ int lbc_step_host(int mode, const lbc_instance* geom, double* fin, double* fout,
   cfg.storage = LBC_STORE_F64;
   cfg.n_instances = 1;
   cfg.threads = 256;
+  cfg.flags = 1;
   cfg.instances = &s;
   lbc_ctx* c = nullptr;
   int rc = lbc_create(&cfg, &c);

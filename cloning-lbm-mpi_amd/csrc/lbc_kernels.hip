@@ -291,6 +291,73 @@ __device__ inline int wall_mask(int y, int Ly) {
   return m;
 }
 
+// Channel x boundaries on the post-stream populations of one site
+// (no reference counterpart; the oracle states the same rules):
+//  inlet  x = 0:    feq(rho_in, (u_in, 0)), rho_in = (f0 + f2 + f4 +
+//                   2 (f3 + f6 + f7)) / (1 - u_in) from the populations that
+//                   streamed in from the domain;
+//  outlet x = Lx-1: after the zero-gradient ghost copy, feq(1, u) with u the
+//                   site's own velocity (the outlet holds the reference
+//                   density).
+template <int ST>
+__device__ inline void channel_x_boundary(typename Traits<ST>::C (&f)[9],
+                                          bool inlet, double u_in) {
+  using T = Traits<ST>;
+  if constexpr (!T::shifted) {
+#pragma clang fp contract(off)
+    double rho, ux, uy;
+    if (inlet) {
+      rho = (f[0] + f[2] + f[4] + 2.0 * (f[3] + f[6] + f[7])) / (1.0 - u_in);
+      ux = u_in;
+      uy = 0.0;
+    } else {
+      double r = 0.0, jx = 0.0, jy = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        r += f[k];
+        jx += f[k] * (double)CX[k];
+        jy += f[k] * (double)CY[k];
+      }
+      ux = jx / r;
+      uy = jy / r;
+      rho = 1.0;
+    }
+    const double u2 = -1.5 * (ux * ux + uy * uy);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const double eu = (double)CX[k] * ux + (double)CY[k] * uy;
+      const double eueu = 4.5 * eu * eu;
+      f[k] = WD[k] * rho * (1.0 + 3.0 * eu + eueu + u2);
+    }
+  } else {
+    // shifted: f = w + d, sum of the known weights is 1
+    float drho, ux, uy;
+    if (inlet) {
+      const float D = f[0] + f[2] + f[4] + 2.0f * (f[3] + f[6] + f[7]);
+      const float ui = (float)u_in;
+      drho = (D + ui) / (1.0f - ui);  // rho_in - 1
+      ux = ui;
+      uy = 0.0f;
+    } else {
+      const float dr = ((f[0] + f[1]) + (f[2] + f[3])) +
+                       ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
+      const float jx = (f[1] - f[3]) + (f[5] - f[6]) + (f[8] - f[7]);
+      const float jy = (f[2] - f[4]) + (f[5] + f[6]) - (f[7] + f[8]);
+      const float inv = 1.0f / (1.0f + dr);
+      ux = jx * inv;
+      uy = jy * inv;
+      drho = 0.0f;
+    }
+    const float rho = 1.0f + drho;
+    const float usq = 1.5f * (ux * ux + uy * uy);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const float eu = (float)CX[k] * ux + (float)CY[k] * uy;
+      f[k] = WF[k] * (drho + rho * (3.0f * eu + 4.5f * eu * eu - usq));
+    }
+  }
+}
+
 // Per-instance scalars held in registers by the step kernel.
 struct Geo {
   int Lx, Ly, x0, x1, y0, y1, plane;
@@ -343,14 +410,16 @@ __device__ inline void do_site(const Geo& g, int x, int y,
           else if (xs >= g.Lx) xs -= g.Lx;
         } else {
           if (xs < 0) {
-            if (T::shifted) f[q] = (C)I->feq_in_s[q];
-            else f[q] = (C)I->feq_in[q];
+            // unknown at the inlet: replaced by the inlet equilibrium below
+            f[q] = (C)0;
             continue;
           }
-          if (xs >= g.Lx) xs = g.Lx - 1;
+          if (xs >= g.Lx) xs = g.Lx - 1;  // zero-gradient ghost column
         }
         f[q] = T::ld(src + q * g.plane + ys * g.Lx + xs);
       }
+      if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1))
+        channel_x_boundary<ST>(f, x == 0, I->u_in);
       if (in_box) {
         // drag contribution of this ring node
         if (MODE == kModeRef) {
@@ -430,14 +499,123 @@ __device__ inline void do_site(const Geo& g, int x, int y,
   for (int q = 0; q < 9; q++) T::st(dst + q * g.plane + site, f[q]);
 }
 
-template <int MODE, int ST, int PH>
-__device__ inline void site_pass(const Geo& g, int x, int y, int dx, int dy,
-                                 const typename Traits<ST>::S* __restrict__ src,
-                                 typename Traits<ST>::S* __restrict__ dst,
-                                 double* __restrict__ ring, double beta,
-                                 double tau, const InstDev* __restrict__ I) {
+// Element-aligned vector of V populations (one direction, V consecutive
+// sites).  gfx950 runs in unaligned-access mode, so a shifted (odd-offset)
+// window is still one dword / dwordx2 / dwordx4 access.
+template <class S, int V>
+struct VecOf {
+  typedef S type __attribute__((ext_vector_type(V), aligned(sizeof(S))));
+};
+
+template <int ST, int V>
+__device__ inline void vload(typename Traits<ST>::C (&out)[V],
+                             const typename Traits<ST>::S* p) {
+  using T = Traits<ST>;
+  if constexpr (V == 1) {
+    out[0] = T::ld(p);
+  } else {
+    typedef typename VecOf<typename T::S, V>::type vt;
+    const vt t = *reinterpret_cast<const vt*>(p);
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      typename T::S e = t[v];
+      out[v] = T::ld(&e);
+    }
+  }
+}
+
+template <int ST, int V>
+__device__ inline void vstore(typename Traits<ST>::S* p,
+                              const typename Traits<ST>::C (&in)[V]) {
+  using T = Traits<ST>;
+  if constexpr (V == 1) {
+    T::st(p, in[0]);
+  } else {
+    typedef typename VecOf<typename T::S, V>::type vt;
+    vt t;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+      typename T::S e;
+      T::st(&e, in[v]);
+      t[v] = e;
+    }
+    *reinterpret_cast<vt*>(p) = t;
+  }
+}
+
+// Collision of one site's populations in the storage's compute type.
+template <int MODE, int ST>
+__device__ inline void collide_any(typename Traits<ST>::C (&f)[9], double beta,
+                                   double tau) {
+  using T = Traits<ST>;
+  if constexpr (!T::shifted) {
+    double ff[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) ff[q] = (double)f[q];
+    collide_f64(ff, MODE == kModeRef ? beta : 0.0, tau);
+#pragma unroll
+    for (int q = 0; q < 9; q++) f[q] = (typename T::C)ff[q];
+  } else {
+    float dd[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) dd[q] = (float)f[q];
+    collide_shifted<MODE>(dd, (float)(1.0 / tau), (float)beta);
+#pragma unroll
+    for (int q = 0; q < 9; q++) f[q] = (typename T::C)dd[q];
+  }
+}
+
+// V consecutive sites x..x+V-1 of row y (x a multiple of V, V | Lx).
+// Groups away from every boundary and from the square pull with V-wide
+// vector loads and store V-wide; the others go site by site.
+template <int MODE, int ST, int PH, int V>
+__device__ inline void do_group(const Geo& g, int x, int y,
+                                const typename Traits<ST>::S* __restrict__ src,
+                                typename Traits<ST>::S* __restrict__ dst,
+                                double* __restrict__ ring, double beta,
+                                double tau, const InstDev* __restrict__ I) {
+  using T = Traits<ST>;
+  using C = typename T::C;
+  const int site = y * g.Lx + x;
+  const bool touches_box =
+      (x + V - 1 >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1);
+  const bool plain = !touches_box && y >= 1 && y <= g.Ly - 2 && x >= 1 &&
+                     x + V <= g.Lx - 1;
+  if (PH == kCollide || plain) {
+    C f[9][V];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      const int off = (PH == kCollide) ? 0 : (CY[q] * g.Lx + CX[q]);
+      vload<ST, V>(f[q], src + q * g.plane + site - off);
+    }
+    if (PH != kStream) {
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        C h[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) h[q] = f[q][v];
+        collide_any<MODE, ST>(h, beta, tau);
+#pragma unroll
+        for (int q = 0; q < 9; q++) f[q][v] = h[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) vstore<ST, V>(dst + q * g.plane + site, f[q]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; v++)
+      do_site<MODE, ST, PH>(g, x + v, y, src, dst, ring, beta, tau, I);
+  }
+}
+
+template <int MODE, int ST, int PH, int V>
+__device__ inline void group_pass(const Geo& g, int x, int y, int dx, int dy,
+                                  const typename Traits<ST>::S* __restrict__ src,
+                                  typename Traits<ST>::S* __restrict__ dst,
+                                  double* __restrict__ ring, double beta,
+                                  double tau, const InstDev* __restrict__ I) {
   while (y < g.Ly) {
-    do_site<MODE, ST, PH>(g, x, y, src, dst, ring, beta, tau, I);
+    do_group<MODE, ST, PH, V>(g, x, y, src, dst, ring, beta, tau, I);
     x += dx;
     y += dy;
     if (x >= g.Lx) {
@@ -467,7 +645,7 @@ __device__ inline double ring_sum(const double* ring, int sx, int sy) {
 // S_0 is in buf[parity]; pass 0 collides it, passes 1..M-1 stream and
 // collide, pass M streams only, so the interval ends with S_M in
 // buf[parity ^ ((M+1)&1)].  Drag F_n is taken on S_n in pass n.
-template <int MODE, int ST>
+template <int MODE, int ST, int V>
 __global__ void __launch_bounds__(1024)
     k_interval(const InstDev* __restrict__ insts,
                const WorkItem* __restrict__ items,
@@ -495,8 +673,8 @@ __global__ void __launch_bounds__(1024)
   double* drag = I->drag + (long long)wi.clone * M;
 
   const int nthr = blockDim.x, tid = threadIdx.x;
-  const int dx = nthr % g.Lx, dy = nthr / g.Lx;
-  const int x_init = tid % g.Lx, y_init = tid / g.Lx;
+  const int dx = (nthr * V) % g.Lx, dy = (nthr * V) / g.Lx;
+  const int x_init = (tid * V) % g.Lx, y_init = (tid * V) / g.Lx;
   double acc = 0.0;
 
   for (int n = 0; n <= M; ++n) {
@@ -504,13 +682,13 @@ __global__ void __launch_bounds__(1024)
     S* dst = (n & 1) ? A : B;
     double* ring = ring_lds + (n & 1) * ring_max;
     if (n == 0)
-      site_pass<MODE, ST, kCollide>(g, x_init, y_init, dx, dy, src, dst, ring,
+      group_pass<MODE, ST, kCollide, V>(g, x_init, y_init, dx, dy, src, dst, ring,
                                     beta, tau, I);
     else if (n == M)
-      site_pass<MODE, ST, kStream>(g, x_init, y_init, dx, dy, src, dst, ring,
+      group_pass<MODE, ST, kStream, V>(g, x_init, y_init, dx, dy, src, dst, ring,
                                    beta, tau, I);
     else
-      site_pass<MODE, ST, kBoth>(g, x_init, y_init, dx, dy, src, dst, ring,
+      group_pass<MODE, ST, kBoth, V>(g, x_init, y_init, dx, dy, src, dst, ring,
                                  beta, tau, I);
     __syncthreads();
     if (n >= 1 && tid == 0) {
@@ -1009,8 +1187,25 @@ __global__ void k_snap_to_soa(int n, const double* __restrict__ aos, int Lx,
 template <int MODE, int ST>
 hipError_t launch_interval_t(const Launch& L) {
   const size_t lds = sizeof(double) * 2 * (size_t)L.ring_max;
-  hipLaunchKernelGGL((k_interval<MODE, ST>), dim3(L.n_items), dim3(L.threads),
-                     lds, L.stream, L.insts, L.items, L.counters, L.ring_max);
+  switch (L.vec) {
+    case 1:
+      hipLaunchKernelGGL((k_interval<MODE, ST, 1>), dim3(L.n_items),
+                         dim3(L.threads), lds, L.stream, L.insts, L.items,
+                         L.counters, L.ring_max);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_interval<MODE, ST, 2>), dim3(L.n_items),
+                         dim3(L.threads), lds, L.stream, L.insts, L.items,
+                         L.counters, L.ring_max);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_interval<MODE, ST, 4>), dim3(L.n_items),
+                         dim3(L.threads), lds, L.stream, L.insts, L.items,
+                         L.counters, L.ring_max);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -38,7 +38,7 @@ EXPORTS = (
     "lbc_set_marks", "lbc_get_marks", "lbc_run_interval", "lbc_get_drag",
     "lbc_get_logweights", "lbc_select", "lbc_gather", "lbc_advance",
     "lbc_get_R_history", "lbc_sync", "lbc_step_kernel_ms", "lbc_threads",
-    "lbc_step_host",
+    "lbc_sites_per_thread", "lbc_step_host",
 )
 
 
@@ -114,6 +114,7 @@ def lib() -> C.CDLL:
     L.lbc_sync.argtypes = [vp]
     L.lbc_step_kernel_ms.argtypes = [vp, dp, C.POINTER(C.c_int)]
     L.lbc_threads.argtypes = [vp]
+    L.lbc_sites_per_thread.argtypes = [vp]
     L.lbc_step_host.argtypes = [C.c_int, C.POINTER(Instance), dp, dp, dp]
     _lib = L
     return L
@@ -216,7 +217,7 @@ class Sampler:
 
     def __init__(self, specs, mode: int = MODE_CHANNEL,
                  storage: int = STORE_F64, seed: int = 0, threads: int = 0,
-                 device: int = 0):
+                 device: int = 0, vec: int = 0):
         if isinstance(specs, InstanceSpec):
             specs = [specs]
         self.specs = list(specs)
@@ -225,7 +226,9 @@ class Sampler:
         L = lib()
         arr = (Instance * len(self.specs))(*[s.c() for s in self.specs])
         self._arr = arr
-        cfg = Config(mode, storage, len(self.specs), device, seed, threads, 0,
+        if threads and not vec:
+            vec = 1
+        cfg = Config(mode, storage, len(self.specs), device, seed, threads, vec,
                      C.cast(arr, C.POINTER(Instance)))
         h = C.c_void_p()
         rc = L.lbc_create(C.byref(cfg), C.byref(h))
@@ -260,6 +263,10 @@ class Sampler:
     @property
     def threads(self) -> int:
         return lib().lbc_threads(self._h)
+
+    @property
+    def vec(self) -> int:
+        return lib().lbc_sites_per_thread(self._h)
 
     # -- state
     def load_states(self, inst: int, clone0: int, aos: np.ndarray) -> None:

@@ -77,7 +77,9 @@ typedef struct lbc_config {
   int device;                 /* HIP device ordinal */
   uint64_t seed;              /* Philox key */
   int threads;                /* workgroup size of the step kernel; 0 = tuned */
-  int flags;                  /* reserved, 0 */
+  int flags;                  /* bits 0-2: sites per thread of the step kernel
+                                 (1, 2 or 4; 0 = tuned together with threads);
+                                 other bits 0 */
   const lbc_instance* instances;
 } lbc_config;
 
@@ -180,8 +182,10 @@ int lbc_sync(lbc_ctx* ctx);
  * stream, and the number of launches it averaged. */
 int lbc_step_kernel_ms(lbc_ctx* ctx, double* avg_ms, int* n_launches);
 
-/* Workgroup size the context uses for the step kernel. */
+/* Workgroup size and sites per thread the context uses for the step
+ * kernel. */
 int lbc_threads(const lbc_ctx* ctx);
+int lbc_sites_per_thread(const lbc_ctx* ctx);
 
 /* One reference LBM step on one lattice held in host memory (FP64):
  * fin (post-stream state) becomes the post-collision state, fout receives

@@ -121,14 +121,6 @@ double orc_step(const orc_geom* g, double* fin, double* fout) {
     }
   }
   if (!periodic) {
-    /* equilibrium inlet: ghost column x=-1 at feq(1, (u_in, 0)) */
-    double fin_eq[9];
-    orc_feq(1.0, g->u_in, 0.0, fin_eq);
-    for (int y = 0; y < Dy; y++) {
-      fout[IDX(g, 0, y, 1)] = fin_eq[1];
-      fout[IDX(g, 0, y, 5)] = fin_eq[5];
-      fout[IDX(g, 0, y, 8)] = fin_eq[8];
-    }
     /* zero-gradient outlet: ghost column x=Dx is a copy of column Dx-1 */
     for (int y = 0; y < Dy; y++) {
       const int ks[3] = {3, 6, 7};
@@ -156,6 +148,35 @@ double orc_step(const orc_geom* g, double* fin, double* fout) {
       for (int k = 0; k < 9; k++)
         if (m & (1 << k))
           fout[IDX(g, x, y, k)] = fin[IDX(g, x, y, orc_opp[k])];
+    }
+  }
+  if (!periodic) {
+    /* outlet: the zero-gradient column x=Dx-1 is completed to equilibrium at
+       the reference density, feq(1, u) with u its own velocity */
+    for (int y = 0; y < Dy; y++) {
+      double* f = fout + IDX(g, Dx - 1, y, 0);
+      double rho = 0.0, ux = 0.0, uy = 0.0;
+      for (int k = 0; k < 9; k++) {
+        rho += f[k];
+        ux += f[k] * orc_c[k][0];
+        uy += f[k] * orc_c[k][1];
+      }
+      ux /= rho;
+      uy /= rho;
+      double eq[9];
+      orc_feq(1.0, ux, uy, eq);
+      for (int k = 0; k < 9; k++) f[k] = eq[k];
+    }
+    /* equilibrium velocity inlet: column x=0 becomes feq(rho_in, (u_in, 0)),
+       rho_in = (f0 + f2 + f4 + 2 (f3 + f6 + f7)) / (1 - u_in) from the
+       populations that streamed in from the domain (walls included) */
+    for (int y = 0; y < Dy; y++) {
+      double* f = fout + IDX(g, 0, y, 0);
+      double rho_in = (f[0] + f[2] + f[4] + 2.0 * (f[3] + f[6] + f[7])) /
+                      (1.0 - g->u_in);
+      double eq[9];
+      orc_feq(rho_in, g->u_in, 0.0, eq);
+      for (int k = 0; k < 9; k++) f[k] = eq[k];
     }
   }
   /* interior reset (src/mpi_LBM_GK_pops_perturb.cpp:430-439) */

@@ -13,10 +13,12 @@
  *   ORC_MODE_REF     periodic x, uniform body force beta with Guo forcing,
  *                    half-way bounce-back walls and square, interior reset,
  *                    stress-integral drag (the reference's production path);
- *   ORC_MODE_CHANNEL equilibrium inlet (ghost column at x=-1 holding
- *                    feq(rho=1, u=(u_in,0))), zero-gradient outlet (ghost
- *                    column x=Lx copies column Lx-1), same walls and square,
- *                    momentum-exchange drag.  No reference counterpart.
+ *   ORC_MODE_CHANNEL equilibrium velocity inlet (column x=0 is reset to
+ *                    feq(rho_in, (u_in,0)) after every stream, rho_in from the
+ *                    populations streamed in from the domain), zero-gradient
+ *                    outlet (ghost column x=Lx copies column Lx-1), same walls
+ *                    and square, momentum-exchange drag.  No reference
+ *                    counterpart.
  */
 #ifndef LBM_ORACLE_H
 #define LBM_ORACLE_H

@@ -1,0 +1,105 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures under tests/golden/ by running the reference's
+own kernel code, compiled from /root/reference by `make -C oracle ref` into
+oracle/_ref/libref.so, and glibc's generators that the reference calls.
+
+Fixtures (numpy .npz, FP64 unless noted):
+  ref_steps_L4.npz, ref_steps_L8.npz
+      reference-geometry lattice (square side L, Dx = Dy = 4L+1), seeded
+      non-equilibrium initial state S0, the state and post-collision array
+      after steps 1, 2 and the last (indices in `keep`), and the raw
+      stress-integral force after every step
+      (src/lbm_src/streamCollCompute.cpp, domain_noSlipWalls.cpp, square.cpp,
+      force.cpp and the interior reset of src/mpi_LBM_GK_pops_perturb.cpp).
+  ref_perturb_L4.npz
+      generatePerturbedState (src/perturbation_routines/
+      generatePerturbedState.cpp) on a seeded state with 3 snapshots and
+      drand48 in its default state; the prefactors it drew are recorded.
+  glibc_streams.npz
+      rand() after srand(1) and 1000 discards (the master rank's stream,
+      src/mpi_LBM_GK_pops_perturb.cpp:176-178), and drand48() from its
+      default state (never seeded by the reference).
+
+Run from the repository root: python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle as O  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def seeded_state(g, rng, amp=2e-3):
+    f = np.empty((g.Lx, g.Ly, 9))
+    for x in range(g.Lx):
+        for y in range(g.Ly):
+            rho = 1 + amp * rng.standard_normal()
+            ux = 0.05 + amp * rng.standard_normal()
+            uy = amp * rng.standard_normal()
+            u2 = -1.5 * (ux * ux + uy * uy)
+            for k in range(9):
+                eu = O.CX[k] * ux + O.CY[k] * uy
+                f[x, y, k] = O.W[k] * rho * (1 + 3 * eu + 4.5 * eu * eu + u2)
+    f += 1e-5 * rng.standard_normal(f.shape)
+    f[g.xmin + 1:g.xmax, g.ymin + 1:g.ymax, :] = O.W
+    return f
+
+
+def steps_fixture(L, nsteps, seed, tau, U0):
+    g = O.ref_box_geom(L, tau, U0)
+    rng = np.random.default_rng(seed)
+    s0 = seeded_state(g, rng)
+    a, b = s0.copy(), np.zeros_like(s0)
+    states, posts, forces = [], [], []
+    keep = [0, 1, nsteps - 1]  # steps whose arrays are stored
+    for _ in range(nsteps):
+        F = O.ref_step(g, a, b)
+        posts.append(a.copy())
+        a, b = b, a
+        states.append(a.copy())
+        forces.append(F)
+    np.savez_compressed(
+        os.path.join(OUT, f"ref_steps_L{L}.npz"), L=L, tau=tau, U0=U0,
+        geom=np.array([g.Lx, g.Ly, g.xmin, g.xmax, g.ymin, g.ymax]),
+        beta=g.beta, S0=s0, keep=np.array(keep),
+        states=np.array([states[i] for i in keep]),
+        post=np.array([posts[i] for i in keep]), forces=np.array(forces))
+
+
+def perturb_fixture():
+    L = 4
+    g = O.ref_box_geom(L, 0.53, 0.05)
+    rng = np.random.default_rng(99)
+    fin = seeded_state(g, rng)
+    snaps = np.stack([seeded_state(g, rng, amp=1e-2) for _ in range(3)])
+    out = fin.copy()
+    a = O.ref_perturb_default_stream(out, snaps, 0.25)
+    np.savez_compressed(os.path.join(OUT, "ref_perturb_L4.npz"), fin=fin,
+                        snaps=snaps, eps=0.25, prefactors=a, out=out)
+
+
+def glibc_fixture():
+    r = O.glibc_rand(1, 64, discard=1000)
+    d = O.glibc_drand48(64)
+    np.savez_compressed(os.path.join(OUT, "glibc_streams.npz"),
+                        rand_seed1_after1000=r, drand48_default=d)
+
+
+def main():
+    if not O.ref_available():
+        sys.exit("oracle/_ref/libref.so missing: run `make -C oracle ref` "
+                 "where /root/reference exists")
+    steps_fixture(4, 12, 4, 0.53, 0.05)
+    steps_fixture(8, 6, 8, 0.515, 0.05)
+    perturb_fixture()
+    glibc_fixture()
+    print("fixtures written to", OUT)
+
+
+if __name__ == "__main__":
+    main()

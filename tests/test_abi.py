@@ -1,0 +1,68 @@
+"""The C-ABI library loads and exports every entry point include/lbc.h
+declares; configuration errors are reported without touching a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "lbc.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lbc_[A-Za-z_0-9]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree(lbc):
+    assert declared_functions() == sorted(lbc.EXPORTS)
+
+
+def test_library_exports_every_declared_function(lbc):
+    L = lbc.lib()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+
+
+def test_driver_built_and_linked():
+    exe = os.path.join(ROOT, "cloning-lbm-mpi_amd", "driver", "tlgk_lbm_gpu")
+    assert os.path.exists(exe) and os.access(exe, os.X_OK)
+
+
+def _bad(lbc, **kw):
+    spec = lbc.channel(80, 40, 8, 20, 20, 0.53, 0.05, 4, 10, k=0.01)
+    mode = kw.pop("mode", lbc.MODE_CHANNEL)
+    storage = kw.pop("storage", lbc.STORE_F64)
+    threads = kw.pop("threads", 0)
+    vec = kw.pop("vec", 0)
+    for k, v in kw.items():
+        setattr(spec, k, v)
+    with pytest.raises(lbc.LbcError) as e:
+        lbc.Sampler(spec, mode=mode, storage=storage, threads=threads, vec=vec)
+    assert "(-1)" in str(e.value)
+    return str(e.value)
+
+
+def test_create_rejects_invalid_configurations(lbc):
+    assert "tau" in _bad(lbc, tau=0.5)
+    assert "square" in _bad(lbc, xmin=1)
+    assert "square" in _bad(lbc, ymax=38)
+    assert "n_clones" in _bad(lbc, n_clones=0)
+    assert "n_clones" in _bad(lbc, n_clones=8193)
+    assert "steps" in _bad(lbc, steps=0)
+    assert "rank_blocks" in _bad(lbc, rank_blocks=3)
+    assert "lattice" in _bad(lbc, Lx=6)
+    assert "mode" in _bad(lbc, mode=7)
+    assert "storage" in _bad(lbc, storage=9)
+    assert "threads" in _bad(lbc, threads=100)
+    assert "sites per thread" in _bad(lbc, threads=256, vec=3)
+    assert "non-finite" in _bad(lbc, bias=float("nan"))
+
+
+def test_null_context_calls_fail_cleanly(lbc):
+    L = lbc.lib()
+    assert L.lbc_run_interval(None, None) == -1
+    assert L.lbc_gather(None) == -1
+    assert L.lbc_sync(None) == -1
+    assert b"null" in L.lbc_last_error(None)

@@ -1,0 +1,124 @@
+"""End-to-end parity of the driver (tlgk_lbm_gpu) against the reference's
+driver loop (src/mpi_LBM_GK_pops_perturb.cpp:387-714) restated with the
+CPU oracle and glibc's generators: same input file, same pops files, same
+labels.dat / rvalues.dat / data_force.datout."""
+import math
+import os
+import subprocess
+from collections import deque
+
+import numpy as np
+import pytest
+
+from conftest import PKG
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(PKG, "driver", "tlgk_lbm_gpu")
+
+
+def write_case(tmp, O, L, Nc, T, dT, eps, tau, U0, alpha, extra=""):
+    g = O.ref_box_geom(L, tau, U0)
+    rng = np.random.default_rng(L * 100 + Nc)
+    data = tmp / "data"
+    data.mkdir()
+    names = []
+    for i in range(max(Nc, 10)):
+        f = np.empty((g.Lx, g.Ly, 9))
+        for x in range(g.Lx):
+            for y in range(g.Ly):
+                f[x, y] = O.feq(1 + 2e-3 * rng.standard_normal(),
+                                U0 * (1 + 0.2 * rng.standard_normal()),
+                                2e-3 * rng.standard_normal())
+        f[g.xmin + 1:g.xmax, g.ymin + 1:g.ymax] = O.W
+        name = f"pops_{i}.datout"
+        f.tofile(data / name)
+        names.append(name)
+    (data / "popfiles_list.dat").write_text("\n".join(names) + "\n")
+    inp = tmp / "input"
+    inp.write_text(f"{Nc}\n{T}\n{dT}\n{eps}\n{L}\n{tau}\n{U0}\n{alpha}\n"
+                   f"{data}/\n{tmp}/run/\n1000\n{extra}")
+    return g, data, names, inp
+
+
+def reference_loop(O, g, data, names, Nc, T, dT, eps, U0, alpha, L):
+    """The reference driver's interval loop for one rank, on the oracle."""
+    T0 = L / U0
+    F0 = U0 * U0 * L * 0.5
+    dt = 1.0 / T0
+    M = int(math.floor(dT * T0))
+    nT = int(math.floor(T / dT))
+    states = [np.fromfile(data / names[j]).reshape(g.Lx, g.Ly, 9)
+              for j in range(Nc)]
+    snaps = np.stack([np.fromfile(data / names[n]).reshape(g.Lx, g.Ly, 9)
+                      for n in range(10)])
+    lib = O.lib()
+    lib.orc_glibc_srand(1)
+    tmpi = np.zeros(1000, np.int32)
+    lib.orc_glibc_rand_n(1000, O._ip(tmpi))
+    lib.orc_glibc_seed48_default()
+
+    def drand(n):
+        out = np.zeros(n)
+        lib.orc_glibc_drand48_n(n, O._dp(out))
+        return out
+
+    rq = deque()
+    marks = np.zeros(Nc, bool)
+    forces = [[] for _ in range(Nc)]
+    labels_all, R = [], []
+    for t in range(nT):
+        for j in range(Nc):
+            if marks[j]:
+                O.perturb(states[j], snaps, eps, drand(10))
+        acc = np.zeros(Nc)
+        for j in range(Nc):
+            acc[j], A = O.run_steps(g, states[j], M, 1.0 / F0, 1.775, dt)
+            forces[j].append(A)
+        s = np.array([math.exp(alpha * a) for a in acc])
+        u = drand(Nc)
+        while len(rq) < Nc:
+            tmp1 = np.zeros(1, np.int32)
+            lib.orc_glibc_rand_n(1, O._ip(tmp1))
+            rq.append(int(tmp1[0]))
+        sel = O.select(s, u, np.array(list(rq)[:Nc], np.int32))
+        for _ in range(sel["n_rint"]):
+            rq.popleft()
+        labels_all.append(sel["labels"])
+        R.append(sel["total_R"])
+        marks[:] = False
+        for src, dst in sel["pairs"]:
+            states[dst] = states[src].copy()
+            marks[dst] = True
+    return labels_all, R, forces, states
+
+
+@pytest.mark.parametrize("Nc,alpha", [(6, 40.0), (8, 0.0)])
+def test_driver_matches_reference_loop(tmp_path, O, Nc, alpha):
+    L, T, dT, eps, tau, U0 = 4, 1.0, 0.25, 0.02, 0.53, 0.05
+    g, data, names, inp = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
+                                     alpha, extra="dump_states = 1\n")
+    print(f"case: driver REF L={L} Nc={Nc} alpha={alpha}", flush=True)
+    r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    labels_ref, R_ref, forces_ref, states_ref = reference_loop(
+        O, g, data, names, Nc, T, dT, eps, U0, alpha, L)
+    run = tmp_path / "run"
+    labels = np.fromfile(run / "labels.dat", dtype=np.int32).reshape(-1, Nc)
+    assert np.array_equal(labels, np.array(labels_ref))
+    R = np.fromfile(run / "rvalues.dat")
+    np.testing.assert_allclose(R, R_ref, rtol=1e-12, atol=0)
+    # the first interval precedes any perturbation: bit-exact
+    for j in range(Nc):
+        Fj = np.fromfile(run / f"clone_{j}" / "data_force.datout").reshape(-1, 20)
+        assert np.array_equal(Fj[0], forces_ref[j][0])
+        np.testing.assert_allclose(Fj, np.array(forces_ref[j]), rtol=1e-10,
+                                   atol=1e-12)
+        fin = np.fromfile(run / f"clone_{j}" / "pops_final.datout").reshape(
+            g.Lx, g.Ly, 9)
+        np.testing.assert_allclose(fin, states_ref[j], rtol=1e-11, atol=0)
+    out = (tmp_path / "output_TLGK.out").read_text()
+    assert "nbCreatedCopies" in out and "Total # copies after cloning/pruning" in out
+    if alpha == 0.0:
+        assert np.all(R == 1.0)

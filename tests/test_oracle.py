@@ -1,0 +1,340 @@
+"""Pins the CPU oracle: against golden fixtures written by the reference's own
+compiled kernels, against the live reference build where it exists, and
+against closed forms and invariants that a plausible mistake would break."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+
+def _geom_from(O, fx, mode=0):
+    Lx, Ly, x0, x1, y0, y1 = [int(v) for v in fx["geom"]]
+    return O.Geom(mode, Lx, Ly, x0, x1, y0, y1, float(fx["tau"]),
+                  float(fx["beta"]), 0.0)
+
+
+@pytest.mark.parametrize("L", [4, 8])
+def test_oracle_matches_reference_fixture(O, L):
+    fx = np.load(os.path.join(GOLDEN, f"ref_steps_L{L}.npz"))
+    g = _geom_from(O, fx)
+    a = fx["S0"].copy()
+    b = np.zeros_like(a)
+    keep = list(fx["keep"])
+    for n in range(len(fx["forces"])):
+        F = O.step(g, a, b)
+        if n in keep:
+            i = keep.index(n)
+            assert np.array_equal(a, fx["post"][i]), f"post-collision step {n}"
+            assert np.array_equal(b, fx["states"][i]), f"state step {n}"
+        assert F == fx["forces"][n], f"force step {n}"
+        a, b = b, a
+
+
+def test_oracle_perturbation_matches_reference_fixture(O):
+    fx = np.load(os.path.join(GOLDEN, "ref_perturb_L4.npz"))
+    f = fx["fin"].copy()
+    O.perturb(f, fx["snaps"], float(fx["eps"]), fx["prefactors"])
+    assert np.array_equal(f, fx["out"])
+
+
+def test_perturbation_prefactors_are_drand48_default_stream(O):
+    fx = np.load(os.path.join(GOLDEN, "ref_perturb_L4.npz"))
+    d = np.load(os.path.join(GOLDEN, "glibc_streams.npz"))["drand48_default"]
+    assert np.array_equal(fx["prefactors"], d[:3])
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(
+    os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+    "_ref", "libref.so")), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("L,tau,U0,seed", [(4, 0.6, 0.05, 1), (5, 0.53, 0.08, 2),
+                                           (6, 0.515, 0.03, 3), (9, 0.7, 0.1, 4)])
+def test_oracle_matches_live_reference(O, L, tau, U0, seed):
+    g = O.ref_box_geom(L, tau, U0)
+    rng = np.random.default_rng(seed)
+    f = np.empty((g.Lx, g.Ly, 9))
+    for x in range(g.Lx):
+        for y in range(g.Ly):
+            f[x, y] = O.feq(1 + 1e-2 * rng.standard_normal(),
+                            U0 + 1e-2 * rng.standard_normal(),
+                            1e-2 * rng.standard_normal())
+    a, b, c, d = f.copy(), np.zeros_like(f), f.copy(), np.zeros_like(f)
+    for _ in range(5):
+        Fo = O.step(g, a, b)
+        Fr = O.ref_step(g, c, d)
+        assert Fo == Fr
+        assert np.array_equal(a, c) and np.array_equal(b, d)
+        a, b, c, d = b, a, d, c
+    # perturbation from the same drand48 position
+    snaps = np.stack([a * (1 + 1e-3 * rng.standard_normal(a.shape))
+                      for _ in range(4)])
+    x1 = a.copy()
+    pref = O.ref_perturb_default_stream(x1, snaps, 0.1, skip=7)
+    x2 = a.copy()
+    O.perturb(x2, snaps, 0.1, pref)
+    assert np.array_equal(x1, x2)
+
+
+# ---------------------------------------------------------------- RNG pins
+def _drand48_closed_form(n):
+    a, c, m = 0x5DEECE66D, 0xB, 1 << 48
+    x = 0x1234ABCD330E
+    out = []
+    for _ in range(n):
+        x = (a * x + c) % m
+        out.append(x / float(m))
+    return np.array(out)
+
+
+def _glibc_random_r_closed_form(seed, n, discard):
+    """glibc random() TYPE_3: r[0]=seed, r[i]=16807*r[i-1] mod (2^31-1),
+    r[31..33] copies, then additive lagged Fibonacci r[i]=r[i-31]+r[i-3]
+    (mod 2^32); the first 310 values after r[33] are discarded and each
+    output is r >> 1."""
+    r = [seed]
+    for i in range(1, 31):
+        r.append((16807 * r[i - 1]) % 2147483647)
+    for i in range(31, 34):
+        r.append(r[i - 31])
+    out = []
+    total = 310 + discard + n
+    for k in range(total):
+        i = 34 + k
+        r.append((r[i - 31] + r[i - 3]) & 0xFFFFFFFF)
+        if k >= 310 + discard:
+            out.append(r[i] >> 1)
+    return np.array(out, dtype=np.int64)
+
+
+def test_glibc_streams(O):
+    fx = np.load(os.path.join(GOLDEN, "glibc_streams.npz"))
+    assert np.array_equal(O.glibc_drand48(64), fx["drand48_default"])
+    assert np.array_equal(fx["drand48_default"], _drand48_closed_form(64))
+    r = O.glibc_rand(1, 64, discard=1000)
+    assert np.array_equal(r, fx["rand_seed1_after1000"])
+    assert np.array_equal(r.astype(np.int64),
+                          _glibc_random_r_closed_form(1, 64, 1000))
+
+
+def test_philox_known_answers(O):
+    # Random123 known-answer vectors for philox4x32-10
+    kat = [
+        ([0, 0, 0, 0], [0, 0],
+         [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2,
+         [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+         [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, exp in kat:
+        assert O.philox(ctr, key).tolist() == exp
+
+
+def test_box_muller_moments(O):
+    n = []
+    for i in range(20000):
+        n.extend(O.box_muller4(O.philox([i, 7, 0, 0], [1, 2])))
+    n = np.array(n)
+    assert abs(n.mean()) < 4 / np.sqrt(n.size)
+    assert abs(n.var() - 1) < 6 * np.sqrt(2 / n.size)
+    # fourth moment of a standard normal is 3
+    assert abs(np.mean(n ** 4) - 3) < 0.1
+
+
+# ---------------------------------------------------------------- physics
+def test_rest_state_is_fixed_point_with_zero_drag(O):
+    g = O.ref_box_geom(6, 0.6, 0.05)
+    g.beta = 0.0
+    f = np.broadcast_to(O.W, (g.Lx, g.Ly, 9)).copy()
+    b = np.zeros_like(f)
+    F = O.step(g, f, b)
+    np.testing.assert_allclose(b, f, rtol=0, atol=1e-15)
+    assert abs(F) < 1e-14
+
+
+def test_ref_mass_conserved(O):
+    g = O.ref_box_geom(6, 0.55, 0.05)
+    rng = np.random.default_rng(0)
+    f = np.empty((g.Lx, g.Ly, 9))
+    for x in range(g.Lx):
+        for y in range(g.Ly):
+            f[x, y] = O.feq(1 + 1e-2 * rng.standard_normal(),
+                            0.05 + 1e-2 * rng.standard_normal(),
+                            1e-2 * rng.standard_normal())
+    fluid = np.ones((g.Lx, g.Ly), bool)
+    fluid[g.xmin + 1:g.xmax, g.ymin + 1:g.ymax] = False
+    m0 = f[fluid].sum()
+    acc, _ = O.run_steps(g, f, 20)
+    assert abs(f[fluid].sum() - m0) / m0 < 1e-13
+
+
+def test_ref_y_mirror_symmetry(O):
+    """Even L centres the square in y; a y-mirrored state stays mirrored."""
+    g = O.ref_box_geom(6, 0.55, 0.05)
+    mir = np.array([0, 1, 4, 3, 2, 8, 7, 6, 5])
+    rng = np.random.default_rng(1)
+    f = np.empty((g.Lx, g.Ly, 9))
+    for x in range(g.Lx):
+        for y in range((g.Ly + 1) // 2):
+            ux = 0.05 + 1e-2 * rng.standard_normal()
+            # the middle row is its own mirror image: no transverse velocity
+            uy = 1e-2 * rng.standard_normal() if 2 * y != g.Ly - 1 else 0.0
+            rho = 1 + 1e-2 * rng.standard_normal()
+            f[x, y] = O.feq(rho, ux, uy)
+            f[x, g.Ly - 1 - y] = O.feq(rho, ux, -uy)
+    f[g.xmin + 1:g.xmax, g.ymin + 1:g.ymax] = O.W
+    assert g.ymin + g.ymax == g.Ly - 1
+    O.run_steps(g, f, 10)
+    np.testing.assert_allclose(f, f[:, ::-1, :][:, :, mir], rtol=0, atol=1e-13)
+
+
+def test_channel_uniform_flow_stays_equilibrium_away_from_obstacle(O):
+    g = O.channel_geom(60, 30, 6, 40, 15, 0.6, 0.05)
+    feq = O.feq(1.0, 0.05, 0.0)
+    f = np.broadcast_to(feq, (g.Lx, g.Ly, 9)).copy()
+    f[g.xmin + 1:g.xmax, g.ymin + 1:g.ymax] = O.W
+    b = np.zeros_like(f)
+    O.step(g, f, b)
+    # columns 0..20 (inlet side, two nodes clear of the square's influence),
+    # rows 1..Ly-2 (away from the walls)
+    np.testing.assert_allclose(b[0:20, 1:-1], np.broadcast_to(feq, b[0:20, 1:-1].shape),
+                               rtol=0, atol=2e-16)
+
+
+def test_channel_momentum_exchange_agrees_with_stress_integral(O):
+    """On a developed flow the two drag functionals agree to discretisation
+    error (a sign, index or factor-2 mistake in either would not)."""
+    g = O.channel_geom(120, 40, 8, 30, 20, 0.56, 0.05)
+    f = O.init_channel(g, 1e-3, 0, 0)
+    O.run_steps(g, f, 3000)
+    Fm = O.lib().orc_force_mea(__import__("ctypes").byref(g), O._dp(f))
+    Fs = O.lib().orc_force_stress(__import__("ctypes").byref(g), O._dp(f))
+    assert Fm > 0 and Fs > 0
+    assert abs(Fm - Fs) / Fm < 0.1, (Fm, Fs)
+
+
+def test_channel_mass_flux_balances_at_steady_state(O):
+    g = O.channel_geom(80, 24, 4, 20, 12, 0.7, 0.04)
+    f = O.init_channel(g, 0.0, 0, 0)
+    O.run_steps(g, f, 6000)
+    jx = f[:, :, [1, 5, 8]].sum(-1) - f[:, :, [3, 6, 7]].sum(-1)
+    q_in = jx[2].sum()
+    q_out = jx[-3].sum()
+    assert abs(q_in - q_out) / q_in < 1e-3
+
+
+def test_noise_is_added_to_fluid_only(O):
+    g = O.channel_geom(30, 20, 4, 10, 10, 0.6, 0.05)
+    f = O.init_channel(g, 1e-3, 3, 1)
+    f0 = f.copy()
+    O.noise(g, f, 1e-4, 3, 1, 1, 0)
+    d = f - f0
+    assert np.all(d[g.xmin + 1:g.xmax, g.ymin + 1:g.ymax] == 0)
+    fluid = np.ones((g.Lx, g.Ly), bool)
+    fluid[g.xmin + 1:g.xmax, g.ymin + 1:g.ymax] = False
+    assert abs(d[fluid].std() / 1e-4 - 1) < 0.05
+
+
+# ---------------------------------------------------------------- selection
+def test_selection_invariants(O):
+    rng = np.random.default_rng(5)
+    for N in (1, 2, 3, 16, 100, 1000):
+        for _ in range(5):
+            s = np.exp(rng.normal(0, 1.0, N))
+            r = O.select(s, rng.random(N), rng.integers(0, 2**31 - 1, N))
+            nb = r["nb_after"]
+            assert nb.sum() == N and nb.min() >= 0
+            srcs, dsts = r["pairs"][:, 0], r["pairs"][:, 1]
+            assert len(set(dsts)) == len(dsts)
+            assert np.all(nb[dsts] == 0)
+            assert np.all(np.diff(srcs) >= 0)
+            for i in np.unique(srcs):
+                assert np.sum(srcs == i) == nb[i] - 1
+            lab = r["labels"]
+            assert np.array_equal(np.bincount(lab, minlength=N), nb)
+
+
+def test_selection_equal_weights_is_identity(O):
+    N = 64
+    r = O.select(np.full(N, 4.0), np.random.default_rng(0).random(N) * 0.999,
+                 np.zeros(N, np.int32))
+    assert len(r["pairs"]) == 0
+    assert np.array_equal(r["labels"], np.arange(N))
+    assert r["total_R"] == 4.0
+
+
+def test_selection_floor_step_is_unbiased(O):
+    """E[floor(s/R + U)] = s/R."""
+    rng = np.random.default_rng(2)
+    N = 8
+    s = np.array([0.1, 0.5, 1.0, 1.3, 2.2, 0.7, 1.9, 0.3])
+    R = s.mean()
+    acc = np.zeros(N)
+    T = 20000
+    for _ in range(T):
+        acc += O.select(s, rng.random(N), rng.integers(0, 2**31 - 1, N))["nb_before"]
+    mean = acc / T
+    sd = np.sqrt((s / R % 1) * (1 - s / R % 1) / T) + 1e-12
+    assert np.all(np.abs(mean - s / R) < 5 * sd + 1e-3)
+
+
+def test_selection_prune_is_uniform_without_replacement(O):
+    """With nb_before fixed, the kills of parent j average dN*nb_j/NcPrime
+    (hypergeometric mean)."""
+    rng = np.random.default_rng(3)
+    s = np.array([3.0, 3.0, 0.5, 0.5, 1.0, 1.0])  # R = 1.5 -> s/R = 2,2,1/3,1/3,2/3,2/3
+    u = np.array([0.5, 0.5, 0.9, 0.9, 0.5, 0.5])   # nb = 2,2,1,1,1,1 -> NcPrime 8
+    kills = np.zeros(6)
+    T = 20000
+    for _ in range(T):
+        r = O.select(s, u, rng.integers(0, 2**31 - 1, 6))
+        assert np.array_equal(r["nb_before"], [2, 2, 1, 1, 1, 1])
+        kills += r["nb_before"] - r["nb_after"]
+    exp = 2 * np.array([2, 2, 1, 1, 1, 1]) / 8
+    assert np.all(np.abs(kills / T - exp) < 0.02)
+
+
+def test_selection_table_matches_reference_printout(O):
+    """The reference's copy-table loop (src/mpi_LBM_GK_pops_perturb.cpp:
+    595-619) printed `9->0 | 12->1 | 12->5` for nb = {0:0, 1:0, 5:0, 9:2,
+    12:3, all others 1} on a 16-clone run; reproduce that table by feeding
+    weights and draws that yield exactly those counts."""
+    N = 16
+    nb = np.ones(N, int)
+    nb[[0, 1, 5]] = 0
+    nb[9] = 2
+    nb[12] = 3
+    s = nb.astype(float) + 0.25  # R = 1.25 -> s/R in (0, 3)
+    R = s.mean()
+    u = (nb - s / R) + 0.5 / R   # floor(s/R + u) = nb exactly
+    assert np.all((u >= 0) & (u < 1))
+    r = O.select(s, u, np.zeros(N, np.int32))
+    assert np.array_equal(r["nb_before"], nb)
+    assert r["pairs"].tolist() == [[9, 0], [12, 1], [12, 5]]
+
+
+def test_scgf_zero_bias_gives_unit_mean_weight(O):
+    N = 32
+    s = np.exp(0.0 * np.random.default_rng(0).normal(size=N))
+    r = O.select(s, np.random.default_rng(1).random(N), np.zeros(N, np.int32))
+    assert r["total_R"] == 1.0
+
+
+def test_brute_force_selection_tiny(O):
+    """N=3: enumerate floor draws on a grid and every prune/add draw residue;
+    the multiset after prune equals the floor multiset minus a uniformly
+    chosen sub-multiset, and the table is the greedy first-empty-slot one."""
+    s = np.array([2.0, 0.5, 0.5])  # s/R = 2, .5, .5
+    for ua in (0.1, 0.6):
+        for ub in (0.2, 0.7):
+            u = np.array([0.3, ua, ub])
+            nb0 = np.floor(s / s.mean() + u).astype(int)
+            for r0 in range(6):
+                r = O.select(s, u, np.array([r0, r0 + 1, r0 + 2], np.int32))
+                assert np.array_equal(r["nb_before"], nb0)
+                assert r["nb_after"].sum() == 3
+                empties = [k for k in range(3) if r["nb_after"][k] == 0]
+                extras = [i for i in range(3) for _ in range(max(r["nb_after"][i] - 1, 0))]
+                assert r["pairs"].tolist() == [[i, k] for i, k in zip(extras, empties)]
