@@ -46,11 +46,16 @@ def spec_for(n_clones: int, steps: int | None = None) -> lbc.InstanceSpec:
                        sigma=w["sigma"])
 
 
-def time_storage(storage: int, K: int, W: int, seed: int = 0) -> dict:
-    """W untimed warm-up intervals then K timed ones (graph replays)."""
+def time_storage(storage: int, K: int, W: int, seed: int = 0,
+                 shape=None) -> dict:
+    """W untimed warm-up intervals then K timed ones (graph replays).
+    shape: (threads, sites per thread, non-temporal) or None (tuned)."""
     spec = spec_for(WORKLOAD["n_clones"])
+    kw = {}
+    if shape:
+        kw = dict(threads=shape[0], vec=shape[1], nt=bool(shape[2]))
     with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=storage,
-                     seed=seed) as s:
+                     seed=seed, **kw) as s:
         s.init_equilibrium(0, 1e-3)
         if W:
             s.advance(W)
@@ -62,6 +67,7 @@ def time_storage(storage: int, K: int, W: int, seed: int = 0) -> dict:
         kms, nk = s.step_kernel_ms()
         R = s.R_history(0)
         threads = s.threads
+        vec = s.vec
     secs = t1 - t0
     lups = spec.n_clones * spec.sites * spec.steps * K
     bpl = lbc.bytes_per_lup(storage)
@@ -79,6 +85,7 @@ def time_storage(storage: int, K: int, W: int, seed: int = 0) -> dict:
         achieved_gbs=achieved,
         roofline_frac=achieved / HBM_PEAK_GBS,
         threads=threads,
+        sites_per_thread=vec,
         logR_last=float(math.log(R[-1])) if len(R) else None,
     )
 
@@ -158,6 +165,21 @@ def cpu_baseline(sample_clones_per_core: int = 2, steps: int = 200) -> dict:
                         f"{secs:.1f} s on {cpu}"))
 
 
+def calibrate_copy() -> int:
+    """A plain device copy of 2^28 FP32 (1 GiB read, 1 GiB written), for
+    scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +187,13 @@ def main() -> int:
                     help="timed cloning intervals")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (torch) so that "
+                         "HBM counters can be scaled against known bytes")
+    ap.add_argument("--shape", default="",
+                    help="threads,sites_per_thread,nontemporal for every "
+                         "storage (skips the tuning at context creation)")
     ap.add_argument("--only", default="",
                     help="comma list of storages to time (f64,f32,f16,bf16)")
     args = ap.parse_args()
@@ -172,6 +201,8 @@ def main() -> int:
         print("bench.py: this build runs on one GPU only (N=1); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
+    if args.calibrate_copy:
+        return calibrate_copy()
     K, W = args.steps, args.warmup
     want = [x for x in args.only.split(",") if x] or ["f16", "f64", "f32",
                                                        "bf16"]
@@ -181,9 +212,10 @@ def main() -> int:
     for name in want:
         print(f"bench: timing {name} ({K} intervals after {W} warm-up)",
               file=sys.stderr, flush=True)
-        runs[name] = time_storage(codes[name], K, W)
+        shape = [int(v) for v in args.shape.split(",")] if args.shape else None
+        runs[name] = time_storage(codes[name], K, W, shape=shape)
     acc = {}
-    for name in want:
+    for name in ([] if args.no_accuracy else want):
         if name == "f64":
             continue
         print(f"bench: accuracy {name} vs f64", file=sys.stderr, flush=True)

@@ -55,7 +55,8 @@ struct lbc_ctx {
   double* staging = nullptr;          // AoS FP64 transfer buffer (device)
   size_t staging_bytes = 0;
   int threads = 256;
-  int vec = 1;                        // sites per thread group in the step kernel
+  int vec = 1;                        // sites per thread per iteration (step kernel)
+  int nt = 0;                         // non-temporal population stores
   int ring_max = 0;
   int n_max = 0;
   std::string err;
@@ -105,11 +106,10 @@ int validate(const lbc_config* cfg, std::string& why) {
     return why = "threads must be 0 or a multiple of 64 in [64, 1024]", -1;
   {
     const int v = cfg->flags & 7;
-    if ((cfg->flags & ~7) || (v != 0 && v != 1 && v != 2 && v != 4))
-      return why = "flags: only bits 0-2 (sites per thread: 0, 1, 2 or 4)", -1;
-    for (int i = 0; v > 1 && i < cfg->n_instances; i++)
-      if (cfg->instances[i].Lx % v)
-        return why = "flags: sites per thread must divide every Lx", -1;
+    const int ntm = (cfg->flags >> 3) & 3;
+    if ((cfg->flags & ~31) || (v != 0 && v != 1) || ntm == 3)
+      return why = "flags: bits 0-2 sites per thread (0 or 1), bits 3-4 "
+                   "non-temporal stores (0 tuned, 1 off, 2 on)", -1;
   }
   for (int i = 0; i < cfg->n_instances; i++) {
     const lbc_instance& s = cfg->instances[i];
@@ -166,6 +166,7 @@ lbc::Launch make_launch(lbc_ctx* c) {
   L.mode = c->cfg.mode;
   L.storage = c->cfg.storage;
   L.vec = c->vec;
+  L.nt = c->nt;
   L.stream = c->stream;
   return L;
 }
@@ -228,12 +229,6 @@ int enqueue_perturb(lbc_ctx* c, bool philox) {
   return LBC_OK;
 }
 
-bool vec_ok(const lbc_ctx* c, int v) {
-  for (const auto& s : c->inst)
-    if (s.Lx % v) return false;
-  return true;
-}
-
 int autotune(lbc_ctx* c) {
   // Time short intervals at each (workgroup size, sites per thread) on the
   // freshly initialised state; the caller re-initialises the state after.
@@ -243,19 +238,22 @@ int autotune(lbc_ctx* c) {
                          hipMemcpyHostToDevice, c->stream));
   const int tcands[3] = {256, 512, 1024};
   const int vfixed = c->cfg.flags & 7;
+  const int ntmode = (c->cfg.flags >> 3) & 3;  // 0 tuned, 1 off, 2 on
   float best = 1e30f;
-  int best_t = 256, best_v = 1;
+  int best_t = 256, best_v = 1, best_nt = 0;
   hipEvent_t e0, e1;
   HIPC(c, hipEventCreate(&e0));
   HIPC(c, hipEventCreate(&e1));
-  for (int v : {1, 2, 4}) {
+  for (int nt = 0; nt < 2; nt++)
+  for (int v : {1}) {
     if (vfixed && v != vfixed) continue;
-    if (!vec_ok(c, v)) continue;
+    if ((ntmode == 1 && nt) || (ntmode == 2 && !nt)) continue;
     for (int t : tcands) {
       if (c->cfg.threads && t != c->cfg.threads) continue;
       lbc::Launch L = make_launch(c);
       L.threads = t;
       L.vec = v;
+      L.nt = nt;
       HIPC(c, lbc::launch_interval(L));  // warm-up
       HIPC(c, hipEventRecord(e0, c->stream));
       for (int r = 0; r < 2; r++) HIPC(c, lbc::launch_interval(L));
@@ -267,6 +265,7 @@ int autotune(lbc_ctx* c) {
         best = ms;
         best_t = t;
         best_v = v;
+        best_nt = nt;
       }
     }
   }
@@ -274,6 +273,7 @@ int autotune(lbc_ctx* c) {
   (void)hipEventDestroy(e1);
   c->threads = best_t;
   c->vec = best_v;
+  c->nt = best_nt;
   HIPC(c, hipMemcpyAsync(c->idd, c->idh.data(), sizeof(InstDev) * c->idh.size(),
                          hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -443,7 +443,8 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
       return bail(fail(c, LBC_E_HIP, "init kernel launch failed"));
   }
   const int vfixed = cfg->flags & 7;
-  if (cfg->threads == 0 || vfixed == 0) {
+  const int ntmode = (cfg->flags >> 3) & 3;
+  if (cfg->threads == 0 || vfixed == 0 || ntmode == 0) {
     int rc = autotune(c);
     if (rc) return bail(rc);
     L = make_launch(c);
@@ -453,6 +454,7 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
   } else {
     c->threads = cfg->threads;
     c->vec = vfixed;
+    c->nt = ntmode == 2;
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(fail(c, LBC_E_HIP, "initialisation failed"));
@@ -784,7 +786,7 @@ int lbc_step_host(int mode, const lbc_instance* geom, double* fin, double* fout,
   cfg.storage = LBC_STORE_F64;
   cfg.n_instances = 1;
   cfg.threads = 256;
-  cfg.flags = 1;
+  cfg.flags = 1 | (1 << 3);
   cfg.instances = &s;
   lbc_ctx* c = nullptr;
   int rc = lbc_create(&cfg, &c);

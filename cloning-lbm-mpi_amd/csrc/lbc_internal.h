@@ -71,7 +71,8 @@ struct Launch {
   int ring_max;
   int mode;
   int storage;
-  int vec;           // sites per thread group (1 or 2)
+  int vec;           // sites per thread per iteration of the step kernel (1, 2, 4)
+  int nt;            // non-temporal population stores
   hipStream_t stream;
 };
 

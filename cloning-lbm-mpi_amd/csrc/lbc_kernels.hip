@@ -186,7 +186,10 @@ __device__ inline void collide_f64(double (&f)[9], double beta, double tau) {
   }
 }
 
-// Same collision on shifted populations d = f - w, FP32.
+// Same collision on shifted populations d = f - w, FP32, written pairwise:
+// for each pair of opposite directions (+e, -e) the equilibrium splits into
+// a common part t = drho + rho(4.5 eu^2 - 1.5 u^2) and an odd part
+// s = 3 rho eu, so feq_q - w_q = w_q (t +/- s).
 template <int MODE>
 __device__ inline void collide_shifted(float (&d)[9], float omega, float beta) {
 #pragma clang fp contract(fast)
@@ -199,19 +202,43 @@ __device__ inline void collide_shifted(float (&d)[9], float omega, float beta) {
   float ux = jx * inv;
   if (MODE == kModeRef) ux += 0.5f * beta * inv;
   const float uy = jy * inv;
-  const float usq = 1.5f * (ux * ux + uy * uy);
-  const float omega1 = 1.0f - omega;
-  const float cforce = 1.0f - 0.5f * omega;
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const float eu = (float)CX[k] * ux + (float)CY[k] * uy;
-    const float deq = WF[k] * (drho + rho * (3.0f * eu + 4.5f * eu * eu - usq));
-    float v = omega1 * d[k] + omega * deq;
-    if (MODE == kModeRef)
-      v += WF[k] * cforce * beta *
-           (3.0f * ((float)CX[k] - ux) + 9.0f * (float)CX[k] * eu);
-    d[k] = v;
+  const float base = drho - rho * (1.5f * (ux * ux + uy * uy));
+  const float r45 = 4.5f * rho, r3 = 3.0f * rho;
+  const float om1 = 1.0f - omega;
+  const float o0 = omega * (4.0f / 9.0f), o1 = omega * (1.0f / 9.0f),
+              o2 = omega * (1.0f / 36.0f);
+  // eu of directions 1, 2, 5, 6 (their opposites 3, 4, 7, 8 have -eu)
+  const float e1 = ux, e2 = uy, e5 = ux + uy, e6 = uy - ux;
+  const float t1 = fmaf(r45 * e1, e1, base), s1 = r3 * e1;
+  const float t2 = fmaf(r45 * e2, e2, base), s2 = r3 * e2;
+  const float t5 = fmaf(r45 * e5, e5, base), s5 = r3 * e5;
+  const float t6 = fmaf(r45 * e6, e6, base), s6 = r3 * e6;
+  float n[9];
+  n[0] = fmaf(o0, base, om1 * d[0]);
+  n[1] = fmaf(o1, t1 + s1, om1 * d[1]);
+  n[3] = fmaf(o1, t1 - s1, om1 * d[3]);
+  n[2] = fmaf(o1, t2 + s2, om1 * d[2]);
+  n[4] = fmaf(o1, t2 - s2, om1 * d[4]);
+  n[5] = fmaf(o2, t5 + s5, om1 * d[5]);
+  n[7] = fmaf(o2, t5 - s5, om1 * d[7]);
+  n[6] = fmaf(o2, t6 + s6, om1 * d[6]);
+  n[8] = fmaf(o2, t6 - s6, om1 * d[8]);
+  if (MODE == kModeRef) {
+    // Guo forcing w_q (1 - omega/2) beta (3 (cx - ux) + 9 cx eu)
+    const float G = (1.0f - 0.5f * omega) * beta;
+    const float a = 6.0f * ux, c = 9.0f * uy;
+    n[0] += (4.0f / 9.0f) * G * (-3.0f * ux);
+    n[1] += (1.0f / 9.0f) * G * (3.0f + a);
+    n[3] += (1.0f / 9.0f) * G * (a - 3.0f);
+    n[2] += (1.0f / 9.0f) * G * (-3.0f * ux);
+    n[4] += (1.0f / 9.0f) * G * (-3.0f * ux);
+    n[5] += (1.0f / 36.0f) * G * (3.0f + a + c);
+    n[7] += (1.0f / 36.0f) * G * (a + c - 3.0f);
+    n[6] += (1.0f / 36.0f) * G * (a - c - 3.0f);
+    n[8] += (1.0f / 36.0f) * G * (3.0f + a - c);
   }
+#pragma unroll
+  for (int k = 0; k < 9; k++) d[k] = n[k];
 }
 
 // Moments and non-equilibrium stress of one node in the order of
@@ -364,15 +391,78 @@ struct Geo {
   int sx, sy;
 };
 
+// Buffer-resource access to one clone's populations: the per-direction
+// plane offsets are wave-uniform (SGPR soffset), so a load or store costs no
+// vector address arithmetic.  dword3 0x00020000 is the raw-buffer format of
+// gfx9-family targets.
+__device__ inline __amdgpu_buffer_rsrc_t clone_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, bytes,
+                                           0x00020000);
+}
+
+template <int ST>
+struct BufIO;
+
+template <>
+struct BufIO<kF64> {
+  __device__ static double ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+  }
+  template <bool NT>
+  __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(
+        __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
+        vo, so, NT ? 2 : 0);
+  }
+};
+
+template <>
+struct BufIO<kF32> {
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+  }
+  template <bool NT>
+  __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo,
+                                          so, NT ? 2 : 0);
+  }
+};
+
+template <>
+struct BufIO<kF16> {
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return (float)__builtin_bit_cast(
+        _Float16, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0));
+  }
+  template <bool NT>
+  __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float v) {
+    __builtin_amdgcn_raw_buffer_store_b16(
+        __builtin_bit_cast(unsigned short, (_Float16)v), r, vo, so, NT ? 2 : 0);
+  }
+};
+
+template <>
+struct BufIO<kBF16> {
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
+    return __uint_as_float((uint32_t)b << 16);
+  }
+  template <bool NT>
+  __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float v) {
+    __builtin_amdgcn_raw_buffer_store_b16(f32_to_bf16(v), r, vo, so, NT ? 2 : 0);
+  }
+};
+
 // One site of one pass.  src/dst point at this clone's populations.
 template <int MODE, int ST, int PH>
 __device__ inline void do_site(const Geo& g, int x, int y,
-                               const typename Traits<ST>::S* __restrict__ src,
-                               typename Traits<ST>::S* __restrict__ dst,
+                               __amdgpu_buffer_rsrc_t rs,
+                               __amdgpu_buffer_rsrc_t rd,
                                double* __restrict__ ring, double beta,
                                double tau, const InstDev* __restrict__ I) {
   using T = Traits<ST>;
   using C = typename T::C;
+  constexpr int ES = (int)sizeof(typename T::S);
   const int site = y * g.Lx + x;
   const bool in_box = (x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1);
   if (PH != kCollide && in_box && x > g.x0 && x < g.x1 && y > g.y0 &&
@@ -381,26 +471,26 @@ __device__ inline void do_site(const Geo& g, int x, int y,
     // its post-collision values never reach a fluid node).  The collide-only
     // pass collides it like the reference does.
 #pragma unroll
-    for (int q = 0; q < 9; q++) T::st(dst + q * g.plane + site, T::rest(q));
+    for (int q = 0; q < 9; q++) BufIO<ST>::template st<false>(rd, (q * g.plane + site) * ES, 0, T::rest(q));
     return;
   }
   C f[9];
   if (PH == kCollide) {
 #pragma unroll
-    for (int q = 0; q < 9; q++) f[q] = T::ld(src + q * g.plane + site);
+    for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::ld(rs, (q * g.plane + site) * ES, 0);
   } else {
     const bool plain = !in_box && y >= 1 && y <= g.Ly - 2 && x >= 1 &&
                        x <= g.Lx - 2;
     if (plain) {
 #pragma unroll
       for (int q = 0; q < 9; q++)
-        f[q] = T::ld(src + q * g.plane + site - CY[q] * g.Lx - CX[q]);
+        f[q] = BufIO<ST>::ld(rs, (q * g.plane + site - CY[q] * g.Lx - CX[q]) * ES, 0);
     } else {
       const int m = wall_mask(y, g.Ly) | square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
 #pragma unroll
       for (int q = 0; q < 9; q++) {
         if ((m >> q) & 1) {
-          f[q] = T::ld(src + OPP[q] * g.plane + site);
+          f[q] = BufIO<ST>::ld(rs, (OPP[q] * g.plane + site) * ES, 0);
           continue;
         }
         int xs = x - CX[q];
@@ -416,7 +506,7 @@ __device__ inline void do_site(const Geo& g, int x, int y,
           }
           if (xs >= g.Lx) xs = g.Lx - 1;  // zero-gradient ghost column
         }
-        f[q] = T::ld(src + q * g.plane + ys * g.Lx + xs);
+        f[q] = BufIO<ST>::ld(rs, (q * g.plane + ys * g.Lx + xs) * ES, 0);
       }
       if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1))
         channel_x_boundary<ST>(f, x == 0, I->u_in);
@@ -496,51 +586,8 @@ __device__ inline void do_site(const Geo& g, int x, int y,
     }
   }
 #pragma unroll
-  for (int q = 0; q < 9; q++) T::st(dst + q * g.plane + site, f[q]);
-}
-
-// Element-aligned vector of V populations (one direction, V consecutive
-// sites).  gfx950 runs in unaligned-access mode, so a shifted (odd-offset)
-// window is still one dword / dwordx2 / dwordx4 access.
-template <class S, int V>
-struct VecOf {
-  typedef S type __attribute__((ext_vector_type(V), aligned(sizeof(S))));
-};
-
-template <int ST, int V>
-__device__ inline void vload(typename Traits<ST>::C (&out)[V],
-                             const typename Traits<ST>::S* p) {
-  using T = Traits<ST>;
-  if constexpr (V == 1) {
-    out[0] = T::ld(p);
-  } else {
-    typedef typename VecOf<typename T::S, V>::type vt;
-    const vt t = *reinterpret_cast<const vt*>(p);
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      typename T::S e = t[v];
-      out[v] = T::ld(&e);
-    }
-  }
-}
-
-template <int ST, int V>
-__device__ inline void vstore(typename Traits<ST>::S* p,
-                              const typename Traits<ST>::C (&in)[V]) {
-  using T = Traits<ST>;
-  if constexpr (V == 1) {
-    T::st(p, in[0]);
-  } else {
-    typedef typename VecOf<typename T::S, V>::type vt;
-    vt t;
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      typename T::S e;
-      T::st(&e, in[v]);
-      t[v] = e;
-    }
-    *reinterpret_cast<vt*>(p) = t;
-  }
+  for (int q = 0; q < 9; q++)
+    BufIO<ST>::template st<false>(rd, (q * g.plane + site) * ES, 0, f[q]);
 }
 
 // Collision of one site's populations in the storage's compute type.
@@ -565,57 +612,59 @@ __device__ inline void collide_any(typename Traits<ST>::C (&f)[9], double beta,
   }
 }
 
-// V consecutive sites x..x+V-1 of row y (x a multiple of V, V | Lx).
-// Groups away from every boundary and from the square pull with V-wide
-// vector loads and store V-wide; the others go site by site.
-template <int MODE, int ST, int PH, int V>
-__device__ inline void do_group(const Geo& g, int x, int y,
-                                const typename Traits<ST>::S* __restrict__ src,
-                                typename Traits<ST>::S* __restrict__ dst,
-                                double* __restrict__ ring, double beta,
-                                double tau, const InstDev* __restrict__ I) {
+// One pass of one thread over its sites: site tid, tid + T, tid + 2T, ...
+// (T = workgroup size).  Sites off the wall rows and outside the square's
+// bounding box take the fast path: nine buffer loads at SGPR offsets
+// q*plane - cy*Lx - cx from the site, the x boundary folded into two
+// per-lane offsets (periodic wrap in REF mode; inlet/outlet ghosts in
+// channel mode, followed by the channel boundary rule), collision, nine
+// buffer stores.  Wall rows, ring and solid sites take the general rule.
+template <int MODE, int ST, int PH, bool NT>
+__device__ inline void site_pass(const Geo& g, int tid, int nthr,
+                                 const typename Traits<ST>::S* __restrict__ src,
+                                 typename Traits<ST>::S* __restrict__ dst,
+                                 double* __restrict__ ring, double beta,
+                                 double tau, const InstDev* __restrict__ I,
+                                 int clone_bytes) {
   using T = Traits<ST>;
   using C = typename T::C;
-  const int site = y * g.Lx + x;
-  const bool touches_box =
-      (x + V - 1 >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1);
-  const bool plain = !touches_box && y >= 1 && y <= g.Ly - 2 && x >= 1 &&
-                     x + V <= g.Lx - 1;
-  if (PH == kCollide || plain) {
-    C f[9][V];
+  constexpr int ES = (int)sizeof(typename T::S);
+  const __amdgpu_buffer_rsrc_t rs = clone_rsrc(src, clone_bytes);
+  const __amdgpu_buffer_rsrc_t rd = clone_rsrc(dst, clone_bytes);
+  int soff[9], sst[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) {
-      const int off = (PH == kCollide) ? 0 : (CY[q] * g.Lx + CX[q]);
-      vload<ST, V>(f[q], src + q * g.plane + site - off);
-    }
-    if (PH != kStream) {
-#pragma unroll
-      for (int v = 0; v < V; v++) {
-        C h[9];
-#pragma unroll
-        for (int q = 0; q < 9; q++) h[q] = f[q][v];
-        collide_any<MODE, ST>(h, beta, tau);
-#pragma unroll
-        for (int q = 0; q < 9; q++) f[q][v] = h[q];
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 9; q++) vstore<ST, V>(dst + q * g.plane + site, f[q]);
-  } else {
-#pragma unroll
-    for (int v = 0; v < V; v++)
-      do_site<MODE, ST, PH>(g, x + v, y, src, dst, ring, beta, tau, I);
+  for (int q = 0; q < 9; q++) {
+    sst[q] = q * g.plane * ES;
+    soff[q] = (PH == kCollide) ? sst[q]
+                               : (q * g.plane - CY[q] * g.Lx - CX[q]) * ES;
   }
-}
-
-template <int MODE, int ST, int PH, int V>
-__device__ inline void group_pass(const Geo& g, int x, int y, int dx, int dy,
-                                  const typename Traits<ST>::S* __restrict__ src,
-                                  typename Traits<ST>::S* __restrict__ dst,
-                                  double* __restrict__ ring, double beta,
-                                  double tau, const InstDev* __restrict__ I) {
+  // x-boundary offsets (elements) for pulls with cx = +1 at x = 0 and
+  // cx = -1 at x = Lx-1
+  const int adjL = (MODE == kModeRef) ? g.Lx : 1;
+  const int adjR = (MODE == kModeRef) ? -g.Lx : -1;
+  const int dx = nthr % g.Lx, dy = nthr / g.Lx;
+  int x = tid % g.Lx, y = tid / g.Lx, site = tid;
   while (y < g.Ly) {
-    do_group<MODE, ST, PH, V>(g, x, y, src, dst, ring, beta, tau, I);
+    const bool in_box = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1;
+    const bool fast = (PH == kCollide) || (!in_box && y >= 1 && y <= g.Ly - 2);
+    if (fast) {
+      const int vo = site * ES;
+      const int voL = (PH != kCollide && x == 0) ? vo + adjL * ES : vo;
+      const int voR = (PH != kCollide && x == g.Lx - 1) ? vo + adjR * ES : vo;
+      C f[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++)
+        f[q] = BufIO<ST>::ld(rs, CX[q] > 0 ? voL : (CX[q] < 0 ? voR : vo),
+                             soff[q]);
+      if (MODE == kModeChannel && PH != kCollide && (x == 0 || x == g.Lx - 1))
+        channel_x_boundary<ST>(f, x == 0, I->u_in);
+      if (PH != kStream) collide_any<MODE, ST>(f, beta, tau);
+#pragma unroll
+      for (int q = 0; q < 9; q++) BufIO<ST>::template st<NT>(rd, vo, sst[q], f[q]);
+    } else {
+      do_site<MODE, ST, PH>(g, x, y, rs, rd, ring, beta, tau, I);
+    }
+    site += nthr;
     x += dx;
     y += dy;
     if (x >= g.Lx) {
@@ -645,7 +694,7 @@ __device__ inline double ring_sum(const double* ring, int sx, int sy) {
 // S_0 is in buf[parity]; pass 0 collides it, passes 1..M-1 stream and
 // collide, pass M streams only, so the interval ends with S_M in
 // buf[parity ^ ((M+1)&1)].  Drag F_n is taken on S_n in pass n.
-template <int MODE, int ST, int V>
+template <int MODE, int ST, bool NT>
 __global__ void __launch_bounds__(1024)
     k_interval(const InstDev* __restrict__ insts,
                const WorkItem* __restrict__ items,
@@ -673,8 +722,7 @@ __global__ void __launch_bounds__(1024)
   double* drag = I->drag + (long long)wi.clone * M;
 
   const int nthr = blockDim.x, tid = threadIdx.x;
-  const int dx = (nthr * V) % g.Lx, dy = (nthr * V) / g.Lx;
-  const int x_init = (tid * V) % g.Lx, y_init = (tid * V) / g.Lx;
+  const int clone_bytes = (int)(I->clone_stride * (long long)sizeof(S));
   double acc = 0.0;
 
   for (int n = 0; n <= M; ++n) {
@@ -682,14 +730,14 @@ __global__ void __launch_bounds__(1024)
     S* dst = (n & 1) ? A : B;
     double* ring = ring_lds + (n & 1) * ring_max;
     if (n == 0)
-      group_pass<MODE, ST, kCollide, V>(g, x_init, y_init, dx, dy, src, dst, ring,
-                                    beta, tau, I);
+      site_pass<MODE, ST, kCollide, NT>(g, tid, nthr, src, dst, ring,
+                                    beta, tau, I, clone_bytes);
     else if (n == M)
-      group_pass<MODE, ST, kStream, V>(g, x_init, y_init, dx, dy, src, dst, ring,
-                                   beta, tau, I);
+      site_pass<MODE, ST, kStream, NT>(g, tid, nthr, src, dst, ring,
+                                   beta, tau, I, clone_bytes);
     else
-      group_pass<MODE, ST, kBoth, V>(g, x_init, y_init, dx, dy, src, dst, ring,
-                                 beta, tau, I);
+      site_pass<MODE, ST, kBoth, NT>(g, tid, nthr, src, dst, ring,
+                                 beta, tau, I, clone_bytes);
     __syncthreads();
     if (n >= 1 && tid == 0) {
 #pragma clang fp contract(off)
@@ -1187,25 +1235,14 @@ __global__ void k_snap_to_soa(int n, const double* __restrict__ aos, int Lx,
 template <int MODE, int ST>
 hipError_t launch_interval_t(const Launch& L) {
   const size_t lds = sizeof(double) * 2 * (size_t)L.ring_max;
-  switch (L.vec) {
-    case 1:
-      hipLaunchKernelGGL((k_interval<MODE, ST, 1>), dim3(L.n_items),
-                         dim3(L.threads), lds, L.stream, L.insts, L.items,
-                         L.counters, L.ring_max);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_interval<MODE, ST, 2>), dim3(L.n_items),
-                         dim3(L.threads), lds, L.stream, L.insts, L.items,
-                         L.counters, L.ring_max);
-      break;
-    case 4:
-      hipLaunchKernelGGL((k_interval<MODE, ST, 4>), dim3(L.n_items),
-                         dim3(L.threads), lds, L.stream, L.insts, L.items,
-                         L.counters, L.ring_max);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
+  if (L.nt)
+    hipLaunchKernelGGL((k_interval<MODE, ST, true>), dim3(L.n_items),
+                       dim3(L.threads), lds, L.stream, L.insts, L.items,
+                       L.counters, L.ring_max);
+  else
+    hipLaunchKernelGGL((k_interval<MODE, ST, false>), dim3(L.n_items),
+                       dim3(L.threads), lds, L.stream, L.insts, L.items,
+                       L.counters, L.ring_max);
   return hipGetLastError();
 }
 

@@ -217,7 +217,7 @@ class Sampler:
 
     def __init__(self, specs, mode: int = MODE_CHANNEL,
                  storage: int = STORE_F64, seed: int = 0, threads: int = 0,
-                 device: int = 0, vec: int = 0):
+                 device: int = 0, vec: int = 0, nt: int | None = None):
         if isinstance(specs, InstanceSpec):
             specs = [specs]
         self.specs = list(specs)
@@ -228,8 +228,11 @@ class Sampler:
         self._arr = arr
         if threads and not vec:
             vec = 1
-        cfg = Config(mode, storage, len(self.specs), device, seed, threads, vec,
-                     C.cast(arr, C.POINTER(Instance)))
+        if nt is None:
+            nt = False if threads else None
+        flags = vec | ((0 if nt is None else (2 if nt else 1)) << 3)
+        cfg = Config(mode, storage, len(self.specs), device, seed, threads,
+                     flags, C.cast(arr, C.POINTER(Instance)))
         h = C.c_void_p()
         rc = L.lbc_create(C.byref(cfg), C.byref(h))
         if rc != 0:

@@ -77,9 +77,10 @@ typedef struct lbc_config {
   int device;                 /* HIP device ordinal */
   uint64_t seed;              /* Philox key */
   int threads;                /* workgroup size of the step kernel; 0 = tuned */
-  int flags;                  /* bits 0-2: sites per thread of the step kernel
-                                 (1, 2 or 4; 0 = tuned together with threads);
-                                 other bits 0 */
+  int flags;                  /* step-kernel shape, 0 = tuned with threads:
+                                 bits 0-2 sites per thread per iteration (1),
+                                 bits 3-4 non-temporal population stores
+                                 (1 off, 2 on); other bits 0 */
   const lbc_instance* instances;
 } lbc_config;
 

@@ -56,7 +56,7 @@ def test_create_rejects_invalid_configurations(lbc):
     assert "mode" in _bad(lbc, mode=7)
     assert "storage" in _bad(lbc, storage=9)
     assert "threads" in _bad(lbc, threads=100)
-    assert "sites per thread" in _bad(lbc, threads=256, vec=3)
+    assert "sites per thread" in _bad(lbc, threads=256, vec=2)
     assert "non-finite" in _bad(lbc, bias=float("nan"))
 
 
