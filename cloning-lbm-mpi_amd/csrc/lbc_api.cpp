@@ -57,6 +57,9 @@ struct lbc_ctx {
   int threads = 256;
   int vec = 1;                        // sites per thread per iteration (step kernel)
   int nt = 0;                         // non-temporal population stores
+  int fused = 0;                      // two passes per sweep (LDS ring)
+  int rch = 0;                        // LDS ring sites per direction
+  int max_lx = 0;
   int ring_max = 0;
   int n_max = 0;
   std::string err;
@@ -107,9 +110,11 @@ int validate(const lbc_config* cfg, std::string& why) {
   {
     const int v = cfg->flags & 7;
     const int ntm = (cfg->flags >> 3) & 3;
-    if ((cfg->flags & ~31) || (v != 0 && v != 1) || ntm == 3)
+    const int fm = (cfg->flags >> 5) & 3;
+    if ((cfg->flags & ~127) || (v != 0 && v != 1) || ntm == 3 || fm == 3)
       return why = "flags: bits 0-2 sites per thread (0 or 1), bits 3-4 "
-                   "non-temporal stores (0 tuned, 1 off, 2 on)", -1;
+                   "non-temporal stores, bits 5-6 two-pass sweeps "
+                   "(0 tuned, 1 off, 2 on)", -1;
   }
   for (int i = 0; i < cfg->n_instances; i++) {
     const lbc_instance& s = cfg->instances[i];
@@ -167,6 +172,8 @@ lbc::Launch make_launch(lbc_ctx* c) {
   L.storage = c->cfg.storage;
   L.vec = c->vec;
   L.nt = c->nt;
+  L.fused = c->fused;
+  L.rch = c->rch;
   L.stream = c->stream;
   return L;
 }
@@ -195,6 +202,21 @@ int check_inst(lbc_ctx* c, int inst) {
   return LBC_OK;
 }
 
+// Buffer flips of one interval: one per pass, or one per sweep of two
+// passes in the fused step kernel.
+int interval_flip(const lbc_ctx* c, int i) {
+  const int passes = c->inst[i].steps + 1;
+  return c->fused ? ((passes + 1) / 2) & 1 : passes & 1;
+}
+
+int upload_flips(lbc_ctx* c) {
+  std::vector<int> flips(lbc::kMaxInstances, 0);
+  for (int i = 0; i < c->cfg.n_instances; i++) flips[i] = interval_flip(c, i);
+  HIPC(c, hipMemcpy(c->flip_mask, flips.data(), sizeof(int) * flips.size(),
+                    hipMemcpyHostToDevice));
+  return LBC_OK;
+}
+
 // Step kernel + parity flip for every instance, bracketed by events.
 int enqueue_interval(lbc_ctx* c, bool timed) {
   lbc::Launch L = make_launch(c);
@@ -214,8 +236,7 @@ int enqueue_interval(lbc_ctx* c, bool timed) {
     c->ev_used += 2;
   }
   HIPC(c, lbc::launch_flip(L, c->cfg.n_instances, c->flip_mask));
-  for (int i = 0; i < c->cfg.n_instances; i++)
-    c->parity[i] ^= (c->inst[i].steps + 1) & 1;
+  for (int i = 0; i < c->cfg.n_instances; i++) c->parity[i] ^= interval_flip(c, i);
   return LBC_OK;
 }
 
@@ -239,21 +260,28 @@ int autotune(lbc_ctx* c) {
   const int tcands[3] = {256, 512, 1024};
   const int vfixed = c->cfg.flags & 7;
   const int ntmode = (c->cfg.flags >> 3) & 3;  // 0 tuned, 1 off, 2 on
+  const int fmode = (c->cfg.flags >> 5) & 3;   // 0 tuned, 1 off, 2 on
   float best = 1e30f;
-  int best_t = 256, best_v = 1, best_nt = 0;
+  int best_t = 256, best_v = 1, best_nt = 0, best_f = 0;
   hipEvent_t e0, e1;
   HIPC(c, hipEventCreate(&e0));
   HIPC(c, hipEventCreate(&e1));
+  for (int fused = 0; fused < 2; fused++)
   for (int nt = 0; nt < 2; nt++)
   for (int v : {1}) {
     if (vfixed && v != vfixed) continue;
     if ((ntmode == 1 && nt) || (ntmode == 2 && !nt)) continue;
+    if ((fmode == 1 && fused) || (fmode == 2 && !fused)) continue;
     for (int t : tcands) {
       if (c->cfg.threads && t != c->cfg.threads) continue;
+      const int rch = lbc::fused_ring_sites(t, c->max_lx, c->cfg.mode);
+      if (fused && !lbc::fused_lds_bytes(c->cfg.storage, c->ring_max, rch)) continue;
       lbc::Launch L = make_launch(c);
       L.threads = t;
       L.vec = v;
       L.nt = nt;
+      L.fused = fused;
+      L.rch = rch;
       HIPC(c, lbc::launch_interval(L));  // warm-up
       HIPC(c, hipEventRecord(e0, c->stream));
       for (int r = 0; r < 2; r++) HIPC(c, lbc::launch_interval(L));
@@ -266,6 +294,7 @@ int autotune(lbc_ctx* c) {
         best_t = t;
         best_v = v;
         best_nt = nt;
+        best_f = fused;
       }
     }
   }
@@ -274,6 +303,8 @@ int autotune(lbc_ctx* c) {
   c->threads = best_t;
   c->vec = best_v;
   c->nt = best_nt;
+  c->fused = best_f;
+  c->rch = lbc::fused_ring_sites(best_t, c->max_lx, c->cfg.mode);
   HIPC(c, hipMemcpyAsync(c->idd, c->idh.data(), sizeof(InstDev) * c->idh.size(),
                          hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -369,6 +400,7 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     for (int k = 0; k < 9; k++) d.feq_in_s[k] = (float)(d.feq_in[k] - kW[k]);
     d.max_intervals = kMaxIntervals;
     c->ring_max = std::max(c->ring_max, d.ring_n);
+    c->max_lx = std::max(c->max_lx, d.Lx);
     c->n_max = std::max(c->n_max, s.n_clones);
     const size_t n = (size_t)s.n_clones;
     int rc = LBC_OK;
@@ -423,14 +455,10 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     for (int j = 0; j < c->inst[i].n_clones; j++) items.push_back({i, j});
   c->n_items = (int)items.size();
   if (dalloc(c, &c->items, sizeof(WorkItem) * items.size())) return bail(LBC_E_NOMEM);
-  std::vector<int> flips(lbc::kMaxInstances, 0);
-  for (int i = 0; i < ni; i++) flips[i] = (c->inst[i].steps + 1) & 1;
   Counters zero{};
   if (hipMemcpy(c->idd, c->idh.data(), sizeof(InstDev) * ni,
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->items, items.data(), sizeof(WorkItem) * items.size(),
-                hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->flip_mask, flips.data(), sizeof(int) * flips.size(),
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->counters, &zero, sizeof zero, hipMemcpyHostToDevice) !=
           hipSuccess)
@@ -444,7 +472,8 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
   }
   const int vfixed = cfg->flags & 7;
   const int ntmode = (cfg->flags >> 3) & 3;
-  if (cfg->threads == 0 || vfixed == 0 || ntmode == 0) {
+  const int fmode = (cfg->flags >> 5) & 3;
+  if (cfg->threads == 0 || vfixed == 0 || ntmode == 0 || fmode == 0) {
     int rc = autotune(c);
     if (rc) return bail(rc);
     L = make_launch(c);
@@ -455,7 +484,14 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     c->threads = cfg->threads;
     c->vec = vfixed;
     c->nt = ntmode == 2;
+    c->fused = fmode == 2;
+    c->rch = lbc::fused_ring_sites(c->threads, c->max_lx, c->cfg.mode);
+    if (c->fused && !lbc::fused_lds_bytes(cfg->storage, c->ring_max, c->rch))
+      return bail(fail(c, LBC_E_ARG,
+                       "fused step kernel: LDS ring does not fit with these "
+                       "threads and storage"));
   }
+  if (int rc = upload_flips(c)) return bail(rc);
   if (hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(fail(c, LBC_E_HIP, "initialisation failed"));
   *out = c;
@@ -717,7 +753,7 @@ int lbc_advance(lbc_ctx* c, int n_intervals) {
     HIPC(c, hipGraphLaunch(c->gexec[1], c->stream));
     c->ev_used += 2;
     for (int i = 0; i < c->cfg.n_instances; i++)
-      c->parity[i] ^= (c->inst[i].steps + 1) & 1;
+      c->parity[i] ^= interval_flip(c, i);
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   return LBC_OK;
@@ -786,7 +822,7 @@ int lbc_step_host(int mode, const lbc_instance* geom, double* fin, double* fout,
   cfg.storage = LBC_STORE_F64;
   cfg.n_instances = 1;
   cfg.threads = 256;
-  cfg.flags = 1 | (1 << 3);
+  cfg.flags = 1 | (1 << 3) | (1 << 5);
   cfg.instances = &s;
   lbc_ctx* c = nullptr;
   int rc = lbc_create(&cfg, &c);

@@ -71,13 +71,21 @@ struct Launch {
   int ring_max;
   int mode;
   int storage;
-  int vec;           // sites per thread per iteration of the step kernel (1, 2, 4)
+  int vec;           // sites per thread per iteration of the step kernel (1)
   int nt;            // non-temporal population stores
+  int fused;         // two passes per sweep through an LDS ring
+  int rch;           // LDS ring sites per direction (fused)
   hipStream_t stream;
 };
 
 // Kernel launchers (lbc_kernels.hip).
 hipError_t launch_interval(const Launch& L);
+// LDS ring sites per direction the fused step kernel needs with `threads`
+// threads for lattices of row length up to max_lx in physics mode `mode`,
+// and the LDS bytes of the whole launch (0 if it does not fit in one CU's
+// 160 KiB).
+int fused_ring_sites(int threads, int max_lx, int mode);
+size_t fused_lds_bytes(int storage, int ring_max, int rch);
 hipError_t launch_flip(const Launch& L, int n_inst, const int* flip_mask_dev);
 hipError_t launch_perturb(const Launch& L, int philox_prefactors,
                           uint32_t seed_lo, uint32_t seed_hi);

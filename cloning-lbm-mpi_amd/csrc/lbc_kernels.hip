@@ -53,6 +53,7 @@ struct Traits<kF64> {
   static constexpr bool shifted = false;
   __device__ static C ld(const S* p) { return *p; }
   __device__ static void st(S* p, C v) { *p = v; }
+  __device__ static S pack(C v) { return v; }
   __device__ static C rest(int q) { return WD[q]; }
 };
 
@@ -63,6 +64,7 @@ struct Traits<kF32> {
   static constexpr bool shifted = true;
   __device__ static C ld(const S* p) { return *p; }
   __device__ static void st(S* p, C v) { *p = v; }
+  __device__ static S pack(C v) { return v; }
   __device__ static C rest(int) { return 0.0f; }
 };
 
@@ -73,6 +75,7 @@ struct Traits<kF16> {
   static constexpr bool shifted = true;
   __device__ static C ld(const S* p) { return (float)*p; }
   __device__ static void st(S* p, C v) { *p = (_Float16)v; }
+  __device__ static S pack(C v) { return (_Float16)v; }
   __device__ static C rest(int) { return 0.0f; }
 };
 
@@ -91,6 +94,7 @@ struct Traits<kBF16> {
   static constexpr bool shifted = true;
   __device__ static C ld(const S* p) { return __uint_as_float((uint32_t)*p << 16); }
   __device__ static void st(S* p, C v) { *p = f32_to_bf16(v); }
+  __device__ static S pack(C v) { return f32_to_bf16(v); }
   __device__ static C rest(int) { return 0.0f; }
 };
 
@@ -190,9 +194,29 @@ __device__ inline void collide_f64(double (&f)[9], double beta, double tau) {
 // for each pair of opposite directions (+e, -e) the equilibrium splits into
 // a common part t = drho + rho(4.5 eu^2 - 1.5 u^2) and an odd part
 // s = 3 rho eu, so feq_q - w_q = w_q (t +/- s).
-template <int MODE>
-__device__ inline void collide_shifted(float (&d)[9], float omega, float beta) {
-#pragma clang fp contract(fast)
+// a*b + c as a storage element of precision ST.  FP16: the exact a*b + c
+// rounded once to half (v_fma_mixlo_f16); FP32: fmaf; BF16: fmaf, then
+// round-to-nearest-even to bfloat16.  Spelling out the half case keeps the
+// compiler from choosing between one rounding (fma_mix) and two (fma, then
+// convert) differently in different kernels.
+template <int ST>
+__device__ inline typename Traits<ST>::S fma_store(float a, float b, float c) {
+  if constexpr (ST == kF16) {
+    unsigned r;
+    asm("v_fma_mixlo_f16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return __builtin_bit_cast(_Float16, (unsigned short)r);
+  } else {
+    return Traits<ST>::pack(fmaf(a, b, c));
+  }
+}
+
+template <int MODE, int ST>
+__device__ inline void collide_shifted(const float (&d)[9],
+                                       typename Traits<ST>::S (&o)[9],
+                                       float omega, float beta) {
+  // Every fused multiply-add is written out (the file compiles with
+  // contraction off), so the rounding is fixed by this source and every
+  // kernel that inlines it produces the same bits.
   const float drho = ((d[0] + d[1]) + (d[2] + d[3])) +
                      ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
   const float jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
@@ -200,9 +224,10 @@ __device__ inline void collide_shifted(float (&d)[9], float omega, float beta) {
   const float rho = 1.0f + drho;
   const float inv = __builtin_amdgcn_rcpf(rho);
   float ux = jx * inv;
-  if (MODE == kModeRef) ux += 0.5f * beta * inv;
+  if (MODE == kModeRef) ux = fmaf(0.5f * beta, inv, ux);
   const float uy = jy * inv;
-  const float base = drho - rho * (1.5f * (ux * ux + uy * uy));
+  const float usq = fmaf(ux, ux, uy * uy);
+  const float base = fmaf(-1.5f * rho, usq, drho);
   const float r45 = 4.5f * rho, r3 = 3.0f * rho;
   const float om1 = 1.0f - omega;
   const float o0 = omega * (4.0f / 9.0f), o1 = omega * (1.0f / 9.0f),
@@ -213,32 +238,32 @@ __device__ inline void collide_shifted(float (&d)[9], float omega, float beta) {
   const float t2 = fmaf(r45 * e2, e2, base), s2 = r3 * e2;
   const float t5 = fmaf(r45 * e5, e5, base), s5 = r3 * e5;
   const float t6 = fmaf(r45 * e6, e6, base), s6 = r3 * e6;
-  float n[9];
-  n[0] = fmaf(o0, base, om1 * d[0]);
-  n[1] = fmaf(o1, t1 + s1, om1 * d[1]);
-  n[3] = fmaf(o1, t1 - s1, om1 * d[3]);
-  n[2] = fmaf(o1, t2 + s2, om1 * d[2]);
-  n[4] = fmaf(o1, t2 - s2, om1 * d[4]);
-  n[5] = fmaf(o2, t5 + s5, om1 * d[5]);
-  n[7] = fmaf(o2, t5 - s5, om1 * d[7]);
-  n[6] = fmaf(o2, t6 + s6, om1 * d[6]);
-  n[8] = fmaf(o2, t6 - s6, om1 * d[8]);
-  if (MODE == kModeRef) {
+  // d_q' = omega_q (feq_q - w_q) + (1 - omega) d_q, omega_q = omega w_q
+  const float oq[9] = {o0, o1, o1, o1, o1, o2, o2, o2, o2};
+  const float tq[9] = {base,    t1 + s1, t2 + s2, t1 - s1, t2 - s2,
+                       t5 + s5, t6 + s6, t5 - s5, t6 - s6};
+  if (MODE != kModeRef) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) o[q] = fma_store<ST>(oq[q], tq[q], om1 * d[q]);
+  } else {
+    float n[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) n[q] = fmaf(oq[q], tq[q], om1 * d[q]);
     // Guo forcing w_q (1 - omega/2) beta (3 (cx - ux) + 9 cx eu)
     const float G = (1.0f - 0.5f * omega) * beta;
-    const float a = 6.0f * ux, c = 9.0f * uy;
-    n[0] += (4.0f / 9.0f) * G * (-3.0f * ux);
-    n[1] += (1.0f / 9.0f) * G * (3.0f + a);
-    n[3] += (1.0f / 9.0f) * G * (a - 3.0f);
-    n[2] += (1.0f / 9.0f) * G * (-3.0f * ux);
-    n[4] += (1.0f / 9.0f) * G * (-3.0f * ux);
-    n[5] += (1.0f / 36.0f) * G * (3.0f + a + c);
-    n[7] += (1.0f / 36.0f) * G * (a + c - 3.0f);
-    n[6] += (1.0f / 36.0f) * G * (a - c - 3.0f);
-    n[8] += (1.0f / 36.0f) * G * (3.0f + a - c);
+    const float g0 = (4.0f / 9.0f) * G, g1 = (1.0f / 9.0f) * G,
+                g2 = (1.0f / 36.0f) * G;
+    const float a = 6.0f * ux, c = 9.0f * uy, m3ux = -3.0f * ux;
+    o[0] = fma_store<ST>(g0, m3ux, n[0]);
+    o[1] = fma_store<ST>(g1, 3.0f + a, n[1]);
+    o[3] = fma_store<ST>(g1, a - 3.0f, n[3]);
+    o[2] = fma_store<ST>(g1, m3ux, n[2]);
+    o[4] = fma_store<ST>(g1, m3ux, n[4]);
+    o[5] = fma_store<ST>(g2, (3.0f + a) + c, n[5]);
+    o[7] = fma_store<ST>(g2, (a - 3.0f) + c, n[7]);
+    o[6] = fma_store<ST>(g2, (a - 3.0f) - c, n[6]);
+    o[8] = fma_store<ST>(g2, (3.0f + a) - c, n[8]);
   }
-#pragma unroll
-  for (int k = 0; k < 9; k++) d[k] = n[k];
 }
 
 // Moments and non-equilibrium stress of one node in the order of
@@ -414,6 +439,11 @@ struct BufIO<kF64> {
         __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
         vo, so, NT ? 2 : 0);
   }
+  template <bool NT>
+  __device__ static void st_raw(__amdgpu_buffer_rsrc_t r, int vo, int so,
+                                double v) {
+    st<NT>(r, vo, so, v);
+  }
 };
 
 template <>
@@ -426,6 +456,11 @@ struct BufIO<kF32> {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo,
                                           so, NT ? 2 : 0);
   }
+  template <bool NT>
+  __device__ static void st_raw(__amdgpu_buffer_rsrc_t r, int vo, int so,
+                                float v) {
+    st<NT>(r, vo, so, v);
+  }
 };
 
 template <>
@@ -435,9 +470,14 @@ struct BufIO<kF16> {
         _Float16, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0));
   }
   template <bool NT>
+  __device__ static void st_raw(__amdgpu_buffer_rsrc_t r, int vo, int so,
+                                _Float16 v) {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v),
+                                          r, vo, so, NT ? 2 : 0);
+  }
+  template <bool NT>
   __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float v) {
-    __builtin_amdgcn_raw_buffer_store_b16(
-        __builtin_bit_cast(unsigned short, (_Float16)v), r, vo, so, NT ? 2 : 0);
+    st_raw<NT>(r, vo, so, (_Float16)v);
   }
 };
 
@@ -448,151 +488,58 @@ struct BufIO<kBF16> {
     return __uint_as_float((uint32_t)b << 16);
   }
   template <bool NT>
+  __device__ static void st_raw(__amdgpu_buffer_rsrc_t r, int vo, int so,
+                                uint16_t v) {
+    __builtin_amdgcn_raw_buffer_store_b16(v, r, vo, so, NT ? 2 : 0);
+  }
+  template <bool NT>
   __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float v) {
-    __builtin_amdgcn_raw_buffer_store_b16(f32_to_bf16(v), r, vo, so, NT ? 2 : 0);
+    st_raw<NT>(r, vo, so, f32_to_bf16(v));
   }
 };
 
-// One site of one pass.  src/dst point at this clone's populations.
-template <int MODE, int ST, int PH>
-__device__ inline void do_site(const Geo& g, int x, int y,
-                               __amdgpu_buffer_rsrc_t rs,
-                               __amdgpu_buffer_rsrc_t rd,
-                               double* __restrict__ ring, double beta,
-                               double tau, const InstDev* __restrict__ I) {
-  using T = Traits<ST>;
-  using C = typename T::C;
-  constexpr int ES = (int)sizeof(typename T::S);
-  const int site = y * g.Lx + x;
-  const bool in_box = (x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1);
-  if (PH != kCollide && in_box && x > g.x0 && x < g.x1 && y > g.y0 &&
-      y < g.y1) {
-    // solid interior: rest state (the reference resets it to w every step;
-    // its post-collision values never reach a fluid node).  The collide-only
-    // pass collides it like the reference does.
-#pragma unroll
-    for (int q = 0; q < 9; q++) BufIO<ST>::template st<false>(rd, (q * g.plane + site) * ES, 0, T::rest(q));
-    return;
+// Population storage of one clone as seen by one pass: HBM through a
+// buffer resource (soffset carries the wave-uniform plane offset), or the
+// LDS ring of the two-level sweep, which holds `rch` consecutive sites
+// (a power of two) per direction.
+template <int ST>
+struct HbmIO {
+  __amdgpu_buffer_rsrc_t r;
+  int plane;
+  static constexpr int ES = (int)sizeof(typename Traits<ST>::S);
+  __device__ typename Traits<ST>::C ld(int q, int site) const {
+    return BufIO<ST>::ld(r, (q * plane + site) * ES, 0);
   }
-  C f[9];
-  if (PH == kCollide) {
-#pragma unroll
-    for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::ld(rs, (q * g.plane + site) * ES, 0);
-  } else {
-    const bool plain = !in_box && y >= 1 && y <= g.Ly - 2 && x >= 1 &&
-                       x <= g.Lx - 2;
-    if (plain) {
-#pragma unroll
-      for (int q = 0; q < 9; q++)
-        f[q] = BufIO<ST>::ld(rs, (q * g.plane + site - CY[q] * g.Lx - CX[q]) * ES, 0);
-    } else {
-      const int m = wall_mask(y, g.Ly) | square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
-#pragma unroll
-      for (int q = 0; q < 9; q++) {
-        if ((m >> q) & 1) {
-          f[q] = BufIO<ST>::ld(rs, (OPP[q] * g.plane + site) * ES, 0);
-          continue;
-        }
-        int xs = x - CX[q];
-        const int ys = y - CY[q];
-        if (MODE == kModeRef) {
-          if (xs < 0) xs += g.Lx;
-          else if (xs >= g.Lx) xs -= g.Lx;
-        } else {
-          if (xs < 0) {
-            // unknown at the inlet: replaced by the inlet equilibrium below
-            f[q] = (C)0;
-            continue;
-          }
-          if (xs >= g.Lx) xs = g.Lx - 1;  // zero-gradient ghost column
-        }
-        f[q] = BufIO<ST>::ld(rs, (q * g.plane + ys * g.Lx + xs) * ES, 0);
-      }
-      if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1))
-        channel_x_boundary<ST>(f, x == 0, I->u_in);
-      if (in_box) {
-        // drag contribution of this ring node
-        if (MODE == kModeRef) {
-          double tW, tE, tN, tS;
-          if (!T::shifted) {
-#pragma clang fp contract(off)
-            double rho, pxx, pxy;
-            double ff[9];
-#pragma unroll
-            for (int q = 0; q < 9; q++) ff[q] = (double)f[q];
-            node_stress_f64(ff, rho, pxx, pxy);
-            const double ot = 1. / 3.;
-            const double cf = 1.0 - .5 * (1. / tau);
-            tW = rho * ot + cf * pxx;
-            tE = -rho * ot - cf * pxx;
-            tN = -cf * pxy;
-            tS = +cf * pxy;
-          } else {
-            float dr, pxx, pxy;
-            float dd[9];
-#pragma unroll
-            for (int q = 0; q < 9; q++) dd[q] = (float)f[q];
-            node_stress_shifted(dd, dr, pxx, pxy);
-            const float cf = 1.0f - 0.5f / (float)tau;
-            // the constant 1/3 of each node cancels between the west and
-            // east faces, which have the same number of nodes
-            tW = (double)(dr * (1.0f / 3.0f) + cf * pxx);
-            tE = -tW;
-            tN = (double)(-cf * pxy);
-            tS = -tN;
-          }
-          if (x == g.x0) ring[y - g.y0] = tW;
-          if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = tE;
-          if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = tN;
-          if (y == g.y0) ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = tS;
-        } else {
-          const int sm = square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
-          double t = 0.0;
-          if (!T::shifted) {
-#pragma clang fp contract(off)
-#pragma unroll
-            for (int q = 0; q < 9; q++)
-              if ((sm >> q) & 1) t += (double)(-2 * CX[q]) * (double)f[q];
-          } else {
-            float tf = 0.0f;
-#pragma unroll
-            for (int q = 0; q < 9; q++)
-              if ((sm >> q) & 1) tf += (float)(-2 * CX[q]) * (float)f[q];
-            t = (double)tf;
-          }
-          if (x == g.x0) ring[y - g.y0] = t;
-          else if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = t;
-          else if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = t;
-          else ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = t;
-        }
-      }
-    }
+  template <bool NT>
+  __device__ void st(int q, int site, typename Traits<ST>::S v) const {
+    BufIO<ST>::template st_raw<NT>(r, (q * plane + site) * ES, 0, v);
   }
-  if (PH != kStream) {
-    if (!T::shifted) {
-      double ff[9];
-#pragma unroll
-      for (int q = 0; q < 9; q++) ff[q] = (double)f[q];
-      collide_f64(ff, MODE == kModeRef ? beta : 0.0, tau);
-#pragma unroll
-      for (int q = 0; q < 9; q++) f[q] = (C)ff[q];
-    } else {
-      float dd[9];
-#pragma unroll
-      for (int q = 0; q < 9; q++) dd[q] = (float)f[q];
-      collide_shifted<MODE>(dd, (float)(1.0 / tau), (float)beta);
-#pragma unroll
-      for (int q = 0; q < 9; q++) f[q] = (C)dd[q];
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 9; q++)
-    BufIO<ST>::template st<false>(rd, (q * g.plane + site) * ES, 0, f[q]);
-}
+};
 
-// Collision of one site's populations in the storage's compute type.
+template <int ST, int RCH>
+struct LdsIO {
+  typedef __attribute__((address_space(3))) typename Traits<ST>::S LS;
+  static constexpr int ES = (int)sizeof(typename Traits<ST>::S);
+  static constexpr int BMASK = RCH * ES - 1;  // ring bytes per direction - 1
+  __attribute__((address_space(3))) char* p;
+  __device__ typename Traits<ST>::C ld_byte(int q, int boff) const {
+    typename Traits<ST>::S e = *(LS*)(p + q * RCH * ES + (boff & BMASK));
+    return Traits<ST>::ld(&e);
+  }
+  __device__ typename Traits<ST>::C ld(int q, int site) const {
+    return ld_byte(q, site * ES);
+  }
+  template <bool NT>
+  __device__ void st(int q, int site, typename Traits<ST>::S v) const {
+    *(LS*)(p + q * RCH * ES + ((site * ES) & BMASK)) = v;
+  }
+};
+
+// Collision of one site's populations in the storage's compute type; the
+// result comes out as storage elements.
 template <int MODE, int ST>
-__device__ inline void collide_any(typename Traits<ST>::C (&f)[9], double beta,
+__device__ inline void collide_any(const typename Traits<ST>::C (&f)[9],
+                                   typename Traits<ST>::S (&o)[9], double beta,
                                    double tau) {
   using T = Traits<ST>;
   if constexpr (!T::shifted) {
@@ -601,75 +548,328 @@ __device__ inline void collide_any(typename Traits<ST>::C (&f)[9], double beta,
     for (int q = 0; q < 9; q++) ff[q] = (double)f[q];
     collide_f64(ff, MODE == kModeRef ? beta : 0.0, tau);
 #pragma unroll
-    for (int q = 0; q < 9; q++) f[q] = (typename T::C)ff[q];
+    for (int q = 0; q < 9; q++) o[q] = ff[q];
   } else {
-    float dd[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) dd[q] = (float)f[q];
-    collide_shifted<MODE>(dd, (float)(1.0 / tau), (float)beta);
-#pragma unroll
-    for (int q = 0; q < 9; q++) f[q] = (typename T::C)dd[q];
+    collide_shifted<MODE, ST>(f, o, (float)(1.0 / tau), (float)beta);
   }
 }
 
-// One pass of one thread over its sites: site tid, tid + T, tid + 2T, ...
-// (T = workgroup size).  Sites off the wall rows and outside the square's
-// bounding box take the fast path: nine buffer loads at SGPR offsets
-// q*plane - cy*Lx - cx from the site, the x boundary folded into two
-// per-lane offsets (periodic wrap in REF mode; inlet/outlet ghosts in
-// channel mode, followed by the channel boundary rule), collision, nine
-// buffer stores.  Wall rows, ring and solid sites take the general rule.
-template <int MODE, int ST, int PH, bool NT>
-__device__ inline void site_pass(const Geo& g, int tid, int nthr,
-                                 const typename Traits<ST>::S* __restrict__ src,
-                                 typename Traits<ST>::S* __restrict__ dst,
-                                 double* __restrict__ ring, double beta,
-                                 double tau, const InstDev* __restrict__ I,
-                                 int clone_bytes) {
+// Storage elements of a site: collided (any pass but stream-only) or packed
+// as they are.
+template <int MODE, int ST>
+__device__ inline void finish_site(const typename Traits<ST>::C (&f)[9],
+                                   typename Traits<ST>::S (&o)[9], int PH,
+                                   double beta, double tau) {
+  if (PH != kStream) {
+    collide_any<MODE, ST>(f, o, beta, tau);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 9; q++) o[q] = Traits<ST>::pack(f[q]);
+  }
+}
+
+// General per-site rule: walls, square ring (with its drag term), solid
+// interior, x boundaries, plain pull.  SRC/DST are HbmIO or LdsIO.
+template <int MODE, int ST, bool NT, class SRC, class DST>
+__device__ inline void do_site(const Geo& g, int x, int y, const SRC& src,
+                               const DST& dst, double* __restrict__ ring,
+                               double beta, double tau,
+                               const InstDev* __restrict__ I, int PH) {
   using T = Traits<ST>;
   using C = typename T::C;
-  constexpr int ES = (int)sizeof(typename T::S);
-  const __amdgpu_buffer_rsrc_t rs = clone_rsrc(src, clone_bytes);
-  const __amdgpu_buffer_rsrc_t rd = clone_rsrc(dst, clone_bytes);
-  int soff[9], sst[9];
+  const int site = y * g.Lx + x;
+  const bool in_box = (x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1);
+  if (PH != kCollide && in_box && x > g.x0 && x < g.x1 && y > g.y0 &&
+      y < g.y1) {
+    // solid interior: rest state (the reference resets it to w every step;
+    // its post-collision values never reach a fluid node).  The collide-only
+    // pass collides it like the reference does.
 #pragma unroll
-  for (int q = 0; q < 9; q++) {
-    sst[q] = q * g.plane * ES;
-    soff[q] = (PH == kCollide) ? sst[q]
-                               : (q * g.plane - CY[q] * g.Lx - CX[q]) * ES;
+    for (int q = 0; q < 9; q++) dst.template st<NT>(q, site, T::pack(T::rest(q)));
+    return;
   }
-  // x-boundary offsets (elements) for pulls with cx = +1 at x = 0 and
-  // cx = -1 at x = Lx-1
-  const int adjL = (MODE == kModeRef) ? g.Lx : 1;
-  const int adjR = (MODE == kModeRef) ? -g.Lx : -1;
+  C f[9];
+  if (PH == kCollide) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) f[q] = src.ld(q, site);
+  } else {
+    const int m = wall_mask(y, g.Ly) | square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      if ((m >> q) & 1) {
+        f[q] = src.ld(OPP[q], site);
+        continue;
+      }
+      int xs = x - CX[q];
+      const int ys = y - CY[q];
+      if (MODE == kModeRef) {
+        if (xs < 0) xs += g.Lx;
+        else if (xs >= g.Lx) xs -= g.Lx;
+      } else {
+        if (xs < 0) {
+          // unknown at the inlet: replaced by the inlet equilibrium below
+          f[q] = (C)0;
+          continue;
+        }
+        if (xs >= g.Lx) xs = g.Lx - 1;  // zero-gradient ghost column
+      }
+      f[q] = src.ld(q, ys * g.Lx + xs);
+    }
+    if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1))
+      channel_x_boundary<ST>(f, x == 0, I->u_in);
+    if (in_box) {
+      // drag contribution of this ring node
+      if (MODE == kModeRef) {
+        double tW, tE, tN, tS;
+        if (!T::shifted) {
+#pragma clang fp contract(off)
+          double rho, pxx, pxy;
+          double ff[9];
+#pragma unroll
+          for (int q = 0; q < 9; q++) ff[q] = (double)f[q];
+          node_stress_f64(ff, rho, pxx, pxy);
+          const double ot = 1. / 3.;
+          const double cf = 1.0 - .5 * (1. / tau);
+          tW = rho * ot + cf * pxx;
+          tE = -rho * ot - cf * pxx;
+          tN = -cf * pxy;
+          tS = +cf * pxy;
+        } else {
+          float dr, pxx, pxy;
+          float dd[9];
+#pragma unroll
+          for (int q = 0; q < 9; q++) dd[q] = (float)f[q];
+          node_stress_shifted(dd, dr, pxx, pxy);
+          const float cf = 1.0f - 0.5f / (float)tau;
+          // the constant 1/3 of each node cancels between the west and
+          // east faces, which have the same number of nodes
+          tW = (double)(dr * (1.0f / 3.0f) + cf * pxx);
+          tE = -tW;
+          tN = (double)(-cf * pxy);
+          tS = -tN;
+        }
+        if (x == g.x0) ring[y - g.y0] = tW;
+        if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = tE;
+        if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = tN;
+        if (y == g.y0) ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = tS;
+      } else {
+        const int sm = square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
+        double t = 0.0;
+        if (!T::shifted) {
+#pragma clang fp contract(off)
+#pragma unroll
+          for (int q = 0; q < 9; q++)
+            if ((sm >> q) & 1) t += (double)(-2 * CX[q]) * (double)f[q];
+        } else {
+          float tf = 0.0f;
+#pragma unroll
+          for (int q = 0; q < 9; q++)
+            if ((sm >> q) & 1) tf += (float)(-2 * CX[q]) * (float)f[q];
+          t = (double)tf;
+        }
+        if (x == g.x0) ring[y - g.y0] = t;
+        else if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = t;
+        else if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = t;
+        else ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = t;
+      }
+    }
+  }
+  typename T::S o[9];
+  finish_site<MODE, ST>(f, o, PH, beta, tau);
+#pragma unroll
+  for (int q = 0; q < 9; q++) dst.template st<NT>(q, site, o[q]);
+}
+
+// Pulls of a fast-path site.  From HBM: one buffer load per direction at
+// the site's byte offset with the wave-uniform soffset q*plane - cy*Lx - cx;
+// from the LDS ring: the ring slot of site - cy*Lx - cx.  voL / voR (sL / sR)
+// fold in the x boundary for directions with cx = +1 / -1.
+template <int ST>
+__device__ inline void pull9(const HbmIO<ST>& io, typename Traits<ST>::C (&f)[9],
+                             int site, int adjL, int adjR, const int (&soff)[9]) {
+  constexpr int ES = HbmIO<ST>::ES;
+  const int vo = site * ES, voL = vo + adjL * ES, voR = vo + adjR * ES;
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+    f[q] = BufIO<ST>::ld(io.r, CX[q] > 0 ? voL : (CX[q] < 0 ? voR : vo), soff[q]);
+}
+
+template <int ST, int RCH>
+__device__ inline void pull9(const LdsIO<ST, RCH>& io,
+                             typename Traits<ST>::C (&f)[9], int site, int adjL,
+                             int adjR, const int (&boff)[9]) {
+  constexpr int ES = LdsIO<ST, RCH>::ES;
+  const int b = site * ES, bL = b + adjL * ES, bR = b + adjR * ES;
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+    f[q] = io.ld_byte(q, (CX[q] > 0 ? bL : (CX[q] < 0 ? bR : b)) - boff[q]);
+}
+
+template <int ST>
+__device__ inline void own9(const HbmIO<ST>& io, typename Traits<ST>::C (&f)[9],
+                            int site, const int (&sown)[9]) {
+  constexpr int ES = HbmIO<ST>::ES;
+#pragma unroll
+  for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::ld(io.r, site * ES, sown[q]);
+}
+
+template <int ST, int RCH>
+__device__ inline void own9(const LdsIO<ST, RCH>& io,
+                            typename Traits<ST>::C (&f)[9], int site,
+                            const int (&)[9]) {
+#pragma unroll
+  for (int q = 0; q < 9; q++) f[q] = io.ld(q, site);
+}
+
+template <int ST, bool NT>
+__device__ inline void put9(const HbmIO<ST>& io, const typename Traits<ST>::S (&o)[9],
+                            int site, const int (&sown)[9]) {
+  constexpr int ES = HbmIO<ST>::ES;
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+    BufIO<ST>::template st_raw<NT>(io.r, site * ES, sown[q], o[q]);
+}
+
+template <int ST, bool NT, int RCH>
+__device__ inline void put9(const LdsIO<ST, RCH>& io,
+                            const typename Traits<ST>::S (&o)[9], int site,
+                            const int (&)[9]) {
+#pragma unroll
+  for (int q = 0; q < 9; q++) io.template st<NT>(q, site, o[q]);
+}
+
+// Per-pass constants of the fast path.
+template <int ST>
+struct PassConst {
+  int boff[9];  // pull offset of direction q, in bytes (LDS source)
+  int soff[9];  // pull soffset of direction q, in bytes (HBM source)
+  int sown[9];  // own-site soffset of direction q, in bytes
+  int adjL, adjR;
+  __device__ PassConst(const Geo& g, int mode) {
+    constexpr int ES = (int)sizeof(typename Traits<ST>::S);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      const int off = CY[q] * g.Lx + CX[q];
+      boff[q] = off * ES;
+      sown[q] = q * g.plane * ES;
+      soff[q] = (q * g.plane - off) * ES;
+    }
+    adjL = (mode == kModeRef) ? g.Lx : 1;
+    adjR = (mode == kModeRef) ? -g.Lx : -1;
+  }
+};
+
+// One site of one pass.  Sites off the wall rows and outside the square's
+// bounding box take the fast path: nine pulls, the x boundary folded into
+// two per-lane offsets (periodic wrap in REF mode; inlet/outlet ghosts in
+// channel mode, followed by the channel boundary rule), collision, nine
+// stores.  Wall rows, ring and solid sites take the general rule.
+template <int MODE, int ST, bool NT, class SRC, class DST>
+__device__ inline void site_step(const Geo& g, int x, int y, int site,
+                                 const SRC& src, const DST& dst,
+                                 const PassConst<ST>& pc,
+                                 double* __restrict__ ring, double beta,
+                                 double tau, const InstDev* __restrict__ I,
+                                 int PH) {
+  using C = typename Traits<ST>::C;
+  const bool in_box = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1;
+  const bool fast = (PH == kCollide) || (!in_box && y >= 1 && y <= g.Ly - 2);
+  if (fast) {
+    C f[9];
+    if (PH == kCollide) {
+      own9(src, f, site, pc.sown);
+    } else {
+      const int aL = (x == 0) ? pc.adjL : 0;
+      const int aR = (x == g.Lx - 1) ? pc.adjR : 0;
+      if constexpr (__is_same(SRC, HbmIO<ST>))
+        pull9<ST>(src, f, site, aL, aR, pc.soff);
+      else
+        pull9(src, f, site, aL, aR, pc.boff);
+      if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1))
+        channel_x_boundary<ST>(f, x == 0, I->u_in);
+    }
+    typename Traits<ST>::S o[9];
+    finish_site<MODE, ST>(f, o, PH, beta, tau);
+    put9<ST, NT>(dst, o, site, pc.sown);
+  } else {
+    do_site<MODE, ST, NT>(g, x, y, src, dst, ring, beta, tau, I, PH);
+  }
+}
+
+// One HBM-to-HBM pass of one thread over its sites tid, tid + T, ...
+template <int MODE, int ST, bool NT>
+__device__ inline void site_pass(const Geo& g, int tid, int nthr,
+                                 const HbmIO<ST>& src, const HbmIO<ST>& dst,
+                                 double* __restrict__ ring, double beta,
+                                 double tau, const InstDev* __restrict__ I,
+                                 int PH) {
+  const PassConst<ST> pc(g, MODE);
   const int dx = nthr % g.Lx, dy = nthr / g.Lx;
   int x = tid % g.Lx, y = tid / g.Lx, site = tid;
   while (y < g.Ly) {
-    const bool in_box = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1;
-    const bool fast = (PH == kCollide) || (!in_box && y >= 1 && y <= g.Ly - 2);
-    if (fast) {
-      const int vo = site * ES;
-      const int voL = (PH != kCollide && x == 0) ? vo + adjL * ES : vo;
-      const int voR = (PH != kCollide && x == g.Lx - 1) ? vo + adjR * ES : vo;
-      C f[9];
-#pragma unroll
-      for (int q = 0; q < 9; q++)
-        f[q] = BufIO<ST>::ld(rs, CX[q] > 0 ? voL : (CX[q] < 0 ? voR : vo),
-                             soff[q]);
-      if (MODE == kModeChannel && PH != kCollide && (x == 0 || x == g.Lx - 1))
-        channel_x_boundary<ST>(f, x == 0, I->u_in);
-      if (PH != kStream) collide_any<MODE, ST>(f, beta, tau);
-#pragma unroll
-      for (int q = 0; q < 9; q++) BufIO<ST>::template st<NT>(rd, vo, sst[q], f[q]);
-    } else {
-      do_site<MODE, ST, PH>(g, x, y, rs, rd, ring, beta, tau, I);
-    }
+    site_step<MODE, ST, NT>(g, x, y, site, src, dst, pc, ring, beta, tau, I, PH);
     site += nthr;
     x += dx;
     y += dy;
     if (x >= g.Lx) {
       x -= g.Lx;
       y += 1;
+    }
+  }
+}
+
+// Reach of a pull in sites: Lx + 1 with clamped x boundaries; with the
+// periodic x wrap a site at x = 0 pulls from x = Lx-1 one row away, up to
+// 2*Lx - 1 sites ahead (and symmetrically behind).
+__host__ __device__ inline int pull_reach(int mode, int Lx) {
+  return mode == kModeRef ? 2 * Lx : Lx + 1;
+}
+
+// Two passes in one sweep over the lattice: pass A reads HBM and writes its
+// post-collision populations into the LDS ring, pass B runs L chunks behind
+// it, reads the ring and writes HBM.  A chunk is T consecutive sites (one per
+// thread); L*T >= the pull reach, so every pull of a B site finds its source
+// already in the ring, and the ring (>= 2L + 2 chunks) is never overwritten
+// while a B site still needs it.  One barrier per chunk.
+template <int MODE, int ST, bool NT, int RCH>
+__device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
+                                   const HbmIO<ST>& src, const HbmIO<ST>& dst,
+                                   const LdsIO<ST, RCH>& mid, double* ringA,
+                                   double* ringB, double beta, double tau,
+                                   const InstDev* __restrict__ I, int PA,
+                                   int PB) {
+  const PassConst<ST> pc(g, MODE);
+  const int CH = nthr;
+  const int L = (pull_reach(MODE, g.Lx) + CH - 1) / CH;
+  const int nch = (g.plane + CH - 1) / CH;
+  const int dx = CH % g.Lx, dy = CH / g.Lx;
+  int xa = tid % g.Lx, ya = tid / g.Lx, sa = tid;
+  int xb = xa, yb = ya, sb = tid;
+  for (int c = 0; c < nch + L; c++) {
+    if (c < nch) {
+      if (sa < g.plane)
+        site_step<MODE, ST, false>(g, xa, ya, sa, src, mid, pc, ringA, beta,
+                                   tau, I, PA);
+      sa += CH;
+      xa += dx;
+      ya += dy;
+      if (xa >= g.Lx) {
+        xa -= g.Lx;
+        ya += 1;
+      }
+    }
+    __syncthreads();
+    if (c >= L) {
+      if (sb < g.plane)
+        site_step<MODE, ST, NT>(g, xb, yb, sb, mid, dst, pc, ringB, beta,
+                                tau, I, PB);
+      sb += CH;
+      xb += dx;
+      yb += dy;
+      if (xb >= g.Lx) {
+        xb -= g.Lx;
+        yb += 1;
+      }
     }
   }
 }
@@ -690,70 +890,140 @@ __device__ inline double ring_sum(const double* ring, int sx, int sy) {
   return fW + fE + fN + fS;
 }
 
-// One cloning interval of one clone per workgroup.  The post-stream state
-// S_0 is in buf[parity]; pass 0 collides it, passes 1..M-1 stream and
-// collide, pass M streams only, so the interval ends with S_M in
-// buf[parity ^ ((M+1)&1)].  Drag F_n is taken on S_n in pass n.
+struct KernelSetup {
+  Geo g;
+  int M;
+  double beta, tau, scale, shift;
+  void* A;
+  void* B;
+  double* drag;
+  int clone_bytes;
+};
+
+template <int ST>
+__device__ inline KernelSetup setup_clone(const InstDev* __restrict__ I,
+                                          const WorkItem& wi, int par) {
+  using S = typename Traits<ST>::S;
+  KernelSetup k;
+  k.g.Lx = I->Lx;
+  k.g.Ly = I->Ly;
+  k.g.x0 = I->xmin;
+  k.g.x1 = I->xmax;
+  k.g.y0 = I->ymin;
+  k.g.y1 = I->ymax;
+  k.g.plane = I->plane;
+  k.g.sx = k.g.x1 - k.g.x0;
+  k.g.sy = k.g.y1 - k.g.y0;
+  k.M = I->steps;
+  k.beta = I->beta;
+  k.tau = I->tau;
+  k.scale = I->scale;
+  k.shift = I->shift;
+  k.A = (S*)I->buf[par] + (long long)wi.clone * I->clone_stride;
+  k.B = (S*)I->buf[par ^ 1] + (long long)wi.clone * I->clone_stride;
+  k.drag = I->drag + (long long)wi.clone * k.M;
+  k.clone_bytes = (int)(I->clone_stride * (long long)sizeof(S));
+  return k;
+}
+
+// Drag of pass n (n >= 1) from its slots: F*scale - shift, accumulated in
+// FP64 in pass order (src/mpi_LBM_GK_pops_perturb.cpp:445-459).
+template <int MODE>
+__device__ inline void take_drag(const KernelSetup& k, const double* ring, int n,
+                                 double& acc) {
+#pragma clang fp contract(off)
+  double F = ring_sum<MODE>(ring, k.g.sx, k.g.sy);
+  F = F * k.scale;
+  F = F - k.shift;
+  k.drag[n - 1] = F;
+  acc += F;
+}
+
+template <int MODE>
+__device__ inline void finish_clone(const InstDev* __restrict__ I,
+                                    const WorkItem& wi, double acc) {
+#pragma clang fp contract(off)
+  const double o = acc * I->dt;
+  I->obs[wi.clone] = o;
+  I->logw[wi.clone] = I->bias * o;
+}
+
+// One cloning interval of one clone per workgroup, one pass per sweep.  The
+// post-stream state S_0 is in buf[parity]; pass 0 collides it, passes
+// 1..M-1 stream and collide, pass M streams only, so the interval ends with
+// S_M in buf[parity ^ ((M+1)&1)].  Drag F_n is taken on S_n in pass n.
 template <int MODE, int ST, bool NT>
 __global__ void __launch_bounds__(1024)
     k_interval(const InstDev* __restrict__ insts,
                const WorkItem* __restrict__ items,
                const Counters* __restrict__ counters, int ring_max) {
-  using S = typename Traits<ST>::S;
   extern __shared__ double ring_lds[];
   const WorkItem wi = items[blockIdx.x];
   const InstDev* __restrict__ I = insts + wi.inst;
-  Geo g;
-  g.Lx = I->Lx;
-  g.Ly = I->Ly;
-  g.x0 = I->xmin;
-  g.x1 = I->xmax;
-  g.y0 = I->ymin;
-  g.y1 = I->ymax;
-  g.plane = I->plane;
-  g.sx = g.x1 - g.x0;
-  g.sy = g.y1 - g.y0;
-  const int M = I->steps;
-  const double beta = I->beta, tau = I->tau;
-  const double scale = I->scale, shift = I->shift;
-  const int par = counters->parity[wi.inst];
-  S* A = (S*)I->buf[par] + (long long)wi.clone * I->clone_stride;
-  S* B = (S*)I->buf[par ^ 1] + (long long)wi.clone * I->clone_stride;
-  double* drag = I->drag + (long long)wi.clone * M;
-
+  const KernelSetup k = setup_clone<ST>(I, wi, counters->parity[wi.inst]);
   const int nthr = blockDim.x, tid = threadIdx.x;
-  const int clone_bytes = (int)(I->clone_stride * (long long)sizeof(S));
+  const HbmIO<ST> ioA{clone_rsrc(k.A, k.clone_bytes), k.g.plane};
+  const HbmIO<ST> ioB{clone_rsrc(k.B, k.clone_bytes), k.g.plane};
   double acc = 0.0;
-
-  for (int n = 0; n <= M; ++n) {
-    const S* src = (n & 1) ? B : A;
-    S* dst = (n & 1) ? A : B;
+  for (int n = 0; n <= k.M; ++n) {
+    const HbmIO<ST>& src = (n & 1) ? ioB : ioA;
+    const HbmIO<ST>& dst = (n & 1) ? ioA : ioB;
     double* ring = ring_lds + (n & 1) * ring_max;
-    if (n == 0)
-      site_pass<MODE, ST, kCollide, NT>(g, tid, nthr, src, dst, ring,
-                                    beta, tau, I, clone_bytes);
-    else if (n == M)
-      site_pass<MODE, ST, kStream, NT>(g, tid, nthr, src, dst, ring,
-                                   beta, tau, I, clone_bytes);
-    else
-      site_pass<MODE, ST, kBoth, NT>(g, tid, nthr, src, dst, ring,
-                                 beta, tau, I, clone_bytes);
+    const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
+    site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, ring, k.beta, k.tau, I,
+                            ph);
     __syncthreads();
-    if (n >= 1 && tid == 0) {
-#pragma clang fp contract(off)
-      double F = ring_sum<MODE>(ring, g.sx, g.sy);
-      F = F * scale;
-      F = F - shift;
-      drag[n - 1] = F;
-      acc += F;
+    if (n >= 1 && tid == 0) take_drag<MODE>(k, ring, n, acc);
+  }
+  if (tid == 0) finish_clone<MODE>(I, wi, acc);
+}
+
+// The same interval, two passes per sweep through HBM (fused_sweep): passes
+// (0,1), (2,3), ...; with M even the last pass M runs alone.  The interval
+// ends with S_M in buf[parity ^ (S & 1)], S = ceil((M+1)/2) sweeps.  LDS:
+// four drag slot arrays (two per sweep, double-buffered over sweeps), then
+// the population ring of 9*rch sites.
+template <int MODE, int ST, bool NT, int RCH>
+__global__ void __launch_bounds__(1024)
+    k_interval_fused(const InstDev* __restrict__ insts,
+                     const WorkItem* __restrict__ items,
+                     const Counters* __restrict__ counters, int ring_max) {
+  extern __shared__ double lds[];
+  const WorkItem wi = items[blockIdx.x];
+  const InstDev* __restrict__ I = insts + wi.inst;
+  const KernelSetup k = setup_clone<ST>(I, wi, counters->parity[wi.inst]);
+  const int nthr = blockDim.x, tid = threadIdx.x;
+  const HbmIO<ST> ioA{clone_rsrc(k.A, k.clone_bytes), k.g.plane};
+  const HbmIO<ST> ioB{clone_rsrc(k.B, k.clone_bytes), k.g.plane};
+  const LdsIO<ST, RCH> mid{
+      (__attribute__((address_space(3))) char*)(lds + 4 * ring_max)};
+  double acc = 0.0;
+  int sweep = 0;
+  for (int n = 0; n <= k.M; sweep++) {
+    const HbmIO<ST>& src = (sweep & 1) ? ioB : ioA;
+    const HbmIO<ST>& dst = (sweep & 1) ? ioA : ioB;
+    double* ringA = lds + ((sweep & 1) * 2) * ring_max;
+    double* ringB = ringA + ring_max;
+    if (n + 1 <= k.M) {
+      const int PA = (n == 0) ? kCollide : kBoth;
+      const int PB = (n + 1 == k.M) ? kStream : kBoth;
+      fused_sweep<MODE, ST, NT, RCH>(k.g, tid, nthr, src, dst, mid, ringA, ringB,
+                                     k.beta, k.tau, I, PA, PB);
+      __syncthreads();
+      if (tid == 0) {
+        if (n >= 1) take_drag<MODE>(k, ringA, n, acc);
+        take_drag<MODE>(k, ringB, n + 1, acc);
+      }
+      n += 2;
+    } else {
+      site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, ringA, k.beta, k.tau, I,
+                              kStream);
+      __syncthreads();
+      if (tid == 0) take_drag<MODE>(k, ringA, n, acc);
+      n += 1;
     }
   }
-  if (tid == 0) {
-#pragma clang fp contract(off)
-    const double o = acc * I->dt;
-    I->obs[wi.clone] = o;
-    I->logw[wi.clone] = I->bias * o;
-  }
+  if (tid == 0) finish_clone<MODE>(I, wi, acc);
 }
 
 __global__ void k_flip(const InstDev* __restrict__ insts, Counters* counters,
@@ -1232,18 +1502,48 @@ __global__ void k_snap_to_soa(int n, const double* __restrict__ aos, int Lx,
   }
 }
 
+template <class K>
+hipError_t raise_lds_limit(K kernel, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  return hipFuncSetAttribute((const void*)kernel,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds);
+}
+
+template <int MODE, int ST, bool NT, int RCH>
+hipError_t launch_fused(const Launch& L) {
+  const size_t lds = sizeof(double) * 4 * (size_t)L.ring_max +
+                     9 * (size_t)RCH * sizeof(typename Traits<ST>::S);
+  hipError_t e = raise_lds_limit(k_interval_fused<MODE, ST, NT, RCH>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_interval_fused<MODE, ST, NT, RCH>), dim3(L.n_items),
+                     dim3(L.threads), lds, L.stream, L.insts, L.items,
+                     L.counters, L.ring_max);
+  return hipGetLastError();
+}
+
+template <int MODE, int ST, bool NT>
+hipError_t launch_interval_k(const Launch& L) {
+  if (L.fused) {
+    switch (L.rch) {
+      case 1024: return launch_fused<MODE, ST, NT, 1024>(L);
+      case 2048: return launch_fused<MODE, ST, NT, 2048>(L);
+      case 4096: return launch_fused<MODE, ST, NT, 4096>(L);
+      case 8192: return launch_fused<MODE, ST, NT, 8192>(L);
+    }
+    return hipErrorInvalidValue;
+  }
+  const size_t lds = sizeof(double) * 2 * (size_t)L.ring_max;
+  hipLaunchKernelGGL((k_interval<MODE, ST, NT>), dim3(L.n_items),
+                     dim3(L.threads), lds, L.stream, L.insts, L.items,
+                     L.counters, L.ring_max);
+  return hipGetLastError();
+}
+
 template <int MODE, int ST>
 hipError_t launch_interval_t(const Launch& L) {
-  const size_t lds = sizeof(double) * 2 * (size_t)L.ring_max;
-  if (L.nt)
-    hipLaunchKernelGGL((k_interval<MODE, ST, true>), dim3(L.n_items),
-                       dim3(L.threads), lds, L.stream, L.insts, L.items,
-                       L.counters, L.ring_max);
-  else
-    hipLaunchKernelGGL((k_interval<MODE, ST, false>), dim3(L.n_items),
-                       dim3(L.threads), lds, L.stream, L.insts, L.items,
-                       L.counters, L.ring_max);
-  return hipGetLastError();
+  return L.nt ? launch_interval_k<MODE, ST, true>(L)
+              : launch_interval_k<MODE, ST, false>(L);
 }
 
 template <int MODE>
@@ -1276,6 +1576,21 @@ hipError_t launch_gather_m(const Launch& L, uint32_t lo, uint32_t hi) {
 }
 
 }  // namespace
+
+int fused_ring_sites(int threads, int max_lx, int mode) {
+  const int L = (pull_reach(mode, max_lx) + threads - 1) / threads;
+  int R = 1;
+  while (R < 2 * L + 2) R <<= 1;
+  const int rch = R * threads;
+  return rch < 1024 ? 1024 : rch;  // compiled ring sizes: 1024 .. 8192
+}
+
+size_t fused_lds_bytes(int storage, int ring_max, int rch) {
+  if (rch > 8192) return 0;
+  const size_t es = storage == kF64 ? 8 : storage == kF32 ? 4 : 2;
+  const size_t b = sizeof(double) * 4 * (size_t)ring_max + 9 * (size_t)rch * es;
+  return b <= 160 * 1024 ? b : 0;
+}
 
 hipError_t launch_interval(const Launch& L) {
   if (L.n_items <= 0) return hipSuccess;

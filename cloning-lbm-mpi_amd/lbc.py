@@ -217,7 +217,8 @@ class Sampler:
 
     def __init__(self, specs, mode: int = MODE_CHANNEL,
                  storage: int = STORE_F64, seed: int = 0, threads: int = 0,
-                 device: int = 0, vec: int = 0, nt: int | None = None):
+                 device: int = 0, vec: int = 0, nt: int | None = None,
+                 fused: bool | None = None):
         if isinstance(specs, InstanceSpec):
             specs = [specs]
         self.specs = list(specs)
@@ -230,7 +231,10 @@ class Sampler:
             vec = 1
         if nt is None:
             nt = False if threads else None
-        flags = vec | ((0 if nt is None else (2 if nt else 1)) << 3)
+        if fused is None:
+            fused = False if threads else None
+        flags = (vec | ((0 if nt is None else (2 if nt else 1)) << 3)
+                 | ((0 if fused is None else (2 if fused else 1)) << 5))
         cfg = Config(mode, storage, len(self.specs), device, seed, threads,
                      flags, C.cast(arr, C.POINTER(Instance)))
         h = C.c_void_p()

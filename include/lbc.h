@@ -80,7 +80,9 @@ typedef struct lbc_config {
   int flags;                  /* step-kernel shape, 0 = tuned with threads:
                                  bits 0-2 sites per thread per iteration (1),
                                  bits 3-4 non-temporal population stores
-                                 (1 off, 2 on); other bits 0 */
+                                 (1 off, 2 on), bits 5-6 two passes per
+                                 sweep through an LDS ring (1 off, 2 on);
+                                 other bits 0 */
   const lbc_instance* instances;
 } lbc_config;
 

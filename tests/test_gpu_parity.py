@@ -44,16 +44,18 @@ def test_ref_step_host_bitexact(O, lbc, L):
     assert Fg == F
 
 
-@pytest.mark.parametrize("storage_threads", [256, 1024])
-def test_ref_interval_f64_bitexact(O, lbc, storage_threads):
+@pytest.mark.parametrize("storage_threads,fused", [(256, False), (1024, False),
+                                                 (256, True), (512, True)])
+def test_ref_interval_f64_bitexact(O, lbc, storage_threads, fused):
     L, M, N = 8, 37, 5
     spec = lbc.ref_box(L, 0.515, 0.05, N, M, alpha=3.0)
     g = O.ref_box_geom(L, 0.515, 0.05)
     rng = np.random.default_rng(11)
     states = np.stack([rand_state(O, g, rng) for _ in range(N)])
-    print(f"case: REF interval F64 L={L} M={M} N={N} threads={storage_threads}",
-          flush=True)
-    with lbc.Sampler(spec, mode=lbc.MODE_REF, threads=storage_threads) as s:
+    print(f"case: REF interval F64 L={L} M={M} N={N} threads={storage_threads} "
+          f"fused={fused}", flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_REF, threads=storage_threads,
+                     fused=fused) as s:
         s.load_states(0, 0, states)
         s.run_interval()
         got = s.store_states(0)
@@ -69,12 +71,15 @@ def test_ref_interval_f64_bitexact(O, lbc, storage_threads):
         assert lw[j] == spec.bias * acc
 
 
-@pytest.mark.parametrize("M", [1, 2, 24])
-def test_channel_interval_f64_bitexact(O, lbc, M):
+@pytest.mark.parametrize("M,fused", [(1, False), (2, False), (24, False),
+                                     (1, True), (2, True), (3, True),
+                                     (24, True)])
+def test_channel_interval_f64_bitexact(O, lbc, M, fused):
     spec = lbc.channel(80, 40, 8, 20, 20, 0.53, 0.05, 3, M, k=0.01)
     g = geom_of(O, spec, O.MODE_CHANNEL)
-    print(f"case: CHANNEL interval F64 80x40 M={M}", flush=True)
-    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, seed=5, threads=512) as s:
+    print(f"case: CHANNEL interval F64 80x40 M={M} fused={fused}", flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, seed=5, threads=256,
+                     fused=fused) as s:
         s.init_equilibrium(0, 1e-3)
         init = s.store_states(0)
         for j in range(3):
@@ -239,3 +244,37 @@ def test_low_precision_channel_interval(O, lbc, storage, tol):
               f"{acc:.9g} rel {rel:.2e}")
         assert rel < tol
         assert np.max(np.abs(got[j] - ref)) < 50 * tol
+
+
+@pytest.mark.parametrize("storage", [1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fused_sweeps_equal_single_passes(O, lbc, storage, mode):
+    """Two passes per sweep through the LDS ring give bit-identical states,
+    drag and weights to one pass per sweep (the ring holds the storage
+    precision), in both modes, with a lattice row longer than one chunk."""
+    if mode == 0:
+        spec = lbc.ref_box(40, 0.55, 0.05, 3, 7, alpha=2.0)   # 161x161
+    else:
+        spec = lbc.channel(300, 40, 8, 75, 20, 0.55, 0.05, 3, 8, k=0.5)
+    print(f"case: fused vs single passes storage={storage} mode={mode}",
+          flush=True)
+    out = []
+    for fused in (False, True):
+        with lbc.Sampler(spec, mode=mode, storage=storage, seed=3, threads=256,
+                         fused=fused) as s:
+            if mode == 0:
+                g = O.ref_box_geom(40, 0.55, 0.05)
+                rng = np.random.default_rng(1)
+                st = np.stack([rand_state(O, g, rng) for _ in range(3)])
+                s.load_states(0, 0, st)
+            else:
+                s.init_equilibrium(0, 1e-3)
+            s.run_interval()
+            s.run_interval()
+            out.append((s.store_states(0), s.drag(0), s.logweights(0)[1]))
+    (a, da, la), (b, db, lb) = out
+    for name, u, v in (("states", a, b), ("drag", da, db), ("logw", la, lb)):
+        bad = np.argwhere(u != v)
+        assert bad.size == 0, (
+            f"{name}: {len(bad)} of {u.size} differ, first at {bad[0].tolist()}"
+            f" ({u[tuple(bad[0])]!r} vs {v[tuple(bad[0])]!r})")
