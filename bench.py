@@ -19,10 +19,11 @@ Prints one JSON line.  Only N=1 is supported (one GPU).
 from __future__ import annotations
 
 import argparse
+import glob
 import json
 import math
 import os
-import statistics
+import re
 import sys
 import threading
 import time
@@ -49,11 +50,13 @@ def spec_for(n_clones: int, steps: int | None = None) -> lbc.InstanceSpec:
 def time_storage(storage: int, K: int, W: int, seed: int = 0,
                  shape=None) -> dict:
     """W untimed warm-up intervals then K timed ones (graph replays).
-    shape: (threads, sites per thread, non-temporal) or None (tuned)."""
+    shape: (threads, sites per thread, non-temporal, two passes per sweep)
+    or None (tuned at context creation)."""
     spec = spec_for(WORKLOAD["n_clones"])
     kw = {}
     if shape:
-        kw = dict(threads=shape[0], vec=shape[1], nt=bool(shape[2]))
+        kw = dict(threads=shape[0], vec=shape[1], nt=bool(shape[2]),
+                  fused=bool(shape[3]) if len(shape) > 3 else False)
     with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=storage,
                      seed=seed, **kw) as s:
         s.init_equilibrium(0, 1e-3)
@@ -68,6 +71,8 @@ def time_storage(storage: int, K: int, W: int, seed: int = 0,
         R = s.R_history(0)
         threads = s.threads
         vec = s.vec
+        nt = s.nt
+        fused = s.fused
     secs = t1 - t0
     lups = spec.n_clones * spec.sites * spec.steps * K
     bpl = lbc.bytes_per_lup(storage)
@@ -86,6 +91,8 @@ def time_storage(storage: int, K: int, W: int, seed: int = 0,
         roofline_frac=achieved / HBM_PEAK_GBS,
         threads=threads,
         sites_per_thread=vec,
+        nontemporal=nt,
+        passes_per_sweep=2 if fused else 1,
         logR_last=float(math.log(R[-1])) if len(R) else None,
     )
 
@@ -123,7 +130,7 @@ def accuracy(storage: int, n: int = 256, warm: int = 10, seed: int = 1) -> dict:
                 logR_fp64=float(logR64))
 
 
-def cpu_baseline(sample_clones_per_core: int = 2, steps: int = 200) -> dict:
+def cpu_baseline(sample_clones_per_core: int = 30, steps: int = 200) -> dict:
     """The CPU oracle (a port of the reference's AoS push step, serial clones
     per worker, one worker thread per core, like one MPI rank per core) on a
     bounded sample of the same workload."""
@@ -165,6 +172,22 @@ def cpu_baseline(sample_clones_per_core: int = 2, steps: int = 200) -> dict:
                         f"{secs:.1f} s on {cpu}"))
 
 
+def measured_traffic(run: dict) -> int | None:
+    """HBM bytes per step-kernel launch for this storage and kernel shape,
+    from the FETCH_SIZE / WRITE_SIZE summaries under profiles/ (copy-
+    calibrated there), or None when no summary covers the shape."""
+    key = (f"storage={run['storage']} threads={run['threads']} "
+           f"nt={int(run['nontemporal'])} passes={run['passes_per_sweep']}")
+    pat = re.compile(r"^traffic_bytes_per_launch (.*): (\d+)\s*$")
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
+        with open(path) as f:
+            for line in f:
+                m = pat.match(line.strip())
+                if m and m.group(1) == key:
+                    return int(m.group(2))
+    return None
+
+
 def calibrate_copy() -> int:
     """A plain device copy of 2^28 FP32 (1 GiB read, 1 GiB written), for
     scaling rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
@@ -192,8 +215,9 @@ def main() -> int:
                     help="run only a plain 1 GiB device copy (torch) so that "
                          "HBM counters can be scaled against known bytes")
     ap.add_argument("--shape", default="",
-                    help="threads,sites_per_thread,nontemporal for every "
-                         "storage (skips the tuning at context creation)")
+                    help="threads,sites_per_thread,nontemporal,two_pass for "
+                         "every storage (skips the tuning at context "
+                         "creation)")
     ap.add_argument("--only", default="",
                     help="comma list of storages to time (f64,f32,f16,bf16)")
     args = ap.parse_args()
@@ -255,7 +279,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(head["roofline_frac"], 4),
-            "traffic": None,
+            "traffic": measured_traffic(head),
         },
         "cpu_baseline": cpu,
         "paths": runs,

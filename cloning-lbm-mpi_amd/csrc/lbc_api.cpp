@@ -324,6 +324,11 @@ int lbc_threads(const lbc_ctx* ctx) { return ctx ? ctx->threads : 0; }
 
 int lbc_sites_per_thread(const lbc_ctx* ctx) { return ctx ? ctx->vec : 0; }
 
+int lbc_step_shape(const lbc_ctx* ctx) {
+  if (!ctx) return 0;
+  return ctx->vec | ((ctx->nt ? 2 : 1) << 3) | ((ctx->fused ? 2 : 1) << 5);
+}
+
 void lbc_destroy(lbc_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);

@@ -382,31 +382,39 @@ __device__ inline void channel_x_boundary(typename Traits<ST>::C (&f)[9],
       f[k] = WD[k] * rho * (1.0 + 3.0 * eu + eueu + u2);
     }
   } else {
-    // shifted: f = w + d, sum of the known weights is 1
-    float drho, ux, uy;
-    if (inlet) {
-      const float D = f[0] + f[2] + f[4] + 2.0f * (f[3] + f[6] + f[7]);
-      const float ui = (float)u_in;
-      drho = (D + ui) / (1.0f - ui);  // rho_in - 1
-      ux = ui;
-      uy = 0.0f;
-    } else {
-      const float dr = ((f[0] + f[1]) + (f[2] + f[3])) +
-                       ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
-      const float jx = (f[1] - f[3]) + (f[5] - f[6]) + (f[8] - f[7]);
-      const float jy = (f[2] - f[4]) + (f[5] + f[6]) - (f[7] + f[8]);
-      const float inv = 1.0f / (1.0f + dr);
-      ux = jx * inv;
-      uy = jy * inv;
-      drho = 0.0f;
-    }
+    // Shifted storage (f = w + d): both rules in one branch-free evaluation,
+    // so a wave holding inlet and outlet lanes runs it once.
+    const float ui = (float)u_in;
+    const float kin = (float)(1.0 / (1.0 - u_in));
+    const float D = (f[0] + f[2] + f[4]) + 2.0f * (f[3] + f[6] + f[7]);
+    const float dr = ((f[0] + f[1]) + (f[2] + f[3])) +
+                     ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
+    const float jx = (f[1] - f[3]) + (f[5] - f[6]) + (f[8] - f[7]);
+    const float jy = (f[2] - f[4]) + (f[5] + f[6]) - (f[7] + f[8]);
+    const float inv = __builtin_amdgcn_rcpf(1.0f + dr);
+    // inlet: rho_in - 1 = (D + u_in) / (1 - u_in), since the weights of the
+    // known directions sum to 1 - (w1 + w5 + w8) and w-shifts cancel
+    const float drho = inlet ? (D + ui) * kin : 0.0f;
+    const float ux = inlet ? ui : jx * inv;
+    const float uy = inlet ? 0.0f : jy * inv;
     const float rho = 1.0f + drho;
-    const float usq = 1.5f * (ux * ux + uy * uy);
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-      const float eu = (float)CX[k] * ux + (float)CY[k] * uy;
-      f[k] = WF[k] * (drho + rho * (3.0f * eu + 4.5f * eu * eu - usq));
-    }
+    const float usq = 1.5f * fmaf(ux, ux, uy * uy);
+    const float A = fmaf(-rho, usq, drho);
+    const float r45 = 4.5f * rho, r3 = 3.0f * rho;
+    const float e1 = ux, e2 = uy, e5 = ux + uy, e6 = uy - ux;
+    const float t1 = fmaf(r45 * e1, e1, A), s1 = r3 * e1;
+    const float t2 = fmaf(r45 * e2, e2, A), s2 = r3 * e2;
+    const float t5 = fmaf(r45 * e5, e5, A), s5 = r3 * e5;
+    const float t6 = fmaf(r45 * e6, e6, A), s6 = r3 * e6;
+    f[0] = WF[0] * A;
+    f[1] = WF[1] * (t1 + s1);
+    f[3] = WF[3] * (t1 - s1);
+    f[2] = WF[2] * (t2 + s2);
+    f[4] = WF[4] * (t2 - s2);
+    f[5] = WF[5] * (t5 + s5);
+    f[7] = WF[7] * (t5 - s5);
+    f[6] = WF[6] * (t6 + s6);
+    f[8] = WF[8] * (t6 - s6);
   }
 }
 
@@ -568,116 +576,66 @@ __device__ inline void finish_site(const typename Traits<ST>::C (&f)[9],
   }
 }
 
-// General per-site rule: walls, square ring (with its drag term), solid
-// interior, x boundaries, plain pull.  SRC/DST are HbmIO or LdsIO.
-template <int MODE, int ST, bool NT, class SRC, class DST>
-__device__ inline void do_site(const Geo& g, int x, int y, const SRC& src,
-                               const DST& dst, double* __restrict__ ring,
-                               double beta, double tau,
-                               const InstDev* __restrict__ I, int PH) {
+// Drag term of a ring node from its post-stream populations, written to
+// the slot(s) of its face(s) in the reference's face order.
+template <int MODE, int ST>
+__device__ inline void ring_drag(const Geo& g, int x, int y,
+                                 const typename Traits<ST>::C (&f)[9],
+                                 double* __restrict__ ring, double tau) {
   using T = Traits<ST>;
-  using C = typename T::C;
-  const int site = y * g.Lx + x;
-  const bool in_box = (x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1);
-  if (PH != kCollide && in_box && x > g.x0 && x < g.x1 && y > g.y0 &&
-      y < g.y1) {
-    // solid interior: rest state (the reference resets it to w every step;
-    // its post-collision values never reach a fluid node).  The collide-only
-    // pass collides it like the reference does.
+  if (MODE == kModeRef) {
+    double tW, tE, tN, tS;
+    if (!T::shifted) {
+#pragma clang fp contract(off)
+      double rho, pxx, pxy;
+      double ff[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) dst.template st<NT>(q, site, T::pack(T::rest(q)));
-    return;
-  }
-  C f[9];
-  if (PH == kCollide) {
+      for (int q = 0; q < 9; q++) ff[q] = (double)f[q];
+      node_stress_f64(ff, rho, pxx, pxy);
+      const double ot = 1. / 3.;
+      const double cf = 1.0 - .5 * (1. / tau);
+      tW = rho * ot + cf * pxx;
+      tE = -rho * ot - cf * pxx;
+      tN = -cf * pxy;
+      tS = +cf * pxy;
+    } else {
+      float dr, pxx, pxy;
+      float dd[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) f[q] = src.ld(q, site);
+      for (int q = 0; q < 9; q++) dd[q] = (float)f[q];
+      node_stress_shifted(dd, dr, pxx, pxy);
+      const float cf = 1.0f - 0.5f / (float)tau;
+      // the constant 1/3 of each node cancels between the west and east
+      // faces, which have the same number of nodes
+      tW = (double)(dr * (1.0f / 3.0f) + cf * pxx);
+      tE = -tW;
+      tN = (double)(-cf * pxy);
+      tS = -tN;
+    }
+    if (x == g.x0) ring[y - g.y0] = tW;
+    if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = tE;
+    if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = tN;
+    if (y == g.y0) ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = tS;
   } else {
-    const int m = wall_mask(y, g.Ly) | square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-      if ((m >> q) & 1) {
-        f[q] = src.ld(OPP[q], site);
-        continue;
-      }
-      int xs = x - CX[q];
-      const int ys = y - CY[q];
-      if (MODE == kModeRef) {
-        if (xs < 0) xs += g.Lx;
-        else if (xs >= g.Lx) xs -= g.Lx;
-      } else {
-        if (xs < 0) {
-          // unknown at the inlet: replaced by the inlet equilibrium below
-          f[q] = (C)0;
-          continue;
-        }
-        if (xs >= g.Lx) xs = g.Lx - 1;  // zero-gradient ghost column
-      }
-      f[q] = src.ld(q, ys * g.Lx + xs);
-    }
-    if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1))
-      channel_x_boundary<ST>(f, x == 0, I->u_in);
-    if (in_box) {
-      // drag contribution of this ring node
-      if (MODE == kModeRef) {
-        double tW, tE, tN, tS;
-        if (!T::shifted) {
-#pragma clang fp contract(off)
-          double rho, pxx, pxy;
-          double ff[9];
-#pragma unroll
-          for (int q = 0; q < 9; q++) ff[q] = (double)f[q];
-          node_stress_f64(ff, rho, pxx, pxy);
-          const double ot = 1. / 3.;
-          const double cf = 1.0 - .5 * (1. / tau);
-          tW = rho * ot + cf * pxx;
-          tE = -rho * ot - cf * pxx;
-          tN = -cf * pxy;
-          tS = +cf * pxy;
-        } else {
-          float dr, pxx, pxy;
-          float dd[9];
-#pragma unroll
-          for (int q = 0; q < 9; q++) dd[q] = (float)f[q];
-          node_stress_shifted(dd, dr, pxx, pxy);
-          const float cf = 1.0f - 0.5f / (float)tau;
-          // the constant 1/3 of each node cancels between the west and
-          // east faces, which have the same number of nodes
-          tW = (double)(dr * (1.0f / 3.0f) + cf * pxx);
-          tE = -tW;
-          tN = (double)(-cf * pxy);
-          tS = -tN;
-        }
-        if (x == g.x0) ring[y - g.y0] = tW;
-        if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = tE;
-        if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = tN;
-        if (y == g.y0) ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = tS;
-      } else {
-        const int sm = square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
-        double t = 0.0;
-        if (!T::shifted) {
+    const int sm = square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
+    double t = 0.0;
+    if (!T::shifted) {
 #pragma clang fp contract(off)
 #pragma unroll
-          for (int q = 0; q < 9; q++)
-            if ((sm >> q) & 1) t += (double)(-2 * CX[q]) * (double)f[q];
-        } else {
-          float tf = 0.0f;
+      for (int q = 0; q < 9; q++)
+        if ((sm >> q) & 1) t += (double)(-2 * CX[q]) * (double)f[q];
+    } else {
+      float tf = 0.0f;
 #pragma unroll
-          for (int q = 0; q < 9; q++)
-            if ((sm >> q) & 1) tf += (float)(-2 * CX[q]) * (float)f[q];
-          t = (double)tf;
-        }
-        if (x == g.x0) ring[y - g.y0] = t;
-        else if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = t;
-        else if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = t;
-        else ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = t;
-      }
+      for (int q = 0; q < 9; q++)
+        if ((sm >> q) & 1) tf += (float)(-2 * CX[q]) * (float)f[q];
+      t = (double)tf;
     }
+    if (x == g.x0) ring[y - g.y0] = t;
+    else if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = t;
+    else if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = t;
+    else ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = t;
   }
-  typename T::S o[9];
-  finish_site<MODE, ST>(f, o, PH, beta, tau);
-#pragma unroll
-  for (int q = 0; q < 9; q++) dst.template st<NT>(q, site, o[q]);
 }
 
 // Pulls of a fast-path site.  From HBM: one buffer load per direction at
@@ -686,9 +644,13 @@ __device__ inline void do_site(const Geo& g, int x, int y, const SRC& src,
 // fold in the x boundary for directions with cx = +1 / -1.
 template <int ST>
 __device__ inline void pull9(const HbmIO<ST>& io, typename Traits<ST>::C (&f)[9],
-                             int site, int adjL, int adjR, const int (&soff)[9]) {
+                             int site, int adjL, int adjR, const int (&soff)[9],
+                             int Lx) {
   constexpr int ES = HbmIO<ST>::ES;
-  const int vo = site * ES, voL = vo + adjL * ES, voR = vo + adjR * ES;
+  // A raw buffer access is range-checked on voffset alone, so voffset must
+  // stay non-negative: directions with cx = -1 carry +Lx sites in voffset
+  // and -Lx in their soffset (adjR >= -Lx).
+  const int vo = site * ES, voL = vo + adjL * ES, voR = vo + (adjR + Lx) * ES;
 #pragma unroll
   for (int q = 0; q < 9; q++)
     f[q] = BufIO<ST>::ld(io.r, CX[q] > 0 ? voL : (CX[q] < 0 ? voR : vo), soff[q]);
@@ -697,12 +659,17 @@ __device__ inline void pull9(const HbmIO<ST>& io, typename Traits<ST>::C (&f)[9]
 template <int ST, int RCH>
 __device__ inline void pull9(const LdsIO<ST, RCH>& io,
                              typename Traits<ST>::C (&f)[9], int site, int adjL,
-                             int adjR, const int (&boff)[9]) {
+                             int adjR, int rowb) {
   constexpr int ES = LdsIO<ST, RCH>::ES;
-  const int b = site * ES, bL = b + adjL * ES, bR = b + adjR * ES;
+  // source byte offsets: column x - cx (x boundary folded in), then the row
+  // y - cy one row-length (rowb bytes) away
+  const int b0 = site * ES;
+  const int bP = b0 + (adjL - 1) * ES, bM = b0 + (adjR + 1) * ES;
+  const int col[9] = {b0, bP, b0, bM, b0, bP, bM, bM, bP};
 #pragma unroll
   for (int q = 0; q < 9; q++)
-    f[q] = io.ld_byte(q, (CX[q] > 0 ? bL : (CX[q] < 0 ? bR : b)) - boff[q]);
+    f[q] = io.ld_byte(q, CY[q] > 0 ? col[q] - rowb
+                                   : (CY[q] < 0 ? col[q] + rowb : col[q]));
 }
 
 template <int ST>
@@ -741,8 +708,9 @@ __device__ inline void put9(const LdsIO<ST, RCH>& io,
 // Per-pass constants of the fast path.
 template <int ST>
 struct PassConst {
-  int boff[9];  // pull offset of direction q, in bytes (LDS source)
-  int soff[9];  // pull soffset of direction q, in bytes (HBM source)
+  int rowb;     // one lattice row in bytes (LDS source)
+  int soff[9];  // pull soffset of direction q, in bytes (HBM source; one
+                // row less for cx = -1, whose voffset is one row ahead)
   int sown[9];  // own-site soffset of direction q, in bytes
   int adjL, adjR;
   __device__ PassConst(const Geo& g, int mode) {
@@ -750,20 +718,23 @@ struct PassConst {
 #pragma unroll
     for (int q = 0; q < 9; q++) {
       const int off = CY[q] * g.Lx + CX[q];
-      boff[q] = off * ES;
       sown[q] = q * g.plane * ES;
-      soff[q] = (q * g.plane - off) * ES;
+      soff[q] = (q * g.plane - off - (CX[q] < 0 ? g.Lx : 0)) * ES;
     }
+    rowb = g.Lx * ES;
     adjL = (mode == kModeRef) ? g.Lx : 1;
     adjR = (mode == kModeRef) ? -g.Lx : -1;
   }
 };
 
-// One site of one pass.  Sites off the wall rows and outside the square's
-// bounding box take the fast path: nine pulls, the x boundary folded into
-// two per-lane offsets (periodic wrap in REF mode; inlet/outlet ghosts in
-// channel mode, followed by the channel boundary rule), collision, nine
-// stores.  Wall rows, ring and solid sites take the general rule.
+// One site of one pass.  Every site pulls its nine populations (the x
+// boundary folded into two per-lane offsets: periodic wrap in REF mode,
+// inlet/outlet ghost columns in channel mode).  A wave holding any wall-row
+// or square lane also loads each lane's own populations and takes the
+// bounce-back ones from there; square ring lanes add their drag term, solid
+// lanes store the rest state.  Channel x-boundary lanes then apply the
+// boundary rule.  Collision (except in the stream-only pass) and nine stores
+// follow.  The collide-only pass just loads the site's own populations.
 template <int MODE, int ST, bool NT, class SRC, class DST>
 __device__ inline void site_step(const Geo& g, int x, int y, int site,
                                  const SRC& src, const DST& dst,
@@ -771,29 +742,46 @@ __device__ inline void site_step(const Geo& g, int x, int y, int site,
                                  double* __restrict__ ring, double beta,
                                  double tau, const InstDev* __restrict__ I,
                                  int PH) {
-  using C = typename Traits<ST>::C;
-  const bool in_box = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1;
-  const bool fast = (PH == kCollide) || (!in_box && y >= 1 && y <= g.Ly - 2);
-  if (fast) {
-    C f[9];
-    if (PH == kCollide) {
-      own9(src, f, site, pc.sown);
-    } else {
-      const int aL = (x == 0) ? pc.adjL : 0;
-      const int aR = (x == g.Lx - 1) ? pc.adjR : 0;
-      if constexpr (__is_same(SRC, HbmIO<ST>))
-        pull9<ST>(src, f, site, aL, aR, pc.soff);
-      else
-        pull9(src, f, site, aL, aR, pc.boff);
+  using T = Traits<ST>;
+  using C = typename T::C;
+  C f[9];
+  bool solid = false;
+  if (PH == kCollide) {
+    own9(src, f, site, pc.sown);
+  } else {
+    const int aL = (x == 0) ? pc.adjL : 0;
+    const int aR = (x == g.Lx - 1) ? pc.adjR : 0;
+    if constexpr (__is_same(SRC, HbmIO<ST>))
+      pull9<ST>(src, f, site, aL, aR, pc.soff, g.Lx);
+    else
+      pull9(src, f, site, aL, aR, pc.rowb);
+    const bool in_box = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1;
+    const bool special = in_box || y == 0 || y == g.Ly - 1;
+    if (__builtin_amdgcn_ballot_w64(special) != 0) {
+      C own[9];
+      own9(src, own, site, pc.sown);
+      const int m = wall_mask(y, g.Ly) |
+                    square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
+#pragma unroll
+      for (int q = 0; q < 9; q++)
+        if ((m >> q) & 1) f[q] = own[OPP[q]];
+      solid = in_box && x > g.x0 && x < g.x1 && y > g.y0 && y < g.y1;
       if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1))
         channel_x_boundary<ST>(f, x == 0, I->u_in);
+      if (in_box && !solid) ring_drag<MODE, ST>(g, x, y, f, ring, tau);
+    } else if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1)) {
+      channel_x_boundary<ST>(f, x == 0, I->u_in);
     }
-    typename Traits<ST>::S o[9];
-    finish_site<MODE, ST>(f, o, PH, beta, tau);
-    put9<ST, NT>(dst, o, site, pc.sown);
-  } else {
-    do_site<MODE, ST, NT>(g, x, y, src, dst, ring, beta, tau, I, PH);
   }
+  typename T::S o[9];
+  finish_site<MODE, ST>(f, o, PH, beta, tau);
+  if (solid) {
+    // the reference resets the solid interior to w every step; its
+    // post-collision values never reach a fluid node
+#pragma unroll
+    for (int q = 0; q < 9; q++) o[q] = T::pack(T::rest(q));
+  }
+  put9<ST, NT>(dst, o, site, pc.sown);
 }
 
 // One HBM-to-HBM pass of one thread over its sites tid, tid + T, ...
@@ -981,13 +969,13 @@ __global__ void __launch_bounds__(1024)
 // The same interval, two passes per sweep through HBM (fused_sweep): passes
 // (0,1), (2,3), ...; with M even the last pass M runs alone.  The interval
 // ends with S_M in buf[parity ^ (S & 1)], S = ceil((M+1)/2) sweeps.  LDS:
-// four drag slot arrays (two per sweep, double-buffered over sweeps), then
-// the population ring of 9*rch sites.
+// the population ring of 9*rch sites, then four drag slot arrays (two per
+// sweep, double-buffered over sweeps).
 template <int MODE, int ST, bool NT, int RCH>
-__global__ void __launch_bounds__(1024)
-    k_interval_fused(const InstDev* __restrict__ insts,
-                     const WorkItem* __restrict__ items,
-                     const Counters* __restrict__ counters, int ring_max) {
+__device__ inline void interval_fused(const InstDev* __restrict__ insts,
+                                      const WorkItem* __restrict__ items,
+                                      const Counters* __restrict__ counters,
+                                      int ring_max) {
   extern __shared__ double lds[];
   const WorkItem wi = items[blockIdx.x];
   const InstDev* __restrict__ I = insts + wi.inst;
@@ -995,14 +983,15 @@ __global__ void __launch_bounds__(1024)
   const int nthr = blockDim.x, tid = threadIdx.x;
   const HbmIO<ST> ioA{clone_rsrc(k.A, k.clone_bytes), k.g.plane};
   const HbmIO<ST> ioB{clone_rsrc(k.B, k.clone_bytes), k.g.plane};
-  const LdsIO<ST, RCH> mid{
-      (__attribute__((address_space(3))) char*)(lds + 4 * ring_max)};
+  // population ring at the start of LDS, drag slots after it
+  const LdsIO<ST, RCH> mid{(__attribute__((address_space(3))) char*)lds};
+  double* const slots = lds + (9 * RCH * LdsIO<ST, RCH>::ES) / 8;
   double acc = 0.0;
   int sweep = 0;
   for (int n = 0; n <= k.M; sweep++) {
     const HbmIO<ST>& src = (sweep & 1) ? ioB : ioA;
     const HbmIO<ST>& dst = (sweep & 1) ? ioA : ioB;
-    double* ringA = lds + ((sweep & 1) * 2) * ring_max;
+    double* ringA = slots + ((sweep & 1) * 2) * ring_max;
     double* ringB = ringA + ring_max;
     if (n + 1 <= k.M) {
       const int PA = (n == 0) ? kCollide : kBoth;
@@ -1024,6 +1013,25 @@ __global__ void __launch_bounds__(1024)
     }
   }
   if (tid == 0) finish_clone<MODE>(I, wi, acc);
+}
+
+template <int MODE, int ST, bool NT, int RCH>
+__global__ void __launch_bounds__(1024)
+    k_interval_fused(const InstDev* __restrict__ insts,
+                     const WorkItem* __restrict__ items,
+                     const Counters* __restrict__ counters, int ring_max) {
+  interval_fused<MODE, ST, NT, RCH>(insts, items, counters, ring_max);
+}
+
+// 2-byte storages: a 512-thread workgroup's ring takes a quarter of the
+// CU's LDS, so registers are capped for eight waves per SIMD and four
+// workgroups share a CU (uncapped, the SGPR count allows seven waves).
+template <int MODE, int ST, bool NT, int RCH>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
+    k_interval_fused_w8(const InstDev* __restrict__ insts,
+                        const WorkItem* __restrict__ items,
+                        const Counters* __restrict__ counters, int ring_max) {
+  interval_fused<MODE, ST, NT, RCH>(insts, items, counters, ring_max);
 }
 
 __global__ void k_flip(const InstDev* __restrict__ insts, Counters* counters,
@@ -1514,11 +1522,19 @@ template <int MODE, int ST, bool NT, int RCH>
 hipError_t launch_fused(const Launch& L) {
   const size_t lds = sizeof(double) * 4 * (size_t)L.ring_max +
                      9 * (size_t)RCH * sizeof(typename Traits<ST>::S);
-  hipError_t e = raise_lds_limit(k_interval_fused<MODE, ST, NT, RCH>, lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_interval_fused<MODE, ST, NT, RCH>), dim3(L.n_items),
-                     dim3(L.threads), lds, L.stream, L.insts, L.items,
-                     L.counters, L.ring_max);
+  if constexpr (sizeof(typename Traits<ST>::S) == 2) {
+    hipError_t e = raise_lds_limit(k_interval_fused_w8<MODE, ST, NT, RCH>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_interval_fused_w8<MODE, ST, NT, RCH>),
+                       dim3(L.n_items), dim3(L.threads), lds, L.stream, L.insts,
+                       L.items, L.counters, L.ring_max);
+  } else {
+    hipError_t e = raise_lds_limit(k_interval_fused<MODE, ST, NT, RCH>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_interval_fused<MODE, ST, NT, RCH>), dim3(L.n_items),
+                       dim3(L.threads), lds, L.stream, L.insts, L.items,
+                       L.counters, L.ring_max);
+  }
   return hipGetLastError();
 }
 

@@ -38,7 +38,7 @@ EXPORTS = (
     "lbc_set_marks", "lbc_get_marks", "lbc_run_interval", "lbc_get_drag",
     "lbc_get_logweights", "lbc_select", "lbc_gather", "lbc_advance",
     "lbc_get_R_history", "lbc_sync", "lbc_step_kernel_ms", "lbc_threads",
-    "lbc_sites_per_thread", "lbc_step_host",
+    "lbc_sites_per_thread", "lbc_step_shape", "lbc_step_host",
 )
 
 
@@ -115,6 +115,7 @@ def lib() -> C.CDLL:
     L.lbc_step_kernel_ms.argtypes = [vp, dp, C.POINTER(C.c_int)]
     L.lbc_threads.argtypes = [vp]
     L.lbc_sites_per_thread.argtypes = [vp]
+    L.lbc_step_shape.argtypes = [vp]
     L.lbc_step_host.argtypes = [C.c_int, C.POINTER(Instance), dp, dp, dp]
     _lib = L
     return L
@@ -274,6 +275,16 @@ class Sampler:
     @property
     def vec(self) -> int:
         return lib().lbc_sites_per_thread(self._h)
+
+    @property
+    def nt(self) -> bool:
+        """Non-temporal population stores in the step kernel."""
+        return ((lib().lbc_step_shape(self._h) >> 3) & 3) == 2
+
+    @property
+    def fused(self) -> bool:
+        """Two LBM passes per HBM sweep (LDS ring) in the step kernel."""
+        return ((lib().lbc_step_shape(self._h) >> 5) & 3) == 2
 
     # -- state
     def load_states(self, inst: int, clone0: int, aos: np.ndarray) -> None:

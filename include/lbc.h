@@ -190,6 +190,12 @@ int lbc_step_kernel_ms(lbc_ctx* ctx, double* avg_ms, int* n_launches);
 int lbc_threads(const lbc_ctx* ctx);
 int lbc_sites_per_thread(const lbc_ctx* ctx);
 
+/* The step-kernel shape the context settled on (tuned or pinned), in the
+ * encoding of lbc_config.flags with every field resolved: sites per thread
+ * in bits 0-2, non-temporal stores in bits 3-4 (1 off, 2 on), two passes
+ * per sweep in bits 5-6 (1 off, 2 on).  0 for a null context. */
+int lbc_step_shape(const lbc_ctx* ctx);
+
 /* One reference LBM step on one lattice held in host memory (FP64):
  * fin (post-stream state) becomes the post-collision state, fout receives
  * the next post-stream state, *force the raw drag on it.  Parity hook for
