@@ -173,11 +173,10 @@ def cpu_baseline(sample_clones_per_core: int = 30, steps: int = 200) -> dict:
 
 
 def measured_traffic(run: dict) -> int | None:
-    """HBM bytes per step-kernel launch for this storage and kernel shape,
-    from the FETCH_SIZE / WRITE_SIZE summaries under profiles/ (copy-
-    calibrated there), or None when no summary covers the shape."""
-    key = (f"storage={run['storage']} threads={run['threads']} "
-           f"nt={int(run['nontemporal'])} passes={run['passes_per_sweep']}")
+    """HBM bytes per step-kernel launch for this storage and number of LBM
+    passes per sweep, from the FETCH_SIZE / WRITE_SIZE summaries under
+    profiles/ (copy-calibrated there), or None when none covers it."""
+    key = f"storage={run['storage']} passes={run['passes_per_sweep']}"
     pat = re.compile(r"^traffic_bytes_per_launch (.*): (\d+)\s*$")
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
         with open(path) as f:
@@ -186,6 +185,29 @@ def measured_traffic(run: dict) -> int | None:
                 if m and m.group(1) == key:
                     return int(m.group(2))
     return None
+
+
+def roofline(run: dict) -> dict:
+    """HBM roofline of the step kernel.  achieved = algorithmic bytes
+    (2*9*b per lattice update) / average kernel time; with two LBM passes
+    per HBM sweep the kernel moves about half of that, so the measured
+    traffic (copy-calibrated FETCH_SIZE + WRITE_SIZE per launch) and the
+    DRAM throughput it implies are given beside it."""
+    traffic = measured_traffic(run)
+    out = {
+        "bound": "hbm",
+        "achieved": round(run["achieved_gbs"], 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(run["roofline_frac"], 4),
+        "traffic": traffic,
+        "alg_bytes_per_launch": run["alg_bytes_per_launch"],
+    }
+    if traffic:
+        dram = traffic / (run["kernel_ms"] * 1e-3) / 1e9
+        out["dram_achieved"] = round(dram, 1)
+        out["dram_frac"] = round(dram / HBM_PEAK_GBS, 4)
+    return out
 
 
 def calibrate_copy() -> int:
@@ -273,14 +295,7 @@ def main() -> int:
             "lattice": [WORKLOAD["Lx"], WORKLOAD["Ly"]],
             "lbm_steps_per_interval": WORKLOAD["steps"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(head["achieved_gbs"], 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(head["roofline_frac"], 4),
-            "traffic": measured_traffic(head),
-        },
+        "roofline": roofline(head),
         "cpu_baseline": cpu,
         "paths": runs,
         "accuracy_vs_fp64": acc,

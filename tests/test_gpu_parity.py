@@ -98,6 +98,54 @@ def test_channel_interval_f64_bitexact(O, lbc, M, fused):
         assert lw[j] == 0.01 * acc
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_batched_uneven_instances_f64_bitexact(O, lbc, fused):
+    """Eight samplers of four lattice sizes and two biases in one launch per
+    interval, with different clone counts and interval lengths: every
+    instance's fields, per-step drag and log-weights equal the oracle's after
+    two intervals, and each instance selects on its own weights."""
+    dims = [(40, 10), (60, 15), (80, 20), (120, 30)]
+    specs = []
+    for i, (Lx, Ly) in enumerate(dims):
+        for j, k in enumerate((0.002, 0.01)):
+            specs.append(lbc.channel(Lx, Ly, Ly // 5, Lx // 4, Ly // 2, 0.53,
+                                     0.05, 2 + i + j, 5 + j, k=k))
+    print(f"case: batched uneven instances fused={fused}", flush=True)
+    rng = np.random.default_rng(11)
+    with lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, seed=9, threads=256,
+                     fused=fused) as s:
+        init = []
+        for i in range(len(specs)):
+            s.init_equilibrium(i, 1e-3)
+            init.append(s.store_states(i))
+        s.run_interval()
+        s.run_interval()
+        for i, spec in enumerate(specs):
+            g = geom_of(O, spec, O.MODE_CHANNEL)
+            got = s.store_states(i)
+            A = s.drag(i)
+            obs, lw = s.logweights(i)
+            for c in range(spec.n_clones):
+                assert np.array_equal(init[i][c],
+                                      O.init_channel(g, 1e-3, 9, c, i)), \
+                    f"instance {i} init clone {c}"
+                ref = init[i][c].copy()
+                O.run_steps(g, ref, spec.steps)
+                acc, Aref = O.run_steps(g, ref, spec.steps)
+                assert np.array_equal(got[c], ref), f"instance {i} clone {c}"
+                assert np.array_equal(A[c], Aref), f"instance {i} drag {c}"
+                assert obs[c] == acc
+                assert lw[c] == spec.bias * acc
+            N = spec.n_clones
+            w = np.exp(rng.normal(0, 1, N))
+            u = rng.random(N)
+            r = rng.integers(0, 2**31 - 1, N, dtype=np.int64).astype(np.int32)
+            sel = s.select(i, weights=w, u=u, rint=r)
+            ref = O.select(w, u, r)
+            assert np.array_equal(sel.pairs, ref["pairs"])
+            assert np.array_equal(sel.labels, ref["labels"])
+
+
 @pytest.mark.parametrize("N,p", [(1, 1), (2, 1), (7, 1), (64, 4), (1000, 8),
                                  (4096, 1), (8192, 2)])
 def test_selection_matches_oracle(O, lbc, N, p):
