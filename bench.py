@@ -10,9 +10,11 @@ selection and the on-device copy/noise, replayed from one HIP graph.
 
 The headline value is the clone-MLUPS of the FP16-shifted storage path
 (FP32 collide, FP64 drag/weights/selection).  The plain FP64 path and the
-FP32 / BF16-shifted paths are timed on the same workload in the same run,
-and the FP16 path's per-interval drag and log-mean-weight errors are
-measured against FP64 from identical states over one interval.
+FP32 / BF16-shifted paths are timed on the same workload in the same run.
+Each low-precision path's per-interval drag, log-mean-weight and
+scaled-cumulant errors are measured against FP64 over several cloning
+steps from identical developed states, with both runs making the FP64
+run's selections (shadow mode).
 
 Prints one JSON line.  Only N=1 is supported (one GPU).
 """
@@ -97,37 +99,76 @@ def time_storage(storage: int, K: int, W: int, seed: int = 0,
     )
 
 
-def accuracy(storage: int, n: int = 256, warm: int = 10, seed: int = 1) -> dict:
-    """Drag and log-mean-weight error of `storage` against FP64 from
-    identical developed states over one interval (no selection between)."""
+def log_mean_exp(v: np.ndarray) -> float:
+    m = float(v.max())
+    return m + math.log(float(np.mean(np.exp(v - m))))
+
+
+def scgf_last(logR: list, dT: float) -> float:
+    """Last value of the running estimate of post_processing/read_rvalue.py
+    (sum of log R over intervals 1..n-2, over T = (n-1) dT)."""
+    n = len(logR)
+    s = 0.0
+    T = 0.0
+    lam = 0.0
+    for i in range(1, n):
+        if i >= 2:
+            s = s + logR[i - 1]
+        T = T + dT
+        lam = (1.0 / T) * s
+    return lam
+
+
+def accuracy(storage: int, n: int = 256, warm: int = 10, intervals: int = 8,
+             seed: int = 1) -> dict:
+    """Errors of `storage` against FP64 over `intervals` cloning steps from
+    identical developed states (n clones after `warm` FP64 intervals), in
+    shadow mode: both samplers select with the FP64 weights and the same
+    draws, so they copy the same clones (with the same noise) and stay
+    paired.  Per interval and clone: the interval drag integral's relative
+    error; per interval: the log mean weight error; at the end: the running
+    scaled-cumulant estimate from each sampler's own mean weights."""
     spec = spec_for(n)
+    dT = spec.steps * spec.dt
+    rng = np.random.default_rng(seed)
     with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=lbc.STORE_F64,
-                     seed=seed) as s64:
-        s64.init_equilibrium(0, 1e-3)
-        s64.advance(warm)
-        states = s64.store_states(0)
+                     seed=seed) as sw:
+        sw.init_equilibrium(0, 1e-3)
+        sw.advance(warm)
+        states = sw.store_states(0)
+    drag, dlr, lr64, lrlp = [], [], [], []
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=lbc.STORE_F64,
+                     seed=seed) as s64, \
+            lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=storage,
+                        seed=seed) as slp:
         s64.load_states(0, 0, states)
-        s64.run_interval()
-        obs64, lw64 = s64.logweights(0)
-    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=storage,
-                     seed=seed) as slp:
         slp.load_states(0, 0, states)
-        slp.run_interval()
-        obslp, lwlp = slp.logweights(0)
-    rel = np.abs(obslp - obs64) / np.abs(obs64)
-
-    def log_mean_exp(v):
-        m = v.max()
-        return m + math.log(np.mean(np.exp(v - m)))
-
-    logR64 = log_mean_exp(lw64)
-    logRlp = log_mean_exp(lwlp)
+        for _ in range(intervals):
+            s64.run_interval()
+            slp.run_interval()
+            o64, w64 = s64.logweights(0)
+            olp, wlp = slp.logweights(0)
+            drag.append(np.abs(olp - o64) / np.abs(o64))
+            lr64.append(log_mean_exp(w64))
+            lrlp.append(log_mean_exp(wlp))
+            dlr.append(abs(lrlp[-1] - lr64[-1]))
+            wsel = np.exp(w64 - w64.max())
+            u = rng.random(n)
+            r = rng.integers(0, 2**31 - 1, n, dtype=np.int64).astype(np.int32)
+            s64.select(0, weights=wsel, u=u, rint=r)
+            slp.select(0, weights=wsel, u=u, rint=r)
+            s64.gather()
+            slp.gather()
+    drag = np.concatenate(drag)
+    psi64, psilp = scgf_last(lr64, dT), scgf_last(lrlp, dT)
     return dict(storage=lbc.STORAGE_NAMES[storage], clones=n,
-                warmup_intervals=warm,
-                drag_rel_err_max=float(rel.max()),
-                drag_rel_err_median=float(np.median(rel)),
-                logR_abs_err=float(abs(logRlp - logR64)),
-                logR_fp64=float(logR64))
+                warmup_intervals=warm, shadow_intervals=intervals,
+                drag_rel_err_max=float(drag.max()),
+                drag_rel_err_median=float(np.median(drag)),
+                logR_abs_err=float(max(dlr)),
+                scgf_fp64=psi64, scgf=psilp,
+                scgf_abs_err=abs(psilp - psi64),
+                scgf_rel_err=abs(psilp - psi64) / abs(psi64) if psi64 else None)
 
 
 def cpu_baseline(sample_clones_per_core: int = 30, steps: int = 200) -> dict:

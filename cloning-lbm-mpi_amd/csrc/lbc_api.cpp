@@ -49,6 +49,7 @@ struct lbc_ctx {
   WorkItem* items = nullptr;
   int n_items = 0;
   int* flip_mask = nullptr;
+  double* scgf = nullptr;             // running estimate of one instance
   std::vector<int> parity;            // host mirror of counters->parity
   std::vector<int> selected;          // selection done since last gather
   std::vector<void*> allocs;
@@ -446,7 +447,8 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
   }
   if (dalloc(c, &c->idd, sizeof(InstDev) * ni) ||
       dalloc(c, &c->counters, sizeof(Counters)) ||
-      dalloc(c, &c->flip_mask, sizeof(int) * lbc::kMaxInstances))
+      dalloc(c, &c->flip_mask, sizeof(int) * lbc::kMaxInstances) ||
+      dalloc(c, &c->scgf, sizeof(double) * (size_t)kMaxIntervals))
     return bail(LBC_E_NOMEM);
   // work items: biggest lattices first so the tail of the launch is short
   std::vector<int> order(ni);
@@ -778,6 +780,24 @@ int lbc_get_R_history(lbc_ctx* c, int inst, double* R, int max_n, int* n_out) {
                         sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
       for (size_t t = 0; t < tmp.size() / 2; t++) R[t] = tmp[2 * t];
     }
+  }
+  return LBC_OK;
+}
+
+int lbc_get_scgf(lbc_ctx* c, int inst, double dT, double* lam, int max_n,
+                 int* n_out) {
+  if (int rc = check_inst(c, inst)) return rc;
+  if (!(dT > 0.0)) return fail(c, LBC_E_ARG, "dT must be positive");
+  Counters h;
+  HIPC(c, hipMemcpyAsync(&h, c->counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  const int n = std::min(h.interval[inst], kMaxIntervals);
+  if (n_out) *n_out = n;
+  if (lam && max_n > 0 && n > 0) {
+    HIPC(c, lbc::launch_scgf(c->idd, inst, n, dT, c->scgf, c->stream));
+    HIPC(c, hipMemcpyAsync(lam, c->scgf, sizeof(double) * std::min(n, max_n),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
   }
   return LBC_OK;
 }

@@ -87,6 +87,8 @@ hipError_t launch_interval(const Launch& L);
 int fused_ring_sites(int threads, int max_lx, int mode);
 size_t fused_lds_bytes(int storage, int ring_max, int rch);
 hipError_t launch_flip(const Launch& L, int n_inst, const int* flip_mask_dev);
+hipError_t launch_scgf(const InstDev* insts, int inst, int n, double dT,
+                       double* lam, hipStream_t stream);
 hipError_t launch_perturb(const Launch& L, int philox_prefactors,
                           uint32_t seed_lo, uint32_t seed_hi);
 hipError_t launch_select(const InstDev* insts, Counters* counters, int inst0,

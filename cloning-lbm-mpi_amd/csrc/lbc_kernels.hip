@@ -1034,6 +1034,26 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
   interval_fused<MODE, ST, NT, RCH>(insts, items, counters, ring_max);
 }
 
+// Running scaled-cumulant estimate from the log mean weights of the first n
+// intervals, in the form of post_processing/read_rvalue.py:14-21: lam[0] = 0;
+// for i >= 1, T accumulates dT by repeated addition and
+// lam[i] = (1/T) * sum_{j=1}^{i-1} log R_j, summed in order (the first
+// interval's R never enters).  One lane: the sum is sequential.
+__global__ void k_scgf(const InstDev* __restrict__ insts, int inst, int n,
+                       double dT, double* __restrict__ lam) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  const double* logR = insts[inst].R_hist;  // (R, log R) per interval
+  double s = 0.0, T = 0.0;
+  lam[0] = 0.0;
+  for (int i = 1; i < n; i++) {
+    if (i >= 2) s = s + logR[2 * (i - 1) + 1];
+    T = T + dT;
+    const double oneOvT = 1. / T;
+    lam[i] = oneOvT * s;
+  }
+}
+
 __global__ void k_flip(const InstDev* __restrict__ insts, Counters* counters,
                        int n_inst, const int* __restrict__ flip_mask) {
   const int i = threadIdx.x;
@@ -1612,6 +1632,13 @@ hipError_t launch_interval(const Launch& L) {
   if (L.n_items <= 0) return hipSuccess;
   return L.mode == kModeRef ? launch_interval_m<kModeRef>(L)
                             : launch_interval_m<kModeChannel>(L);
+}
+
+hipError_t launch_scgf(const InstDev* insts, int inst, int n, double dT,
+                       double* lam, hipStream_t stream) {
+  hipLaunchKernelGGL(k_scgf, dim3(1), dim3(64), 0, stream, insts, inst, n, dT,
+                     lam);
+  return hipGetLastError();
 }
 
 hipError_t launch_flip(const Launch& L, int n_inst, const int* flip_mask_dev) {

@@ -37,7 +37,7 @@ EXPORTS = (
     "lbc_store_states", "lbc_init_equilibrium", "lbc_set_snapshots",
     "lbc_set_marks", "lbc_get_marks", "lbc_run_interval", "lbc_get_drag",
     "lbc_get_logweights", "lbc_select", "lbc_gather", "lbc_advance",
-    "lbc_get_R_history", "lbc_sync", "lbc_step_kernel_ms", "lbc_threads",
+    "lbc_get_R_history", "lbc_get_scgf", "lbc_sync", "lbc_step_kernel_ms", "lbc_threads",
     "lbc_sites_per_thread", "lbc_step_shape", "lbc_step_host",
 )
 
@@ -111,6 +111,8 @@ def lib() -> C.CDLL:
     L.lbc_advance.argtypes = [vp, C.c_int]
     L.lbc_get_R_history.argtypes = [vp, C.c_int, dp, C.c_int,
                                     C.POINTER(C.c_int)]
+    L.lbc_get_scgf.argtypes = [vp, C.c_int, C.c_double, dp, C.c_int,
+                               C.POINTER(C.c_int)]
     L.lbc_sync.argtypes = [vp]
     L.lbc_step_kernel_ms.argtypes = [vp, dp, C.POINTER(C.c_int)]
     L.lbc_threads.argtypes = [vp]
@@ -377,6 +379,18 @@ class Sampler:
         out = np.zeros(max(n.value, 1))
         self._check(lib().lbc_get_R_history(self._h, inst, _dp(out), n.value,
                                             C.byref(n)), "lbc_get_R_history")
+        return out[: n.value]
+
+    def scgf(self, inst: int, dT: float) -> np.ndarray:
+        """Running scaled-cumulant estimate after every interval so far
+        (FP64, on the device), read_rvalue.py's form with interval length
+        dT."""
+        n = C.c_int()
+        self._check(lib().lbc_get_scgf(self._h, inst, dT, None, 0, C.byref(n)),
+                    "lbc_get_scgf")
+        out = np.zeros(max(n.value, 1))
+        self._check(lib().lbc_get_scgf(self._h, inst, dT, _dp(out), n.value,
+                                       C.byref(n)), "lbc_get_scgf")
         return out[: n.value]
 
     def sync(self) -> None:

@@ -177,6 +177,16 @@ int lbc_advance(lbc_ctx* ctx, int n_intervals);
 int lbc_get_R_history(lbc_ctx* ctx, int inst, double* R, int max_n,
                       int* n_out);
 
+/* Running large-deviation (scaled cumulant generating function) estimate
+ * after every interval so far, computed in FP64 on the device from the log
+ * mean weights the selection kept, in the form of
+ * post_processing/read_rvalue.py:14-21: lam[0] = 0 and, for i >= 1,
+ * lam[i] = (1/T_i) * sum_{j=1}^{i-1} log R_j with T_i = dT + ... + dT
+ * (i terms; the script hard-codes dT = 0.3).  *n_out receives the number of
+ * intervals; at most max_n values are written. */
+int lbc_get_scgf(lbc_ctx* ctx, int inst, double dT, double* lam, int max_n,
+                 int* n_out);
+
 /* Wait for the context's stream. */
 int lbc_sync(lbc_ctx* ctx);
 

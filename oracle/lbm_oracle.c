@@ -489,6 +489,20 @@ void orc_init_channel(const orc_geom* g, double* f, double amp, uint32_t seed_lo
   }
 }
 
+/* post_processing/read_rvalue.py:14-21: the inner sum restarts from zero
+ * for every i and adds log r[1..i-1] in order, which is the running sum
+ * below; T += 0.3 per i there, dT here; lam = (1/T) * s. */
+void orc_scgf(const double* logR, int n, double dT, double* lam) {
+  double s = 0.0, T = 0.0;
+  if (n > 0) lam[0] = 0.0;
+  for (int i = 1; i < n; i++) {
+    if (i >= 2) s = s + logR[i - 1];
+    T = T + dT;
+    const double oneOvT = 1. / T;
+    lam[i] = oneOvT * s;
+  }
+}
+
 void orc_glibc_srand(unsigned seed) { srand(seed); }
 void orc_glibc_rand_n(int n, int32_t* out) {
   for (int i = 0; i < n; i++) out[i] = rand();

@@ -108,6 +108,12 @@ void orc_init_channel(const orc_geom* g, double* f, double amp, uint32_t seed_lo
 /* Equilibrium of (rho, ux, uy) in the reference's arithmetic order. */
 void orc_feq(double rho, double ux, double uy, double feq[9]);
 
+/* Running scaled-cumulant estimate from the per-interval log mean weights,
+ * as post_processing/read_rvalue.py:14-21 forms it: lam[0] = 0 and, for
+ * i >= 1, T accumulates dT by repeated addition and
+ * lam[i] = (1/T) * sum_{j=1}^{i-1} logR[j] (logR[0] never enters). */
+void orc_scgf(const double* logR, int n, double dT, double* lam);
+
 /* glibc RNG access (the reference's own generators). */
 void orc_glibc_srand(unsigned seed);
 void orc_glibc_rand_n(int n, int32_t* out);

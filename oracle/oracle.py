@@ -113,6 +113,8 @@ def lib() -> C.CDLL:
         L.orc_feq.argtypes = [C.c_double, C.c_double, C.c_double, dp]
         L.orc_bb_mask.argtypes = [gp, C.c_int, C.c_int]
         L.orc_bb_mask.restype = C.c_int
+        L.orc_scgf.argtypes = [dp, C.c_int, C.c_double, dp]
+        L.orc_scgf.restype = None
         L.orc_glibc_srand.argtypes = [C.c_uint]
         L.orc_glibc_rand_n.argtypes = [C.c_int, ip]
         L.orc_glibc_seed48_default.argtypes = []
@@ -187,6 +189,14 @@ def run_clones(g: Geom, states: np.ndarray, M: int, scale=1.0, shift=0.0,
     lib().orc_run_clones(C.byref(g), _dp(states), _dp(scratch), n, M, scale,
                          shift, dt, _dp(acc))
     return acc
+
+
+def scgf(logR: np.ndarray, dT: float) -> np.ndarray:
+    """Running scaled-cumulant estimate lam[0..n) from log mean weights."""
+    logR = np.ascontiguousarray(logR, np.float64)
+    lam = np.zeros(len(logR))
+    lib().orc_scgf(_dp(logR), len(logR), dT, _dp(lam))
+    return lam
 
 
 def perturb(fin: np.ndarray, snaps: np.ndarray, eps: float, a: np.ndarray):

@@ -326,3 +326,26 @@ def test_fused_sweeps_equal_single_passes(O, lbc, storage, mode):
         assert bad.size == 0, (
             f"{name}: {len(bad)} of {u.size} differ, first at {bad[0].tolist()}"
             f" ({u[tuple(bad[0])]!r} vs {v[tuple(bad[0])]!r})")
+
+
+def test_scgf_estimate_matches_oracle(O, lbc):
+    """The device's running large-deviation estimate equals the oracle's
+    restatement of read_rvalue.py on the sampler's own mean weights; with
+    zero bias every mean weight is exactly 1 and the estimate exactly 0."""
+    print("case: SCGF estimate 80x40, 64 clones, 12 cloning steps", flush=True)
+    dT = 20.0
+    for k in (0.05, 0.0):
+        spec = lbc.channel(80, 40, 8, 20, 20, 0.53, 0.05, 64, 20, k=k,
+                           sigma=1e-4)
+        with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, seed=2, threads=256) as s:
+            s.init_equilibrium(0, 1e-3)
+            s.advance(12)
+            R = s.R_history(0)
+            lam = s.scgf(0, dT)
+        assert len(R) == 12 and len(lam) == 12
+        if k == 0.0:
+            assert np.all(R == 1.0) and not np.any(lam)
+        else:
+            np.testing.assert_allclose(lam, O.scgf(np.log(R), dT), rtol=1e-12,
+                                       atol=1e-16)
+            assert np.any(lam != 0.0)

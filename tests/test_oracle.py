@@ -338,3 +338,24 @@ def test_brute_force_selection_tiny(O):
                 empties = [k for k in range(3) if r["nb_after"][k] == 0]
                 extras = [i for i in range(3) for _ in range(max(r["nb_after"][i] - 1, 0))]
                 assert r["pairs"].tolist() == [[i, k] for i, k in zip(extras, empties)]
+
+
+def test_scgf_estimator_closed_forms(O):
+    """read_rvalue.py's running estimate: constant log R = c gives
+    lam_i = (i - 1) c / (i dT); the first interval's R never enters; zero bias
+    (R = 1 every interval) gives zero; a sign or off-by-one in either the sum
+    or the time fails one of these."""
+    n, dT, c = 50, 0.3, 0.0173
+    lam = O.scgf(np.full(n, c), dT)
+    i = np.arange(1, n)
+    assert lam[0] == 0.0
+    np.testing.assert_allclose(lam[1:], (i - 1) * c / (i * dT), rtol=1e-13)
+    logR = np.random.default_rng(3).normal(0, 0.1, n)
+    a, b = logR.copy(), logR.copy()
+    b[0] = 40.0
+    assert np.array_equal(O.scgf(a, dT), O.scgf(b, dT))
+    assert not np.any(O.scgf(np.zeros(n), dT))
+    # the last estimate is the mean log R over intervals 1..n-2 scaled by
+    # (n-2)/((n-1) dT)
+    np.testing.assert_allclose(O.scgf(logR, dT)[-1],
+                               logR[1:n - 1].sum() / ((n - 1) * dT), rtol=1e-12)
