@@ -121,13 +121,15 @@ def scgf_last(logR: list, dT: float) -> float:
 
 def accuracy(storage: int, n: int = 256, warm: int = 10, intervals: int = 8,
              seed: int = 1) -> dict:
-    """Errors of `storage` against FP64 over `intervals` cloning steps from
-    identical developed states (n clones after `warm` FP64 intervals), in
-    shadow mode: both samplers select with the FP64 weights and the same
-    draws, so they copy the same clones (with the same noise) and stay
-    paired.  Per interval and clone: the interval drag integral's relative
-    error; per interval: the log mean weight error; at the end: the running
-    scaled-cumulant estimate from each sampler's own mean weights."""
+    """Errors of `storage` against FP64 from identical developed states
+    (n clones after `warm` FP64 intervals): over the first interval (the
+    stated bar), and over `intervals` cloning steps in shadow mode, where
+    both samplers select with the FP64 weights and the same draws, so they
+    copy the same clones (with the same noise) and stay paired while their
+    states drift apart.  Per interval and clone: the interval drag
+    integral's relative error; per interval: the log mean weight error; at
+    the end: the running scaled-cumulant estimate from each sampler's own
+    mean weights."""
     spec = spec_for(n)
     dT = spec.steps * spec.dt
     rng = np.random.default_rng(seed)
@@ -159,16 +161,25 @@ def accuracy(storage: int, n: int = 256, warm: int = 10, intervals: int = 8,
             slp.select(0, weights=wsel, u=u, rint=r)
             s64.gather()
             slp.gather()
+    first = drag[0]
     drag = np.concatenate(drag)
     psi64, psilp = scgf_last(lr64, dT), scgf_last(lrlp, dT)
     return dict(storage=lbc.STORAGE_NAMES[storage], clones=n,
-                warmup_intervals=warm, shadow_intervals=intervals,
-                drag_rel_err_max=float(drag.max()),
-                drag_rel_err_median=float(np.median(drag)),
-                logR_abs_err=float(max(dlr)),
-                scgf_fp64=psi64, scgf=psilp,
-                scgf_abs_err=abs(psilp - psi64),
-                scgf_rel_err=abs(psilp - psi64) / abs(psi64) if psi64 else None)
+                warmup_intervals=warm,
+                # the first interval, from identical states
+                drag_rel_err_max=float(first.max()),
+                drag_rel_err_median=float(np.median(first)),
+                logR_abs_err=float(dlr[0]),
+                # the whole shadow run
+                shadow=dict(
+                    intervals=intervals,
+                    drag_rel_err_max=float(drag.max()),
+                    drag_rel_err_median=float(np.median(drag)),
+                    logR_abs_err_max=float(max(dlr)),
+                    scgf_fp64=psi64, scgf=psilp,
+                    scgf_abs_err=abs(psilp - psi64),
+                    scgf_rel_err=(abs(psilp - psi64) / abs(psi64)
+                                  if psi64 else None)))
 
 
 def cpu_baseline(sample_clones_per_core: int = 30, steps: int = 200) -> dict:
