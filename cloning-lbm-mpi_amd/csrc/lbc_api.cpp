@@ -21,11 +21,6 @@ namespace {
 
 thread_local std::string g_err;
 
-constexpr double kW[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,
-                          1.0 / 9.0,  1.0 / 9.0,  1.0 / 36.0,
-                          1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
-constexpr int kCX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
-constexpr int kCY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 constexpr int kMaxClones = 8192;
 constexpr int kMaxIntervals = 1 << 16;
 
@@ -88,15 +83,6 @@ int fail(lbc_ctx* c, int code, const std::string& msg) {
                   std::string(#expr) + ": " + hipGetErrorString(e_));        \
   } while (0)
 
-void feq_ref_order(double rho, double ux, double uy, double out[9]) {
-  double u2 = -1.5 * (ux * ux + uy * uy);
-  for (int k = 0; k < 9; k++) {
-    double eu = kCX[k] * ux + kCY[k] * uy;
-    double eueu = 4.5 * eu * eu;
-    out[k] = kW[k] * rho * (1.0 + 3.0 * eu + eueu + u2);
-  }
-}
-
 int validate(const lbc_config* cfg, std::string& why) {
   if (!cfg || !cfg->instances) return why = "null config", -1;
   if (cfg->mode != LBC_MODE_REF && cfg->mode != LBC_MODE_CHANNEL)
@@ -126,7 +112,9 @@ int validate(const lbc_config* cfg, std::string& why) {
       return -1;
     };
     if (s.Lx < 8 || s.Ly < 8) return bad("lattice must be at least 8x8");
-    if ((long long)s.Lx * s.Ly * 9 > (1LL << 30)) return bad("lattice too large");
+    if ((long long)s.Lx * s.Ly * 9 > (1LL << 30) || s.Lx > 32767 ||
+        s.Ly > 32767)
+      return bad("lattice too large");
     if (s.xmin < 2 || s.xmax > s.Lx - 3 || s.ymin < 2 || s.ymax > s.Ly - 3)
       return bad("square must keep two fluid nodes from every boundary");
     if (s.xmax - s.xmin < 2 || s.ymax - s.ymin < 2)
@@ -207,7 +195,7 @@ int check_inst(lbc_ctx* c, int inst) {
 // passes in the fused step kernel.
 int interval_flip(const lbc_ctx* c, int i) {
   const int passes = c->inst[i].steps + 1;
-  return c->fused ? ((passes + 1) / 2) & 1 : passes & 1;
+  return c->fused ? lbc::fused_sweeps(c->inst[i].steps) & 1 : passes & 1;
 }
 
 int upload_flips(lbc_ctx* c) {
@@ -402,8 +390,7 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     d.bias = s.bias;
     d.eps = s.eps;
     d.sigma = s.noise_sigma;
-    feq_ref_order(1.0, d.u_in, 0.0, d.feq_in);
-    for (int k = 0; k < 9; k++) d.feq_in_s[k] = (float)(d.feq_in[k] - kW[k]);
+    d.inlet_k = 1.0 / (1.0 - d.u_in);
     d.max_intervals = kMaxIntervals;
     c->ring_max = std::max(c->ring_max, d.ring_n);
     c->max_lx = std::max(c->max_lx, d.Lx);
@@ -444,6 +431,19 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     if (hipMemcpy(d.labels, ident.data(), sizeof(int32_t) * n,
                   hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(c, LBC_E_HIP, "label init failed"));
+    // wave table: one word per block of 64 sites, plus a chunk's worth of
+    // blocks past the lattice for the last, partial chunk
+    std::vector<uint32_t> waves((d.plane + 63) / 64 + 1024 / 64, 0u);
+    for (int b = 0; b * 64 < d.plane; b++)
+      waves[b] = lbc::wave_word(d.Lx, d.Ly, d.xmin, d.xmax, d.ymin, d.ymax,
+                                64 * b);
+    uint32_t* wt = nullptr;
+    if (dalloc(c, &wt, sizeof(uint32_t) * waves.size()))
+      return bail(LBC_E_NOMEM);
+    if (hipMemcpy(wt, waves.data(), sizeof(uint32_t) * waves.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(c, LBC_E_HIP, "wave table upload failed"));
+    d.wave_info = wt;
   }
   if (dalloc(c, &c->idd, sizeof(InstDev) * ni) ||
       dalloc(c, &c->counters, sizeof(Counters)) ||

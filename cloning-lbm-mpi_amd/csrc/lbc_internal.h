@@ -21,10 +21,9 @@ struct InstDev {
   int rank_blocks;
   int clone_base;          // first global clone index of this instance
   double tau, beta, u_in;
+  double inlet_k;          // 1 / (1 - u_in)
   double scale, shift, dt, bias;
   double eps, sigma;
-  double feq_in[9];        // inlet ghost populations feq(1,(u_in,0)), FP64
-  float feq_in_s[9];       // the same minus w_q, FP32
   double* drag;            // [n_clones][steps]
   double* obs;             // [n_clones]  dt*sum A
   double* logw;            // [n_clones]
@@ -42,6 +41,7 @@ struct InstDev {
   double* prefactors;      // [n_clones][n_snapshots]
   double* snapshots;       // [n_snapshots][9*plane] FP64 SoA
   double* norms;           // [n_clones][2] REF perturbation norms
+  const uint32_t* wave_info;  // one word per block of 64 sites (wave_word)
   int max_intervals;
   int pad_;
 };
@@ -85,6 +85,40 @@ hipError_t launch_interval(const Launch& L);
 // and the LDS bytes of the whole launch (0 if it does not fit in one CU's
 // 160 KiB).
 int fused_ring_sites(int threads, int max_lx, int mode);
+
+// Wave table of an instance: for the block of 64 consecutive sites starting
+// at site0 = 64*b, the first site's x (bits 0-14) and y (bits 16-30), and
+// whether the block holds an x = 0 or x = Lx-1 site (bit 15) or a wall-row
+// or square-box site (bit 31).  Lattices narrower than 64 sites mark every
+// block with both.  Coordinates must stay below 2^15.
+constexpr uint32_t kWaveCoordMask = 0x7FFFu;
+constexpr int kWaveEdgeBit = 15;
+constexpr int kWaveSpecialBit = 31;
+
+inline uint32_t wave_word(int Lx, int Ly, int x0, int x1, int y0, int y1,
+                          int site0) {
+  const int x = site0 % Lx, y = site0 / Lx;
+  bool edge = true, special = true;
+  if (Lx >= 64) {
+    const int xe = x + 63;  // last site's x before wrapping to row y + 1
+    const bool wraps = xe >= Lx;
+    edge = (x == 0) || (xe >= Lx - 1);
+    const int ylast = wraps ? y + 1 : y;
+    const bool wall = (y == 0) || (ylast >= Ly - 1);
+    const bool box0 = y >= y0 && y <= y1 && x <= x1 && (wraps ? Lx - 1 : xe) >= x0;
+    const bool box1 = wraps && ylast >= y0 && ylast <= y1 && xe - Lx >= x0;
+    special = wall || box0 || box1;
+  }
+  return (uint32_t)x | ((uint32_t)edge << kWaveEdgeBit) | ((uint32_t)y << 16) |
+         ((uint32_t)special << kWaveSpecialBit);
+}
+
+// HBM sweeps of one interval of M steps in the two-pass step kernel: the
+// collide-only pass alone, passes 1..M-1 in pairs (an odd one out alone),
+// the stream-only pass alone.  Each sweep swaps the ping-pong buffers.
+constexpr int fused_sweeps(int M) {
+  return M < 1 ? 1 : 2 + (M - 1) / 2 + ((M - 1) & 1);
+}
 size_t fused_lds_bytes(int storage, int ring_max, int rch);
 hipError_t launch_flip(const Launch& L, int n_inst, const int* flip_mask_dev);
 hipError_t launch_scgf(const InstDev* insts, int inst, int n, double dT,

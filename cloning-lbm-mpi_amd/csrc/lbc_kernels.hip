@@ -353,7 +353,8 @@ __device__ inline int wall_mask(int y, int Ly) {
 //                   density).
 template <int ST>
 __device__ inline void channel_x_boundary(typename Traits<ST>::C (&f)[9],
-                                          bool inlet, double u_in) {
+                                          bool inlet, double u_in,
+                                          double inlet_k) {
   using T = Traits<ST>;
   if constexpr (!T::shifted) {
 #pragma clang fp contract(off)
@@ -385,7 +386,7 @@ __device__ inline void channel_x_boundary(typename Traits<ST>::C (&f)[9],
     // Shifted storage (f = w + d): both rules in one branch-free evaluation,
     // so a wave holding inlet and outlet lanes runs it once.
     const float ui = (float)u_in;
-    const float kin = (float)(1.0 / (1.0 - u_in));
+    const float kin = (float)inlet_k;  // 1 / (1 - u_in)
     const float D = (f[0] + f[2] + f[4]) + 2.0f * (f[3] + f[6] + f[7]);
     const float dr = ((f[0] + f[1]) + (f[2] + f[3])) +
                      ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
@@ -438,8 +439,13 @@ struct BufIO;
 
 template <>
 struct BufIO<kF64> {
-  __device__ static double ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+  using R = double;  // raw element as loaded
+  __device__ static R raw(__amdgpu_buffer_rsrc_t r, int vo, int so) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+  }
+  __device__ static double cvt(R v) { return v; }
+  __device__ static double ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return raw(r, vo, so);
   }
   template <bool NT>
   __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, double v) {
@@ -456,8 +462,13 @@ struct BufIO<kF64> {
 
 template <>
 struct BufIO<kF32> {
-  __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+  using R = float;
+  __device__ static R raw(__amdgpu_buffer_rsrc_t r, int vo, int so) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+  }
+  __device__ static float cvt(R v) { return v; }
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return raw(r, vo, so);
   }
   template <bool NT>
   __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float v) {
@@ -473,9 +484,13 @@ struct BufIO<kF32> {
 
 template <>
 struct BufIO<kF16> {
+  using R = unsigned short;
+  __device__ static R raw(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
+  }
+  __device__ static float cvt(R v) { return (float)__builtin_bit_cast(_Float16, v); }
   __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
-    return (float)__builtin_bit_cast(
-        _Float16, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0));
+    return cvt(raw(r, vo, so));
   }
   template <bool NT>
   __device__ static void st_raw(__amdgpu_buffer_rsrc_t r, int vo, int so,
@@ -491,9 +506,13 @@ struct BufIO<kF16> {
 
 template <>
 struct BufIO<kBF16> {
+  using R = unsigned short;
+  __device__ static R raw(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
+  }
+  __device__ static float cvt(R v) { return __uint_as_float((uint32_t)v << 16); }
   __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
-    const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
-    return __uint_as_float((uint32_t)b << 16);
+    return cvt(raw(r, vo, so));
   }
   template <bool NT>
   __device__ static void st_raw(__amdgpu_buffer_rsrc_t r, int vo, int so,
@@ -524,14 +543,21 @@ struct HbmIO {
   }
 };
 
+//
+// The ring sits at LDS address 0 (the step kernels declare no static LDS, so
+// their dynamic LDS starts there) and is addressed absolutely: the direction
+// plane goes into the instruction's offset field and no base is added.
 template <int ST, int RCH>
 struct LdsIO {
   typedef __attribute__((address_space(3))) typename Traits<ST>::S LS;
   static constexpr int ES = (int)sizeof(typename Traits<ST>::S);
   static constexpr int BMASK = RCH * ES - 1;  // ring bytes per direction - 1
-  __attribute__((address_space(3))) char* p;
+  static constexpr int BYTES = 9 * RCH * ES;  // whole ring
+  __device__ static LS* at(int q, int boff) {
+    return (LS*)(size_t)(unsigned)(q * RCH * ES + (boff & BMASK));
+  }
   __device__ typename Traits<ST>::C ld_byte(int q, int boff) const {
-    typename Traits<ST>::S e = *(LS*)(p + q * RCH * ES + (boff & BMASK));
+    typename Traits<ST>::S e = *at(q, boff);
     return Traits<ST>::ld(&e);
   }
   __device__ typename Traits<ST>::C ld(int q, int site) const {
@@ -539,7 +565,7 @@ struct LdsIO {
   }
   template <bool NT>
   __device__ void st(int q, int site, typename Traits<ST>::S v) const {
-    *(LS*)(p + q * RCH * ES + ((site * ES) & BMASK)) = v;
+    *at(q, site * ES) = v;
   }
 };
 
@@ -638,6 +664,15 @@ __device__ inline void ring_drag(const Geo& g, int x, int y,
   }
 }
 
+// Uses all nine loaded registers at one point, so the compiler issues the
+// nine loads back to back and waits once, instead of interleaving a
+// conversion (and a wait) after each load.  Emits no instruction.
+template <class R>
+__device__ inline void join9(R (&v)[9]) {
+  asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]),
+               "+v"(v[5]), "+v"(v[6]), "+v"(v[7]), "+v"(v[8]));
+}
+
 // Pulls of a fast-path site.  From HBM: one buffer load per direction at
 // the site's byte offset with the wave-uniform soffset q*plane - cy*Lx - cx;
 // from the LDS ring: the ring slot of site - cy*Lx - cx.  voL / voR (sL / sR)
@@ -651,9 +686,13 @@ __device__ inline void pull9(const HbmIO<ST>& io, typename Traits<ST>::C (&f)[9]
   // stay non-negative: directions with cx = -1 carry +Lx sites in voffset
   // and -Lx in their soffset (adjR >= -Lx).
   const int vo = site * ES, voL = vo + adjL * ES, voR = vo + (adjR + Lx) * ES;
+  typename BufIO<ST>::R v[9];
 #pragma unroll
   for (int q = 0; q < 9; q++)
-    f[q] = BufIO<ST>::ld(io.r, CX[q] > 0 ? voL : (CX[q] < 0 ? voR : vo), soff[q]);
+    v[q] = BufIO<ST>::raw(io.r, CX[q] > 0 ? voL : (CX[q] < 0 ? voR : vo), soff[q]);
+  join9(v);
+#pragma unroll
+  for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(v[q]);
 }
 
 template <int ST, int RCH>
@@ -676,8 +715,12 @@ template <int ST>
 __device__ inline void own9(const HbmIO<ST>& io, typename Traits<ST>::C (&f)[9],
                             int site, const int (&sown)[9]) {
   constexpr int ES = HbmIO<ST>::ES;
+  typename BufIO<ST>::R v[9];
 #pragma unroll
-  for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::ld(io.r, site * ES, sown[q]);
+  for (int q = 0; q < 9; q++) v[q] = BufIO<ST>::raw(io.r, site * ES, sown[q]);
+  join9(v);
+#pragma unroll
+  for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(v[q]);
 }
 
 template <int ST, int RCH>
@@ -705,7 +748,7 @@ __device__ inline void put9(const LdsIO<ST, RCH>& io,
   for (int q = 0; q < 9; q++) io.template st<NT>(q, site, o[q]);
 }
 
-// Per-pass constants of the fast path.
+// Per-pass constants of the pull and store addressing.
 template <int ST>
 struct PassConst {
   int rowb;     // one lattice row in bytes (LDS source)
@@ -727,17 +770,58 @@ struct PassConst {
   }
 };
 
-// One site of one pass.  Every site pulls its nine populations (the x
-// boundary folded into two per-lane offsets: periodic wrap in REF mode,
-// inlet/outlet ghost columns in channel mode).  A wave holding any wall-row
-// or square lane also loads each lane's own populations and takes the
-// bounce-back ones from there; square ring lanes add their drag term, solid
-// lanes store the rest state.  Channel x-boundary lanes then apply the
-// boundary rule.  Collision (except in the stream-only pass) and nine stores
-// follow.  The collide-only pass just loads the site's own populations.
+// What one wave's 64 consecutive sites touch (decoded from the instance's
+// wave table, one word per block of 64 sites, read with a scalar load):
+// wave-uniform, so the per-site branches on it are scalar and most waves
+// compute no lane coordinates at all.
+struct WaveGeo {
+  int x, y;      // first site
+  bool edge;     // holds an x = 0 or x = Lx-1 site
+  bool special;  // holds a wall-row site or a site of the square's box
+};
+
+__device__ inline WaveGeo wave_of(const InstDev* __restrict__ I, int block) {
+  const __attribute__((address_space(4))) uint32_t* t =
+      (const __attribute__((address_space(4))) uint32_t*)I->wave_info;
+  const uint32_t w = t[block];
+  WaveGeo g;
+  g.x = (int)(w & kWaveCoordMask);
+  g.edge = (w >> kWaveEdgeBit) & 1u;
+  g.y = (int)((w >> 16) & kWaveCoordMask);
+  g.special = (w >> kWaveSpecialBit) & 1u;
+  return g;
+}
+
+// Lane coordinates inside a wave (edge / special waves only).
+__device__ inline void lane_xy(const Geo& g, const WaveGeo& w, int lane, int& x,
+                               int& y) {
+  x = w.x + lane;
+  y = w.y;
+  if (g.Lx >= 64) {
+    if (x >= g.Lx) {
+      x -= g.Lx;
+      y += 1;
+    }
+  } else {
+    const int r = x / g.Lx;
+    y += r;
+    x -= r * g.Lx;
+  }
+}
+
+// One site of one pass.  The collide-only pass loads the site's own
+// populations.  Otherwise every site pulls its nine populations: waves away
+// from the x boundaries, the walls and the square take the plain pulls;
+// the others compute lane coordinates, fold the x boundary into two per-lane
+// offsets (periodic wrap in REF mode, inlet/outlet ghost columns in channel
+// mode), and, holding wall or square lanes, also load each lane's own
+// populations and take the bounce-back ones from there (ring lanes add
+// their drag term, solid lanes store the rest state); channel x-boundary
+// lanes apply the boundary rule.  Collision (except in the stream-only
+// pass) and nine stores follow.
 template <int MODE, int ST, bool NT, class SRC, class DST>
-__device__ inline void site_step(const Geo& g, int x, int y, int site,
-                                 const SRC& src, const DST& dst,
+__device__ inline void site_step(const Geo& g, const WaveGeo& w, int lane,
+                                 int site, const SRC& src, const DST& dst,
                                  const PassConst<ST>& pc,
                                  double* __restrict__ ring, double beta,
                                  double tau, const InstDev* __restrict__ I,
@@ -748,16 +832,21 @@ __device__ inline void site_step(const Geo& g, int x, int y, int site,
   bool solid = false;
   if (PH == kCollide) {
     own9(src, f, site, pc.sown);
+  } else if (!w.edge && !w.special) {
+    if constexpr (__is_same(SRC, HbmIO<ST>))
+      pull9<ST>(src, f, site, 0, 0, pc.soff, g.Lx);
+    else
+      pull9(src, f, site, 0, 0, pc.rowb);
   } else {
+    int x, y;
+    lane_xy(g, w, lane, x, y);
     const int aL = (x == 0) ? pc.adjL : 0;
     const int aR = (x == g.Lx - 1) ? pc.adjR : 0;
     if constexpr (__is_same(SRC, HbmIO<ST>))
       pull9<ST>(src, f, site, aL, aR, pc.soff, g.Lx);
     else
       pull9(src, f, site, aL, aR, pc.rowb);
-    const bool in_box = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1;
-    const bool special = in_box || y == 0 || y == g.Ly - 1;
-    if (__builtin_amdgcn_ballot_w64(special) != 0) {
+    if (w.special) {
       C own[9];
       own9(src, own, site, pc.sown);
       const int m = wall_mask(y, g.Ly) |
@@ -765,17 +854,18 @@ __device__ inline void site_step(const Geo& g, int x, int y, int site,
 #pragma unroll
       for (int q = 0; q < 9; q++)
         if ((m >> q) & 1) f[q] = own[OPP[q]];
+      const bool in_box = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1;
       solid = in_box && x > g.x0 && x < g.x1 && y > g.y0 && y < g.y1;
       if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1))
-        channel_x_boundary<ST>(f, x == 0, I->u_in);
+        channel_x_boundary<ST>(f, x == 0, I->u_in, I->inlet_k);
       if (in_box && !solid) ring_drag<MODE, ST>(g, x, y, f, ring, tau);
     } else if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1)) {
-      channel_x_boundary<ST>(f, x == 0, I->u_in);
+      channel_x_boundary<ST>(f, x == 0, I->u_in, I->inlet_k);
     }
   }
   typename T::S o[9];
   finish_site<MODE, ST>(f, o, PH, beta, tau);
-  if (solid) {
+  if (w.special && solid) {
     // the reference resets the solid interior to w every step; its
     // post-collision values never reach a fluid node
 #pragma unroll
@@ -792,17 +882,16 @@ __device__ inline void site_pass(const Geo& g, int tid, int nthr,
                                  double tau, const InstDev* __restrict__ I,
                                  int PH) {
   const PassConst<ST> pc(g, MODE);
-  const int dx = nthr % g.Lx, dy = nthr / g.Lx;
-  int x = tid % g.Lx, y = tid / g.Lx, site = tid;
-  while (y < g.Ly) {
-    site_step<MODE, ST, NT>(g, x, y, site, src, dst, pc, ring, beta, tau, I, PH);
+  const int lane = tid & 63;
+  const int wblk = __builtin_amdgcn_readfirstlane(tid) >> 6;  // wave in chunk
+  const int bpc = nthr >> 6;                                  // waves per chunk
+  const int nch = (g.plane + nthr - 1) / nthr;
+  int site = tid;
+  for (int c = 0; c < nch; c++) {
+    if (site < g.plane)
+      site_step<MODE, ST, NT>(g, wave_of(I, c * bpc + wblk), lane, site, src,
+                              dst, pc, ring, beta, tau, I, PH);
     site += nthr;
-    x += dx;
-    y += dy;
-    if (x >= g.Lx) {
-      x -= g.Lx;
-      y += 1;
-    }
   }
 }
 
@@ -813,51 +902,40 @@ __host__ __device__ inline int pull_reach(int mode, int Lx) {
   return mode == kModeRef ? 2 * Lx : Lx + 1;
 }
 
-// Two passes in one sweep over the lattice: pass A reads HBM and writes its
-// post-collision populations into the LDS ring, pass B runs L chunks behind
-// it, reads the ring and writes HBM.  A chunk is T consecutive sites (one per
-// thread); L*T >= the pull reach, so every pull of a B site finds its source
-// already in the ring, and the ring (>= 2L + 2 chunks) is never overwritten
-// while a B site still needs it.  One barrier per chunk.
+// Two stream-and-collide passes in one sweep over the lattice: pass A reads
+// HBM and writes its post-collision populations into the LDS ring, pass B
+// runs L chunks behind it, reads the ring and writes HBM.  A chunk is T
+// consecutive sites (one per thread); L*T >= the pull reach, so every pull
+// of a B site finds its source already in the ring, and the ring (>= 2L + 2
+// chunks) is never overwritten while a B site still needs it.  One barrier
+// per chunk.
 template <int MODE, int ST, bool NT, int RCH>
 __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
                                    const HbmIO<ST>& src, const HbmIO<ST>& dst,
                                    const LdsIO<ST, RCH>& mid, double* ringA,
                                    double* ringB, double beta, double tau,
-                                   const InstDev* __restrict__ I, int PA,
-                                   int PB) {
+                                   const InstDev* __restrict__ I) {
   const PassConst<ST> pc(g, MODE);
   const int CH = nthr;
   const int L = (pull_reach(MODE, g.Lx) + CH - 1) / CH;
   const int nch = (g.plane + CH - 1) / CH;
-  const int dx = CH % g.Lx, dy = CH / g.Lx;
-  int xa = tid % g.Lx, ya = tid / g.Lx, sa = tid;
-  int xb = xa, yb = ya, sb = tid;
+  const int lane = tid & 63;
+  const int wblk = __builtin_amdgcn_readfirstlane(tid) >> 6;  // wave in chunk
+  const int bpc = CH >> 6;                                    // waves per chunk
+  int sa = tid, sb = tid;
   for (int c = 0; c < nch + L; c++) {
     if (c < nch) {
       if (sa < g.plane)
-        site_step<MODE, ST, false>(g, xa, ya, sa, src, mid, pc, ringA, beta,
-                                   tau, I, PA);
+        site_step<MODE, ST, false>(g, wave_of(I, c * bpc + wblk), lane, sa, src,
+                                   mid, pc, ringA, beta, tau, I, kBoth);
       sa += CH;
-      xa += dx;
-      ya += dy;
-      if (xa >= g.Lx) {
-        xa -= g.Lx;
-        ya += 1;
-      }
     }
     __syncthreads();
     if (c >= L) {
       if (sb < g.plane)
-        site_step<MODE, ST, NT>(g, xb, yb, sb, mid, dst, pc, ringB, beta,
-                                tau, I, PB);
+        site_step<MODE, ST, NT>(g, wave_of(I, (c - L) * bpc + wblk), lane, sb,
+                                mid, dst, pc, ringB, beta, tau, I, kBoth);
       sb += CH;
-      xb += dx;
-      yb += dy;
-      if (xb >= g.Lx) {
-        xb -= g.Lx;
-        yb += 1;
-      }
     }
   }
 }
@@ -966,11 +1044,13 @@ __global__ void __launch_bounds__(1024)
   if (tid == 0) finish_clone<MODE>(I, wi, acc);
 }
 
-// The same interval, two passes per sweep through HBM (fused_sweep): passes
-// (0,1), (2,3), ...; with M even the last pass M runs alone.  The interval
-// ends with S_M in buf[parity ^ (S & 1)], S = ceil((M+1)/2) sweeps.  LDS:
-// the population ring of 9*rch sites, then four drag slot arrays (two per
-// sweep, double-buffered over sweeps).
+// The same interval with two passes per HBM sweep where it can: the
+// collide-only pass 0 runs alone, passes 1..M-1 (stream and collide) run in
+// pairs through the LDS ring (fused_sweep; an odd one out alone), and the
+// stream-only pass M runs alone, fused_sweeps(M) sweeps in all, so the
+// interval ends with S_M in buf[parity ^ (fused_sweeps(M) & 1)].  LDS: the
+// population ring of 9*rch sites, then four drag slot arrays (two per sweep,
+// double-buffered over sweeps).
 template <int MODE, int ST, bool NT, int RCH>
 __device__ inline void interval_fused(const InstDev* __restrict__ insts,
                                       const WorkItem* __restrict__ items,
@@ -983,9 +1063,8 @@ __device__ inline void interval_fused(const InstDev* __restrict__ insts,
   const int nthr = blockDim.x, tid = threadIdx.x;
   const HbmIO<ST> ioA{clone_rsrc(k.A, k.clone_bytes), k.g.plane};
   const HbmIO<ST> ioB{clone_rsrc(k.B, k.clone_bytes), k.g.plane};
-  // population ring at the start of LDS, drag slots after it
-  const LdsIO<ST, RCH> mid{(__attribute__((address_space(3))) char*)lds};
-  double* const slots = lds + (9 * RCH * LdsIO<ST, RCH>::ES) / 8;
+  const LdsIO<ST, RCH> mid{};
+  double* const slots = lds + LdsIO<ST, RCH>::BYTES / 8;
   double acc = 0.0;
   int sweep = 0;
   for (int n = 0; n <= k.M; sweep++) {
@@ -993,22 +1072,21 @@ __device__ inline void interval_fused(const InstDev* __restrict__ insts,
     const HbmIO<ST>& dst = (sweep & 1) ? ioA : ioB;
     double* ringA = slots + ((sweep & 1) * 2) * ring_max;
     double* ringB = ringA + ring_max;
-    if (n + 1 <= k.M) {
-      const int PA = (n == 0) ? kCollide : kBoth;
-      const int PB = (n + 1 == k.M) ? kStream : kBoth;
+    if (n >= 1 && n + 1 <= k.M - 1) {
       fused_sweep<MODE, ST, NT, RCH>(k.g, tid, nthr, src, dst, mid, ringA, ringB,
-                                     k.beta, k.tau, I, PA, PB);
+                                     k.beta, k.tau, I);
       __syncthreads();
       if (tid == 0) {
-        if (n >= 1) take_drag<MODE>(k, ringA, n, acc);
+        take_drag<MODE>(k, ringA, n, acc);
         take_drag<MODE>(k, ringB, n + 1, acc);
       }
       n += 2;
     } else {
+      const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
       site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, ringA, k.beta, k.tau, I,
-                              kStream);
+                              ph);
       __syncthreads();
-      if (tid == 0) take_drag<MODE>(k, ringA, n, acc);
+      if (n >= 1 && tid == 0) take_drag<MODE>(k, ringA, n, acc);
       n += 1;
     }
   }
