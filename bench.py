@@ -52,13 +52,13 @@ def spec_for(n_clones: int, steps: int | None = None) -> lbc.InstanceSpec:
 def time_storage(storage: int, K: int, W: int, seed: int = 0,
                  shape=None) -> dict:
     """W untimed warm-up intervals then K timed ones (graph replays).
-    shape: (threads, sites per thread, non-temporal, two passes per sweep)
-    or None (tuned at context creation)."""
+    shape: (threads, sites per thread, non-temporal, LBM passes per HBM
+    sweep) or None (tuned at context creation)."""
     spec = spec_for(WORKLOAD["n_clones"])
     kw = {}
     if shape:
         kw = dict(threads=shape[0], vec=shape[1], nt=bool(shape[2]),
-                  fused=bool(shape[3]) if len(shape) > 3 else False)
+                  passes=shape[3] if len(shape) > 3 else 1)
     with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=storage,
                      seed=seed, **kw) as s:
         s.init_equilibrium(0, 1e-3)
@@ -74,7 +74,7 @@ def time_storage(storage: int, K: int, W: int, seed: int = 0,
         threads = s.threads
         vec = s.vec
         nt = s.nt
-        fused = s.fused
+        passes = s.passes
     secs = t1 - t0
     lups = spec.n_clones * spec.sites * spec.steps * K
     bpl = lbc.bytes_per_lup(storage)
@@ -94,7 +94,7 @@ def time_storage(storage: int, K: int, W: int, seed: int = 0,
         threads=threads,
         sites_per_thread=vec,
         nontemporal=nt,
-        passes_per_sweep=2 if fused else 1,
+        passes_per_sweep=passes,
         logR_last=float(math.log(R[-1])) if len(R) else None,
     )
 
@@ -241,8 +241,8 @@ def measured_traffic(run: dict) -> int | None:
 
 def roofline(run: dict) -> dict:
     """HBM roofline of the step kernel.  achieved = algorithmic bytes
-    (2*9*b per lattice update) / average kernel time; with two LBM passes
-    per HBM sweep the kernel moves about half of that, so the measured
+    (2*9*b per lattice update) / average kernel time; with P LBM passes
+    per HBM sweep the kernel moves about 1/P of that, so the measured
     traffic (copy-calibrated FETCH_SIZE + WRITE_SIZE per launch) and the
     DRAM throughput it implies are given beside it."""
     traffic = measured_traffic(run)
@@ -289,8 +289,8 @@ def main() -> int:
                     help="run only a plain 1 GiB device copy (torch) so that "
                          "HBM counters can be scaled against known bytes")
     ap.add_argument("--shape", default="",
-                    help="threads,sites_per_thread,nontemporal,two_pass for "
-                         "every storage (skips the tuning at context "
+                    help="threads,sites_per_thread,nontemporal,passes_per_sweep"
+                         " for every storage (skips the tuning at context "
                          "creation)")
     ap.add_argument("--only", default="",
                     help="comma list of storages to time (f64,f32,f16,bf16)")

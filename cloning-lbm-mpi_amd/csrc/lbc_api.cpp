@@ -53,7 +53,7 @@ struct lbc_ctx {
   int threads = 256;
   int vec = 1;                        // sites per thread per iteration (step kernel)
   int nt = 0;                         // non-temporal population stores
-  int fused = 0;                      // two passes per sweep (LDS ring)
+  int passes = 1;                     // LBM passes per HBM sweep (LDS rings)
   int rch = 0;                        // LDS ring sites per direction
   int max_lx = 0;
   int ring_max = 0;
@@ -97,11 +97,10 @@ int validate(const lbc_config* cfg, std::string& why) {
   {
     const int v = cfg->flags & 7;
     const int ntm = (cfg->flags >> 3) & 3;
-    const int fm = (cfg->flags >> 5) & 3;
-    if ((cfg->flags & ~127) || (v != 0 && v != 1) || ntm == 3 || fm == 3)
+    if ((cfg->flags & ~127) || (v != 0 && v != 1) || ntm == 3)
       return why = "flags: bits 0-2 sites per thread (0 or 1), bits 3-4 "
-                   "non-temporal stores, bits 5-6 two-pass sweeps "
-                   "(0 tuned, 1 off, 2 on)", -1;
+                   "non-temporal stores (0 tuned, 1 off, 2 on), bits 5-6 "
+                   "LBM passes per HBM sweep (0 tuned, 1, 2 or 3)", -1;
   }
   for (int i = 0; i < cfg->n_instances; i++) {
     const lbc_instance& s = cfg->instances[i];
@@ -161,7 +160,7 @@ lbc::Launch make_launch(lbc_ctx* c) {
   L.storage = c->cfg.storage;
   L.vec = c->vec;
   L.nt = c->nt;
-  L.fused = c->fused;
+  L.passes = c->passes;
   L.rch = c->rch;
   L.stream = c->stream;
   return L;
@@ -191,11 +190,12 @@ int check_inst(lbc_ctx* c, int inst) {
   return LBC_OK;
 }
 
-// Buffer flips of one interval: one per pass, or one per sweep of two
-// passes in the fused step kernel.
+// Buffer flips of one interval: one per HBM sweep (one per pass with one
+// pass per sweep).
 int interval_flip(const lbc_ctx* c, int i) {
   const int passes = c->inst[i].steps + 1;
-  return c->fused ? lbc::fused_sweeps(c->inst[i].steps) & 1 : passes & 1;
+  return c->passes > 1 ? lbc::fused_sweeps(c->inst[i].steps, c->passes) & 1
+                       : passes & 1;
 }
 
 int upload_flips(lbc_ctx* c) {
@@ -249,27 +249,29 @@ int autotune(lbc_ctx* c) {
   const int tcands[3] = {256, 512, 1024};
   const int vfixed = c->cfg.flags & 7;
   const int ntmode = (c->cfg.flags >> 3) & 3;  // 0 tuned, 1 off, 2 on
-  const int fmode = (c->cfg.flags >> 5) & 3;   // 0 tuned, 1 off, 2 on
+  const int pmode = (c->cfg.flags >> 5) & 3;   // 0 tuned, else passes per sweep
   float best = 1e30f;
-  int best_t = 256, best_v = 1, best_nt = 0, best_f = 0;
+  int best_t = 256, best_v = 1, best_nt = 0, best_p = 1;
   hipEvent_t e0, e1;
   HIPC(c, hipEventCreate(&e0));
   HIPC(c, hipEventCreate(&e1));
-  for (int fused = 0; fused < 2; fused++)
+  for (int passes = 1; passes <= 3; passes++)
   for (int nt = 0; nt < 2; nt++)
   for (int v : {1}) {
     if (vfixed && v != vfixed) continue;
     if ((ntmode == 1 && nt) || (ntmode == 2 && !nt)) continue;
-    if ((fmode == 1 && fused) || (fmode == 2 && !fused)) continue;
+    if (pmode && passes != pmode) continue;
     for (int t : tcands) {
       if (c->cfg.threads && t != c->cfg.threads) continue;
       const int rch = lbc::fused_ring_sites(t, c->max_lx, c->cfg.mode);
-      if (fused && !lbc::fused_lds_bytes(c->cfg.storage, c->ring_max, rch)) continue;
+      if (passes > 1 &&
+          !lbc::fused_lds_bytes(c->cfg.storage, c->ring_max, rch, passes))
+        continue;
       lbc::Launch L = make_launch(c);
       L.threads = t;
       L.vec = v;
       L.nt = nt;
-      L.fused = fused;
+      L.passes = passes;
       L.rch = rch;
       HIPC(c, lbc::launch_interval(L));  // warm-up
       HIPC(c, hipEventRecord(e0, c->stream));
@@ -283,7 +285,7 @@ int autotune(lbc_ctx* c) {
         best_t = t;
         best_v = v;
         best_nt = nt;
-        best_f = fused;
+        best_p = passes;
       }
     }
   }
@@ -292,7 +294,7 @@ int autotune(lbc_ctx* c) {
   c->threads = best_t;
   c->vec = best_v;
   c->nt = best_nt;
-  c->fused = best_f;
+  c->passes = best_p;
   c->rch = lbc::fused_ring_sites(best_t, c->max_lx, c->cfg.mode);
   HIPC(c, hipMemcpyAsync(c->idd, c->idh.data(), sizeof(InstDev) * c->idh.size(),
                          hipMemcpyHostToDevice, c->stream));
@@ -315,7 +317,7 @@ int lbc_sites_per_thread(const lbc_ctx* ctx) { return ctx ? ctx->vec : 0; }
 
 int lbc_step_shape(const lbc_ctx* ctx) {
   if (!ctx) return 0;
-  return ctx->vec | ((ctx->nt ? 2 : 1) << 3) | ((ctx->fused ? 2 : 1) << 5);
+  return ctx->vec | ((ctx->nt ? 2 : 1) << 3) | (ctx->passes << 5);
 }
 
 void lbc_destroy(lbc_ctx* c) {
@@ -431,9 +433,10 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     if (hipMemcpy(d.labels, ident.data(), sizeof(int32_t) * n,
                   hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(c, LBC_E_HIP, "label init failed"));
-    // wave table: one word per block of 64 sites, plus a chunk's worth of
-    // blocks past the lattice for the last, partial chunk
-    std::vector<uint32_t> waves((d.plane + 63) / 64 + 1024 / 64, 0u);
+    // wave table: one word per block of 64 sites, plus two chunks' worth of
+    // blocks past the lattice (the last, partial chunk and the step kernel's
+    // one-chunk lookahead)
+    std::vector<uint32_t> waves((d.plane + 63) / 64 + 2 * 1024 / 64, 0u);
     for (int b = 0; b * 64 < d.plane; b++)
       waves[b] = lbc::wave_word(d.Lx, d.Ly, d.xmin, d.xmax, d.ymin, d.ymax,
                                 64 * b);
@@ -479,8 +482,8 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
   }
   const int vfixed = cfg->flags & 7;
   const int ntmode = (cfg->flags >> 3) & 3;
-  const int fmode = (cfg->flags >> 5) & 3;
-  if (cfg->threads == 0 || vfixed == 0 || ntmode == 0 || fmode == 0) {
+  const int pmode = (cfg->flags >> 5) & 3;
+  if (cfg->threads == 0 || vfixed == 0 || ntmode == 0 || pmode == 0) {
     int rc = autotune(c);
     if (rc) return bail(rc);
     L = make_launch(c);
@@ -491,12 +494,13 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     c->threads = cfg->threads;
     c->vec = vfixed;
     c->nt = ntmode == 2;
-    c->fused = fmode == 2;
+    c->passes = pmode;
     c->rch = lbc::fused_ring_sites(c->threads, c->max_lx, c->cfg.mode);
-    if (c->fused && !lbc::fused_lds_bytes(cfg->storage, c->ring_max, c->rch))
+    if (c->passes > 1 &&
+        !lbc::fused_lds_bytes(cfg->storage, c->ring_max, c->rch, c->passes))
       return bail(fail(c, LBC_E_ARG,
-                       "fused step kernel: LDS ring does not fit with these "
-                       "threads and storage"));
+                       "step kernel: the LDS rings of this many passes per "
+                       "sweep do not fit with these threads and storage"));
   }
   if (int rc = upload_flips(c)) return bail(rc);
   if (hipStreamSynchronize(c->stream) != hipSuccess)

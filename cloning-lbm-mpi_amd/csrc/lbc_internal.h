@@ -73,7 +73,7 @@ struct Launch {
   int storage;
   int vec;           // sites per thread per iteration of the step kernel (1)
   int nt;            // non-temporal population stores
-  int fused;         // two passes per sweep through an LDS ring
+  int passes;        // LBM passes per HBM sweep (1, 2 or 3)
   int rch;           // LDS ring sites per direction (fused)
   hipStream_t stream;
 };
@@ -113,13 +113,14 @@ inline uint32_t wave_word(int Lx, int Ly, int x0, int x1, int y0, int y1,
          ((uint32_t)special << kWaveSpecialBit);
 }
 
-// HBM sweeps of one interval of M steps in the two-pass step kernel: the
-// collide-only pass alone, passes 1..M-1 in pairs (an odd one out alone),
-// the stream-only pass alone.  Each sweep swaps the ping-pong buffers.
-constexpr int fused_sweeps(int M) {
-  return M < 1 ? 1 : 2 + (M - 1) / 2 + ((M - 1) & 1);
+// HBM sweeps of one interval of M steps with P passes per sweep (P >= 2):
+// the collide-only pass alone, passes 1..M-1 P at a time (the remainder
+// alone), the stream-only pass alone.  Each sweep swaps the ping-pong
+// buffers.
+constexpr int fused_sweeps(int M, int P) {
+  return M < 1 ? 1 : 2 + (M - 1) / P + (M - 1) % P;
 }
-size_t fused_lds_bytes(int storage, int ring_max, int rch);
+size_t fused_lds_bytes(int storage, int ring_max, int rch, int passes);
 hipError_t launch_flip(const Launch& L, int n_inst, const int* flip_mask_dev);
 hipError_t launch_scgf(const InstDev* insts, int inst, int n, double dT,
                        double* lam, hipStream_t stream);

@@ -544,17 +544,18 @@ struct HbmIO {
 };
 
 //
-// The ring sits at LDS address 0 (the step kernels declare no static LDS, so
-// their dynamic LDS starts there) and is addressed absolutely: the direction
-// plane goes into the instruction's offset field and no base is added.
-template <int ST, int RCH>
+// The rings sit from LDS address 0 (the step kernels declare no static LDS,
+// so their dynamic LDS starts there), ring k at k * BYTES, and are addressed
+// absolutely: ring and direction plane go into the instruction's offset
+// field and no base is added.
+template <int ST, int RCH, int RING = 0>
 struct LdsIO {
   typedef __attribute__((address_space(3))) typename Traits<ST>::S LS;
   static constexpr int ES = (int)sizeof(typename Traits<ST>::S);
   static constexpr int BMASK = RCH * ES - 1;  // ring bytes per direction - 1
-  static constexpr int BYTES = 9 * RCH * ES;  // whole ring
+  static constexpr int BYTES = 9 * RCH * ES;  // one ring
   __device__ static LS* at(int q, int boff) {
-    return (LS*)(size_t)(unsigned)(q * RCH * ES + (boff & BMASK));
+    return (LS*)(size_t)(unsigned)(RING * BYTES + q * RCH * ES + (boff & BMASK));
   }
   __device__ typename Traits<ST>::C ld_byte(int q, int boff) const {
     typename Traits<ST>::S e = *at(q, boff);
@@ -677,29 +678,11 @@ __device__ inline void join9(R (&v)[9]) {
 // the site's byte offset with the wave-uniform soffset q*plane - cy*Lx - cx;
 // from the LDS ring: the ring slot of site - cy*Lx - cx.  voL / voR (sL / sR)
 // fold in the x boundary for directions with cx = +1 / -1.
-template <int ST>
-__device__ inline void pull9(const HbmIO<ST>& io, typename Traits<ST>::C (&f)[9],
-                             int site, int adjL, int adjR, const int (&soff)[9],
-                             int Lx) {
-  constexpr int ES = HbmIO<ST>::ES;
-  // A raw buffer access is range-checked on voffset alone, so voffset must
-  // stay non-negative: directions with cx = -1 carry +Lx sites in voffset
-  // and -Lx in their soffset (adjR >= -Lx).
-  const int vo = site * ES, voL = vo + adjL * ES, voR = vo + (adjR + Lx) * ES;
-  typename BufIO<ST>::R v[9];
-#pragma unroll
-  for (int q = 0; q < 9; q++)
-    v[q] = BufIO<ST>::raw(io.r, CX[q] > 0 ? voL : (CX[q] < 0 ? voR : vo), soff[q]);
-  join9(v);
-#pragma unroll
-  for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(v[q]);
-}
-
-template <int ST, int RCH>
-__device__ inline void pull9(const LdsIO<ST, RCH>& io,
+template <int ST, int RCH, int RING>
+__device__ inline void pull9(const LdsIO<ST, RCH, RING>& io,
                              typename Traits<ST>::C (&f)[9], int site, int adjL,
                              int adjR, int rowb) {
-  constexpr int ES = LdsIO<ST, RCH>::ES;
+  constexpr int ES = LdsIO<ST, RCH, RING>::ES;
   // source byte offsets: column x - cx (x boundary folded in), then the row
   // y - cy one row-length (rowb bytes) away
   const int b0 = site * ES;
@@ -723,8 +706,8 @@ __device__ inline void own9(const HbmIO<ST>& io, typename Traits<ST>::C (&f)[9],
   for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(v[q]);
 }
 
-template <int ST, int RCH>
-__device__ inline void own9(const LdsIO<ST, RCH>& io,
+template <int ST, int RCH, int RING>
+__device__ inline void own9(const LdsIO<ST, RCH, RING>& io,
                             typename Traits<ST>::C (&f)[9], int site,
                             const int (&)[9]) {
 #pragma unroll
@@ -740,8 +723,8 @@ __device__ inline void put9(const HbmIO<ST>& io, const typename Traits<ST>::S (&
     BufIO<ST>::template st_raw<NT>(io.r, site * ES, sown[q], o[q]);
 }
 
-template <int ST, bool NT, int RCH>
-__device__ inline void put9(const LdsIO<ST, RCH>& io,
+template <int ST, bool NT, int RCH, int RING>
+__device__ inline void put9(const LdsIO<ST, RCH, RING>& io,
                             const typename Traits<ST>::S (&o)[9], int site,
                             const int (&)[9]) {
 #pragma unroll
@@ -809,43 +792,84 @@ __device__ inline void lane_xy(const Geo& g, const WaveGeo& w, int lane, int& x,
   }
 }
 
-// One site of one pass.  The collide-only pass loads the site's own
-// populations.  Otherwise every site pulls its nine populations: waves away
-// from the x boundaries, the walls and the square take the plain pulls;
-// the others compute lane coordinates, fold the x boundary into two per-lane
-// offsets (periodic wrap in REF mode, inlet/outlet ghost columns in channel
-// mode), and, holding wall or square lanes, also load each lane's own
-// populations and take the bounce-back ones from there (ring lanes add
-// their drag term, solid lanes store the rest state); channel x-boundary
-// lanes apply the boundary rule.  Collision (except in the stream-only
-// pass) and nine stores follow.
-template <int MODE, int ST, bool NT, class SRC, class DST>
-__device__ inline void site_step(const Geo& g, const WaveGeo& w, int lane,
-                                 int site, const SRC& src, const DST& dst,
-                                 const PassConst<ST>& pc,
-                                 double* __restrict__ ring, double beta,
-                                 double tau, const InstDev* __restrict__ I,
-                                 int PH) {
-  using T = Traits<ST>;
-  using C = typename T::C;
-  C f[9];
-  bool solid = false;
-  if (PH == kCollide) {
-    own9(src, f, site, pc.sown);
-  } else if (!w.edge && !w.special) {
-    if constexpr (__is_same(SRC, HbmIO<ST>))
-      pull9<ST>(src, f, site, 0, 0, pc.soff, g.Lx);
-    else
-      pull9(src, f, site, 0, 0, pc.rowb);
-  } else {
+// x-boundary offsets of a lane for the pulls of directions with cx = +1 / -1
+// (periodic wrap in REF mode, inlet/outlet ghost columns in channel mode);
+// zero away from the x boundaries.
+template <int ST>
+__device__ inline void x_adjust(const Geo& g, const WaveGeo& w, int lane,
+                                const PassConst<ST>& pc, int& aL, int& aR) {
+  aL = aR = 0;
+  if (w.edge) {
     int x, y;
     lane_xy(g, w, lane, x, y);
-    const int aL = (x == 0) ? pc.adjL : 0;
-    const int aR = (x == g.Lx - 1) ? pc.adjR : 0;
-    if constexpr (__is_same(SRC, HbmIO<ST>))
-      pull9<ST>(src, f, site, aL, aR, pc.soff, g.Lx);
-    else
-      pull9(src, f, site, aL, aR, pc.rowb);
+    aL = (x == 0) ? pc.adjL : 0;
+    aR = (x == g.Lx - 1) ? pc.adjR : 0;
+  }
+}
+
+// Issues the nine HBM pulls of a site (raw elements; their wait comes at
+// the first use, so a caller can issue them a chunk ahead).
+template <int ST>
+__device__ inline void pull_issue(const Geo& g, const WaveGeo& w, int lane,
+                                  int site, const HbmIO<ST>& src,
+                                  const PassConst<ST>& pc,
+                                  typename BufIO<ST>::R (&v)[9]) {
+  constexpr int ES = HbmIO<ST>::ES;
+  int aL, aR;
+  x_adjust(g, w, lane, pc, aL, aR);
+  // A raw buffer access is range-checked on voffset alone, so voffset must
+  // stay non-negative: directions with cx = -1 carry +Lx sites in voffset
+  // and -Lx in their soffset (adjR >= -Lx).
+  const int vo = site * ES, voL = vo + aL * ES, voR = vo + (aR + g.Lx) * ES;
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+    v[q] = BufIO<ST>::raw(src.r, CX[q] > 0 ? voL : (CX[q] < 0 ? voR : vo),
+                          pc.soff[q]);
+}
+
+// The nine pulled populations of a site.
+template <int ST>
+__device__ inline void pull_site(const Geo& g, const WaveGeo& w, int lane,
+                                 int site, const HbmIO<ST>& src,
+                                 const PassConst<ST>& pc,
+                                 typename Traits<ST>::C (&f)[9]) {
+  typename BufIO<ST>::R v[9];
+  pull_issue(g, w, lane, site, src, pc, v);
+  join9(v);
+#pragma unroll
+  for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(v[q]);
+}
+
+template <int ST, int RCH, int RING>
+__device__ inline void pull_site(const Geo& g, const WaveGeo& w, int lane,
+                                 int site, const LdsIO<ST, RCH, RING>& src,
+                                 const PassConst<ST>& pc,
+                                 typename Traits<ST>::C (&f)[9]) {
+  int aL, aR;
+  x_adjust(g, w, lane, pc, aL, aR);
+  pull9(src, f, site, aL, aR, pc.rowb);
+}
+
+// The rest of one site of one pass, given its nine pulled populations f
+// (own populations in the collide-only pass).  Waves holding wall or square
+// lanes load each lane's own populations and take the bounce-back ones
+// from there (ring lanes add their drag term, solid lanes store the rest
+// state); channel x-boundary lanes apply the boundary rule.  Collision
+// (except in the stream-only pass) and nine stores follow.
+template <int MODE, int ST, bool NT, class SRC, class DST>
+__device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
+                                   int site, typename Traits<ST>::C (&f)[9],
+                                   const SRC& src, const DST& dst,
+                                   const PassConst<ST>& pc,
+                                   double* __restrict__ ring, double beta,
+                                   double tau, const InstDev* __restrict__ I,
+                                   int PH) {
+  using T = Traits<ST>;
+  using C = typename T::C;
+  bool solid = false;
+  if (PH != kCollide && (w.edge || w.special)) {
+    int x, y;
+    lane_xy(g, w, lane, x, y);
     if (w.special) {
       C own[9];
       own9(src, own, site, pc.sown);
@@ -872,6 +896,25 @@ __device__ inline void site_step(const Geo& g, const WaveGeo& w, int lane,
     for (int q = 0; q < 9; q++) o[q] = T::pack(T::rest(q));
   }
   put9<ST, NT>(dst, o, site, pc.sown);
+}
+
+// One site of one pass: the collide-only pass loads the site's own
+// populations, every other pass pulls its nine populations (x boundary
+// folded in), then site_finish.
+template <int MODE, int ST, bool NT, class SRC, class DST>
+__device__ inline void site_step(const Geo& g, const WaveGeo& w, int lane,
+                                 int site, const SRC& src, const DST& dst,
+                                 const PassConst<ST>& pc,
+                                 double* __restrict__ ring, double beta,
+                                 double tau, const InstDev* __restrict__ I,
+                                 int PH) {
+  typename Traits<ST>::C f[9];
+  if (PH == kCollide)
+    own9(src, f, site, pc.sown);
+  else
+    pull_site(g, w, lane, site, src, pc, f);
+  site_finish<MODE, ST, NT>(g, w, lane, site, f, src, dst, pc, ring, beta, tau,
+                            I, PH);
 }
 
 // One HBM-to-HBM pass of one thread over its sites tid, tid + T, ...
@@ -902,19 +945,30 @@ __host__ __device__ inline int pull_reach(int mode, int Lx) {
   return mode == kModeRef ? 2 * Lx : Lx + 1;
 }
 
-// Two stream-and-collide passes in one sweep over the lattice: pass A reads
-// HBM and writes its post-collision populations into the LDS ring, pass B
-// runs L chunks behind it, reads the ring and writes HBM.  A chunk is T
-// consecutive sites (one per thread); L*T >= the pull reach, so every pull
-// of a B site finds its source already in the ring, and the ring (>= 2L + 2
-// chunks) is never overwritten while a B site still needs it.  One barrier
-// per chunk.
-template <int MODE, int ST, bool NT, int RCH>
+// Workgroup barrier that orders LDS traffic only: LDS writes before it are
+// visible to LDS reads after it, while HBM loads and stores in flight keep
+// going (__syncthreads would wait for them too).  Inside one sweep nothing
+// reads HBM that the sweep writes, so this is all the chunk steps need.  The
+// memory clobber keeps the compiler from moving memory accesses across.
+__device__ inline void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// NP (2 or 3) stream-and-collide passes in one sweep over the lattice.  The
+// first pass reads HBM and writes its post-collision populations into LDS
+// ring 0; each later pass runs L chunks behind the one before, reads the
+// previous ring, and writes the next ring or (the last pass) HBM.  A chunk
+// is T consecutive sites (one per thread); L*T >= the pull reach, so every
+// pull finds its source already in the ring, and a ring (>= 2L + 2 chunks)
+// is never overwritten while a later pass still needs it.  One LDS barrier
+// between consecutive passes of a chunk; the sweep's caller ends it with a
+// full barrier.  drag[k*ring_max] holds pass k's drag slots.
+template <int MODE, int ST, bool NT, int RCH, int NP>
 __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
                                    const HbmIO<ST>& src, const HbmIO<ST>& dst,
-                                   const LdsIO<ST, RCH>& mid, double* ringA,
-                                   double* ringB, double beta, double tau,
-                                   const InstDev* __restrict__ I) {
+                                   double* drag, int ring_max, double beta,
+                                   double tau, const InstDev* __restrict__ I) {
+  static_assert(NP == 2 || NP == 3, "two or three passes per sweep");
   const PassConst<ST> pc(g, MODE);
   const int CH = nthr;
   const int L = (pull_reach(MODE, g.Lx) + CH - 1) / CH;
@@ -922,20 +976,57 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
   const int lane = tid & 63;
   const int wblk = __builtin_amdgcn_readfirstlane(tid) >> 6;  // wave in chunk
   const int bpc = CH >> 6;                                    // waves per chunk
-  int sa = tid, sb = tid;
-  for (int c = 0; c < nch + L; c++) {
+  const LdsIO<ST, RCH, 0> r0{};
+  const LdsIO<ST, RCH, 1> r1{};
+  // The first pass's HBM pulls are issued one chunk ahead, so they travel
+  // while the later passes of the chunk before compute.  They are issued
+  // for every lane: past the lattice they read padding or, past the clone,
+  // zeros (raw buffer range check), and nobody uses them; a conditional
+  // load would make the compiler carry the old values along and wait.
+  typename BufIO<ST>::R pre[9];
+  pull_issue(g, wave_of(I, wblk), lane, tid, src, pc, pre);
+  for (int c = 0; c < nch + (NP - 1) * L; c++) {
     if (c < nch) {
-      if (sa < g.plane)
-        site_step<MODE, ST, false>(g, wave_of(I, c * bpc + wblk), lane, sa, src,
-                                   mid, pc, ringA, beta, tau, I, kBoth);
-      sa += CH;
+      const int s = c * CH + tid;
+      const WaveGeo w = wave_of(I, c * bpc + wblk);
+      typename Traits<ST>::C f[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(pre[q]);
+      // the converted values first, then the next chunk's loads into the
+      // same registers (loads cannot move above a memory clobber)
+      asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]), "+v"(f[3]),
+                   "+v"(f[4]), "+v"(f[5]), "+v"(f[6]), "+v"(f[7]), "+v"(f[8])
+                   :: "memory");
+      pull_issue(g, wave_of(I, (c + 1) * bpc + wblk), lane, s + CH, src, pc,
+                 pre);
+      if (s < g.plane)
+        site_finish<MODE, ST, false>(g, w, lane, s, f, src, r0, pc, drag, beta,
+                                     tau, I, kBoth);
     }
-    __syncthreads();
-    if (c >= L) {
-      if (sb < g.plane)
-        site_step<MODE, ST, NT>(g, wave_of(I, (c - L) * bpc + wblk), lane, sb,
-                                mid, dst, pc, ringB, beta, tau, I, kBoth);
-      sb += CH;
+    lds_barrier();
+    if constexpr (NP == 3) {
+      const int cm = c - L;
+      if (cm >= 0 && cm < nch) {
+        const int s = cm * CH + tid;
+        if (s < g.plane)
+          site_step<MODE, ST, false>(g, wave_of(I, cm * bpc + wblk), lane, s, r0,
+                                     r1, pc, drag + ring_max, beta, tau, I,
+                                     kBoth);
+      }
+      lds_barrier();
+    }
+    const int cz = c - (NP - 1) * L;
+    if (cz >= 0) {
+      const int s = cz * CH + tid;
+      if (s < g.plane) {
+        double* dz = drag + (NP - 1) * ring_max;
+        if constexpr (NP == 3)
+          site_step<MODE, ST, NT>(g, wave_of(I, cz * bpc + wblk), lane, s, r1,
+                                  dst, pc, dz, beta, tau, I, kBoth);
+        else
+          site_step<MODE, ST, NT>(g, wave_of(I, cz * bpc + wblk), lane, s, r0,
+                                  dst, pc, dz, beta, tau, I, kBoth);
+      }
     }
   }
 }
@@ -1044,14 +1135,14 @@ __global__ void __launch_bounds__(1024)
   if (tid == 0) finish_clone<MODE>(I, wi, acc);
 }
 
-// The same interval with two passes per HBM sweep where it can: the
-// collide-only pass 0 runs alone, passes 1..M-1 (stream and collide) run in
-// pairs through the LDS ring (fused_sweep; an odd one out alone), and the
-// stream-only pass M runs alone, fused_sweeps(M) sweeps in all, so the
-// interval ends with S_M in buf[parity ^ (fused_sweeps(M) & 1)].  LDS: the
-// population ring of 9*rch sites, then four drag slot arrays (two per sweep,
-// double-buffered over sweeps).
-template <int MODE, int ST, bool NT, int RCH>
+// The same interval with NP passes per HBM sweep where it can: the
+// collide-only pass 0 runs alone, passes 1..M-1 (stream and collide) run NP
+// at a time through the LDS rings (fused_sweep; the remainder alone), and
+// the stream-only pass M runs alone, fused_sweeps(M, NP) sweeps in all, so
+// the interval ends with S_M in buf[parity ^ (fused_sweeps(M, NP) & 1)].
+// LDS: NP-1 population rings of 9*rch sites, then 2*NP drag slot arrays
+// (NP per sweep, double-buffered over sweeps).
+template <int MODE, int ST, bool NT, int RCH, int NP>
 __device__ inline void interval_fused(const InstDev* __restrict__ insts,
                                       const WorkItem* __restrict__ items,
                                       const Counters* __restrict__ counters,
@@ -1063,53 +1154,51 @@ __device__ inline void interval_fused(const InstDev* __restrict__ insts,
   const int nthr = blockDim.x, tid = threadIdx.x;
   const HbmIO<ST> ioA{clone_rsrc(k.A, k.clone_bytes), k.g.plane};
   const HbmIO<ST> ioB{clone_rsrc(k.B, k.clone_bytes), k.g.plane};
-  const LdsIO<ST, RCH> mid{};
-  double* const slots = lds + LdsIO<ST, RCH>::BYTES / 8;
+  double* const slots = lds + (NP - 1) * (LdsIO<ST, RCH>::BYTES / 8);
   double acc = 0.0;
   int sweep = 0;
   for (int n = 0; n <= k.M; sweep++) {
     const HbmIO<ST>& src = (sweep & 1) ? ioB : ioA;
     const HbmIO<ST>& dst = (sweep & 1) ? ioA : ioB;
-    double* ringA = slots + ((sweep & 1) * 2) * ring_max;
-    double* ringB = ringA + ring_max;
-    if (n >= 1 && n + 1 <= k.M - 1) {
-      fused_sweep<MODE, ST, NT, RCH>(k.g, tid, nthr, src, dst, mid, ringA, ringB,
-                                     k.beta, k.tau, I);
+    double* drag = slots + (sweep & 1) * NP * ring_max;
+    if (n >= 1 && n + NP - 1 <= k.M - 1) {
+      fused_sweep<MODE, ST, NT, RCH, NP>(k.g, tid, nthr, src, dst, drag,
+                                         ring_max, k.beta, k.tau, I);
       __syncthreads();
-      if (tid == 0) {
-        take_drag<MODE>(k, ringA, n, acc);
-        take_drag<MODE>(k, ringB, n + 1, acc);
-      }
-      n += 2;
+      if (tid == 0)
+        for (int p = 0; p < NP; p++)
+          take_drag<MODE>(k, drag + p * ring_max, n + p, acc);
+      n += NP;
     } else {
       const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
-      site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, ringA, k.beta, k.tau, I,
+      site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, drag, k.beta, k.tau, I,
                               ph);
       __syncthreads();
-      if (n >= 1 && tid == 0) take_drag<MODE>(k, ringA, n, acc);
+      if (n >= 1 && tid == 0) take_drag<MODE>(k, drag, n, acc);
       n += 1;
     }
   }
   if (tid == 0) finish_clone<MODE>(I, wi, acc);
 }
 
-template <int MODE, int ST, bool NT, int RCH>
+template <int MODE, int ST, bool NT, int RCH, int NP>
 __global__ void __launch_bounds__(1024)
     k_interval_fused(const InstDev* __restrict__ insts,
                      const WorkItem* __restrict__ items,
                      const Counters* __restrict__ counters, int ring_max) {
-  interval_fused<MODE, ST, NT, RCH>(insts, items, counters, ring_max);
+  interval_fused<MODE, ST, NT, RCH, NP>(insts, items, counters, ring_max);
 }
 
-// 2-byte storages: a 512-thread workgroup's ring takes a quarter of the
-// CU's LDS, so registers are capped for eight waves per SIMD and four
-// workgroups share a CU (uncapped, the SGPR count allows seven waves).
+// 2-byte storages, two passes per sweep: a 512-thread workgroup's ring
+// takes a quarter of the CU's LDS, so registers are capped for eight waves
+// per SIMD and four workgroups share a CU (uncapped, the SGPR count allows
+// seven waves).
 template <int MODE, int ST, bool NT, int RCH>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
     k_interval_fused_w8(const InstDev* __restrict__ insts,
                         const WorkItem* __restrict__ items,
                         const Counters* __restrict__ counters, int ring_max) {
-  interval_fused<MODE, ST, NT, RCH>(insts, items, counters, ring_max);
+  interval_fused<MODE, ST, NT, RCH, 2>(insts, items, counters, ring_max);
 }
 
 // Running scaled-cumulant estimate from the log mean weights of the first n
@@ -1616,37 +1705,41 @@ hipError_t raise_lds_limit(K kernel, size_t lds) {
                              (int)lds);
 }
 
-template <int MODE, int ST, bool NT, int RCH>
+template <int MODE, int ST, bool NT, int RCH, int NP>
 hipError_t launch_fused(const Launch& L) {
-  const size_t lds = sizeof(double) * 4 * (size_t)L.ring_max +
-                     9 * (size_t)RCH * sizeof(typename Traits<ST>::S);
-  if constexpr (sizeof(typename Traits<ST>::S) == 2) {
+  const size_t lds = fused_lds_bytes(ST, L.ring_max, RCH, NP);
+  if (!lds) return hipErrorInvalidValue;
+  if constexpr (sizeof(typename Traits<ST>::S) == 2 && NP == 2) {
     hipError_t e = raise_lds_limit(k_interval_fused_w8<MODE, ST, NT, RCH>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_interval_fused_w8<MODE, ST, NT, RCH>),
                        dim3(L.n_items), dim3(L.threads), lds, L.stream, L.insts,
                        L.items, L.counters, L.ring_max);
   } else {
-    hipError_t e = raise_lds_limit(k_interval_fused<MODE, ST, NT, RCH>, lds);
+    hipError_t e = raise_lds_limit(k_interval_fused<MODE, ST, NT, RCH, NP>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_interval_fused<MODE, ST, NT, RCH>), dim3(L.n_items),
-                       dim3(L.threads), lds, L.stream, L.insts, L.items,
-                       L.counters, L.ring_max);
+    hipLaunchKernelGGL((k_interval_fused<MODE, ST, NT, RCH, NP>),
+                       dim3(L.n_items), dim3(L.threads), lds, L.stream, L.insts,
+                       L.items, L.counters, L.ring_max);
   }
   return hipGetLastError();
 }
 
+template <int MODE, int ST, bool NT, int NP>
+hipError_t launch_fused_r(const Launch& L) {
+  switch (L.rch) {
+    case 1024: return launch_fused<MODE, ST, NT, 1024, NP>(L);
+    case 2048: return launch_fused<MODE, ST, NT, 2048, NP>(L);
+    case 4096: return launch_fused<MODE, ST, NT, 4096, NP>(L);
+    case 8192: return launch_fused<MODE, ST, NT, 8192, NP>(L);
+  }
+  return hipErrorInvalidValue;
+}
+
 template <int MODE, int ST, bool NT>
 hipError_t launch_interval_k(const Launch& L) {
-  if (L.fused) {
-    switch (L.rch) {
-      case 1024: return launch_fused<MODE, ST, NT, 1024>(L);
-      case 2048: return launch_fused<MODE, ST, NT, 2048>(L);
-      case 4096: return launch_fused<MODE, ST, NT, 4096>(L);
-      case 8192: return launch_fused<MODE, ST, NT, 8192>(L);
-    }
-    return hipErrorInvalidValue;
-  }
+  if (L.passes == 2) return launch_fused_r<MODE, ST, NT, 2>(L);
+  if (L.passes == 3) return launch_fused_r<MODE, ST, NT, 3>(L);
   const size_t lds = sizeof(double) * 2 * (size_t)L.ring_max;
   hipLaunchKernelGGL((k_interval<MODE, ST, NT>), dim3(L.n_items),
                      dim3(L.threads), lds, L.stream, L.insts, L.items,
@@ -1699,10 +1792,11 @@ int fused_ring_sites(int threads, int max_lx, int mode) {
   return rch < 1024 ? 1024 : rch;  // compiled ring sizes: 1024 .. 8192
 }
 
-size_t fused_lds_bytes(int storage, int ring_max, int rch) {
-  if (rch > 8192) return 0;
+size_t fused_lds_bytes(int storage, int ring_max, int rch, int passes) {
+  if (rch > 8192 || passes < 2) return 0;
   const size_t es = storage == kF64 ? 8 : storage == kF32 ? 4 : 2;
-  const size_t b = sizeof(double) * 4 * (size_t)ring_max + 9 * (size_t)rch * es;
+  const size_t b = sizeof(double) * 2 * passes * (size_t)ring_max +
+                   (passes - 1) * 9 * (size_t)rch * es;
   return b <= 160 * 1024 ? b : 0;
 }
 

@@ -221,7 +221,11 @@ class Sampler:
     def __init__(self, specs, mode: int = MODE_CHANNEL,
                  storage: int = STORE_F64, seed: int = 0, threads: int = 0,
                  device: int = 0, vec: int = 0, nt: int | None = None,
-                 fused: bool | None = None):
+                 fused: bool | None = None, passes: int | None = None):
+        """threads = 0 tunes the step kernel's shape at creation; with
+        threads given, unspecified nt / passes default to off / one pass.
+        passes: LBM passes per HBM sweep (1, 2 or 3); fused=True is
+        passes=2, fused=False passes=1."""
         if isinstance(specs, InstanceSpec):
             specs = [specs]
         self.specs = list(specs)
@@ -234,10 +238,12 @@ class Sampler:
             vec = 1
         if nt is None:
             nt = False if threads else None
-        if fused is None:
-            fused = False if threads else None
+        if passes is None and fused is not None:
+            passes = 2 if fused else 1
+        if passes is None:
+            passes = 1 if threads else 0
         flags = (vec | ((0 if nt is None else (2 if nt else 1)) << 3)
-                 | ((0 if fused is None else (2 if fused else 1)) << 5))
+                 | (passes << 5))
         cfg = Config(mode, storage, len(self.specs), device, seed, threads,
                      flags, C.cast(arr, C.POINTER(Instance)))
         h = C.c_void_p()
@@ -284,9 +290,13 @@ class Sampler:
         return ((lib().lbc_step_shape(self._h) >> 3) & 3) == 2
 
     @property
+    def passes(self) -> int:
+        """LBM passes per HBM sweep of the step kernel (LDS rings between)."""
+        return (lib().lbc_step_shape(self._h) >> 5) & 3
+
+    @property
     def fused(self) -> bool:
-        """Two LBM passes per HBM sweep (LDS ring) in the step kernel."""
-        return ((lib().lbc_step_shape(self._h) >> 5) & 3) == 2
+        return self.passes > 1
 
     # -- state
     def load_states(self, inst: int, clone0: int, aos: np.ndarray) -> None:

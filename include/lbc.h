@@ -80,9 +80,10 @@ typedef struct lbc_config {
   int flags;                  /* step-kernel shape, 0 = tuned with threads:
                                  bits 0-2 sites per thread per iteration (1),
                                  bits 3-4 non-temporal population stores
-                                 (1 off, 2 on), bits 5-6 two passes per
-                                 sweep through an LDS ring (1 off, 2 on);
-                                 other bits 0 */
+                                 (1 off, 2 on), bits 5-6 LBM passes per HBM
+                                 sweep, chained through LDS rings (1, 2 or
+                                 3); other bits 0.  Every shape gives the
+                                 same results. */
   const lbc_instance* instances;
 } lbc_config;
 
@@ -202,8 +203,8 @@ int lbc_sites_per_thread(const lbc_ctx* ctx);
 
 /* The step-kernel shape the context settled on (tuned or pinned), in the
  * encoding of lbc_config.flags with every field resolved: sites per thread
- * in bits 0-2, non-temporal stores in bits 3-4 (1 off, 2 on), two passes
- * per sweep in bits 5-6 (1 off, 2 on).  0 for a null context. */
+ * in bits 0-2, non-temporal stores in bits 3-4 (1 off, 2 on), LBM passes
+ * per HBM sweep in bits 5-6 (1, 2 or 3).  0 for a null context. */
 int lbc_step_shape(const lbc_ctx* ctx);
 
 /* One reference LBM step on one lattice held in host memory (FP64):

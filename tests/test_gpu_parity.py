@@ -44,18 +44,18 @@ def test_ref_step_host_bitexact(O, lbc, L):
     assert Fg == F
 
 
-@pytest.mark.parametrize("storage_threads,fused", [(256, False), (1024, False),
-                                                 (256, True), (512, True)])
-def test_ref_interval_f64_bitexact(O, lbc, storage_threads, fused):
+@pytest.mark.parametrize("storage_threads,passes", [(256, 1), (1024, 1),
+                                                  (256, 2), (512, 2), (256, 3)])
+def test_ref_interval_f64_bitexact(O, lbc, storage_threads, passes):
     L, M, N = 8, 37, 5
     spec = lbc.ref_box(L, 0.515, 0.05, N, M, alpha=3.0)
     g = O.ref_box_geom(L, 0.515, 0.05)
     rng = np.random.default_rng(11)
     states = np.stack([rand_state(O, g, rng) for _ in range(N)])
     print(f"case: REF interval F64 L={L} M={M} N={N} threads={storage_threads} "
-          f"fused={fused}", flush=True)
+          f"passes={passes}", flush=True)
     with lbc.Sampler(spec, mode=lbc.MODE_REF, threads=storage_threads,
-                     fused=fused) as s:
+                     passes=passes) as s:
         s.load_states(0, 0, states)
         s.run_interval()
         got = s.store_states(0)
@@ -71,15 +71,19 @@ def test_ref_interval_f64_bitexact(O, lbc, storage_threads, fused):
         assert lw[j] == spec.bias * acc
 
 
-@pytest.mark.parametrize("M,fused", [(1, False), (2, False), (24, False),
-                                     (1, True), (2, True), (3, True),
-                                     (24, True)])
-def test_channel_interval_f64_bitexact(O, lbc, M, fused):
+@pytest.mark.parametrize("M,passes", [(1, 1), (2, 1), (24, 1),
+                                      (1, 2), (2, 2), (3, 2), (24, 2),
+                                      (1, 3), (2, 3), (3, 3), (4, 3), (5, 3),
+                                      (24, 3)])
+def test_channel_interval_f64_bitexact(O, lbc, M, passes):
+    """Bit-exact against the oracle for every interval length that changes
+    how the passes group into sweeps (lone collide pass, groups of 2 or 3
+    stream-collide passes, remainders, lone stream pass)."""
     spec = lbc.channel(80, 40, 8, 20, 20, 0.53, 0.05, 3, M, k=0.01)
     g = geom_of(O, spec, O.MODE_CHANNEL)
-    print(f"case: CHANNEL interval F64 80x40 M={M} fused={fused}", flush=True)
+    print(f"case: CHANNEL interval F64 80x40 M={M} passes={passes}", flush=True)
     with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, seed=5, threads=256,
-                     fused=fused) as s:
+                     passes=passes) as s:
         s.init_equilibrium(0, 1e-3)
         init = s.store_states(0)
         for j in range(3):
@@ -98,8 +102,8 @@ def test_channel_interval_f64_bitexact(O, lbc, M, fused):
         assert lw[j] == 0.01 * acc
 
 
-@pytest.mark.parametrize("fused", [False, True])
-def test_batched_uneven_instances_f64_bitexact(O, lbc, fused):
+@pytest.mark.parametrize("passes", [1, 2, 3])
+def test_batched_uneven_instances_f64_bitexact(O, lbc, passes):
     """Eight samplers of four lattice sizes and two biases in one launch per
     interval, with different clone counts and interval lengths: every
     instance's fields, per-step drag and log-weights equal the oracle's after
@@ -110,10 +114,10 @@ def test_batched_uneven_instances_f64_bitexact(O, lbc, fused):
         for j, k in enumerate((0.002, 0.01)):
             specs.append(lbc.channel(Lx, Ly, Ly // 5, Lx // 4, Ly // 2, 0.53,
                                      0.05, 2 + i + j, 5 + j, k=k))
-    print(f"case: batched uneven instances fused={fused}", flush=True)
+    print(f"case: batched uneven instances passes={passes}", flush=True)
     rng = np.random.default_rng(11)
     with lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, seed=9, threads=256,
-                     fused=fused) as s:
+                     passes=passes) as s:
         init = []
         for i in range(len(specs)):
             s.init_equilibrium(i, 1e-3)
@@ -296,20 +300,21 @@ def test_low_precision_channel_interval(O, lbc, storage, tol):
 
 @pytest.mark.parametrize("storage", [1, 2, 3])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_fused_sweeps_equal_single_passes(O, lbc, storage, mode):
-    """Two passes per sweep through the LDS ring give bit-identical states,
-    drag and weights to one pass per sweep (the ring holds the storage
-    precision), in both modes, with a lattice row longer than one chunk."""
+def test_multi_pass_sweeps_equal_single_passes(O, lbc, storage, mode):
+    """Two or three passes per sweep through the LDS rings give bit-identical
+    states, drag and weights to one pass per sweep (the rings hold the
+    storage precision), in both modes, with a lattice row longer than one
+    chunk."""
     if mode == 0:
         spec = lbc.ref_box(40, 0.55, 0.05, 3, 7, alpha=2.0)   # 161x161
     else:
         spec = lbc.channel(300, 40, 8, 75, 20, 0.55, 0.05, 3, 8, k=0.5)
-    print(f"case: fused vs single passes storage={storage} mode={mode}",
-          flush=True)
+    print(f"case: multi-pass vs single-pass sweeps storage={storage} "
+          f"mode={mode}", flush=True)
     out = []
-    for fused in (False, True):
+    for passes in (1, 2, 3):
         with lbc.Sampler(spec, mode=mode, storage=storage, seed=3, threads=256,
-                         fused=fused) as s:
+                         passes=passes) as s:
             if mode == 0:
                 g = O.ref_box_geom(40, 0.55, 0.05)
                 rng = np.random.default_rng(1)
@@ -320,12 +325,15 @@ def test_fused_sweeps_equal_single_passes(O, lbc, storage, mode):
             s.run_interval()
             s.run_interval()
             out.append((s.store_states(0), s.drag(0), s.logweights(0)[1]))
-    (a, da, la), (b, db, lb) = out
-    for name, u, v in (("states", a, b), ("drag", da, db), ("logw", la, lb)):
-        bad = np.argwhere(u != v)
-        assert bad.size == 0, (
-            f"{name}: {len(bad)} of {u.size} differ, first at {bad[0].tolist()}"
-            f" ({u[tuple(bad[0])]!r} vs {v[tuple(bad[0])]!r})")
+    (a, da, la) = out[0]
+    for p, (b, db, lb) in zip((2, 3), out[1:]):
+        for name, u, v in (("states", a, b), ("drag", da, db),
+                           ("logw", la, lb)):
+            bad = np.argwhere(u != v)
+            assert bad.size == 0, (
+                f"{p} passes, {name}: {len(bad)} of {u.size} differ, first at "
+                f"{bad[0].tolist()} ({u[tuple(bad[0])]!r} vs "
+                f"{v[tuple(bad[0])]!r})")
 
 
 def test_scgf_estimate_matches_oracle(O, lbc):
