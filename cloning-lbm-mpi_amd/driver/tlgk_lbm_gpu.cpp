@@ -26,7 +26,10 @@
 //     force_io    1 (default) | 0: per-clone force files
 //     zero_mean   1 (default) | 0: subtract AVG = 1.775 from F/F0
 //     seed        Philox key
-//     threads, vec  step-kernel shape (0 = tuned)
+//     threads     step-kernel workgroup size (0 = tuned)
+//     passes      LBM passes per HBM sweep, 1-3 (0 = tuned)
+//     nt          non-temporal population stores: 1 off, 2 on (0 = tuned)
+//     flags       the whole lbc_config.flags word (instead of passes / nt)
 //   channel mode (no reference counterpart; lattice given explicitly):
 //     Lx Ly side cx cy u_in k sigma steps intervals
 //   extra instances, batched into the same launches (channel mode):
@@ -307,7 +310,12 @@ int main(int argc, char** argv) {
   cfg.device = 0;
   cfg.seed = (uint64_t)kvi(in, "seed", 0);
   cfg.threads = (int)kvi(in, "threads", 0);
-  cfg.flags = (int)kvi(in, "vec", 0);
+  cfg.flags = (int)kvi(in, "flags", kvi(in, "vec", 0));
+  if (in.kv.count("passes"))
+    cfg.flags = (cfg.flags & ~(3 << 5)) | (int)(kvi(in, "passes", 0) & 3) << 5;
+  if (in.kv.count("nt"))
+    cfg.flags = (cfg.flags & ~(3 << 3)) | (int)(kvi(in, "nt", 0) & 3) << 3;
+  if ((cfg.flags & 7) == 0 && (cfg.flags >> 3)) cfg.flags |= 1;
   cfg.instances = insts.data();
   lbc_ctx* ctx = nullptr;
   check(lbc_create(&cfg, &ctx), nullptr, "lbc_create");

@@ -15,6 +15,7 @@ from conftest import PKG
 pytestmark = pytest.mark.gpu
 
 DRIVER = os.path.join(PKG, "driver", "tlgk_lbm_gpu")
+RECON = os.path.join(PKG, "post", "reconstruct")
 
 
 def write_case(tmp, O, L, Nc, T, dT, eps, tau, U0, alpha, extra=""):
@@ -122,3 +123,19 @@ def test_driver_matches_reference_loop(tmp_path, O, Nc, alpha):
     assert "nbCreatedCopies" in out and "Total # copies after cloning/pruning" in out
     if alpha == 0.0:
         assert np.all(R == 1.0)
+    # the ancestral drag trajectory of every final slot, reconstructed from
+    # the driver's files, follows the reference loop's genealogy
+    nt = labels.shape[0]
+    r = subprocess.run([RECON, str(run) + "/", str(Nc), str(nt), "20",
+                        str(tmp_path / "traj") + "/", "all"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    for j in range(Nc):
+        want, k = [None] * nt, j
+        for t in range(nt - 1, -1, -1):
+            want[t] = forces_ref[k][t]
+            if t > 0:
+                k = labels_ref[t - 1][k]
+        got = np.fromfile(tmp_path / "traj" / f"clone_{j}" /
+                          "data_force.datout").reshape(nt, 20)
+        np.testing.assert_allclose(got, np.array(want), rtol=1e-10, atol=1e-12)
