@@ -19,10 +19,18 @@ Fixtures (numpy .npz, FP64 unless noted):
       rand() after srand(1) and 1000 discards (the master rank's stream,
       src/mpi_LBM_GK_pops_perturb.cpp:176-178), and drand48() from its
       default state (never seeded by the reference).
+  bodyforce_L4.npz
+      every output of the reference's warm-up program (src/lbm_src/
+      main_bodyforce.cpp, built by `make -C oracle ref` into
+      oracle/_ref/run_bodyforce) for square side 4 from rest, 100 steps:
+      the input file, parameters.datout, data_force.datout, ux_t.datout,
+      pops.datout and the VTK frames fluid_t<i>.vtk (text).
 
 Run from the repository root: python tests/golden/make_golden.py
 """
 import os
+import shutil
+import subprocess
 import sys
 
 import numpy as np
@@ -90,6 +98,29 @@ def glibc_fixture():
                         rand_seed1_after1000=r, drand48_default=d)
 
 
+BODYFORCE_INPUT = "1\n100\n4\n0.53\n0.1\nout/\n0\n50\n1\n1\n"
+
+
+def bodyforce_fixture():
+    exe = os.path.join(ROOT, "oracle", "_ref", "run_bodyforce")
+    run = os.path.join(ROOT, "oracle", "_ref", "bodyforce_run")
+    shutil.rmtree(run, ignore_errors=True)
+    os.makedirs(run)
+    with open(os.path.join(run, "input.datin"), "w") as f:
+        f.write(BODYFORCE_INPUT)
+    subprocess.run([exe], cwd=run, check=True, stdout=subprocess.DEVNULL)
+    out = os.path.join(run, "out")
+    vtk = sorted(n for n in os.listdir(out) if n.startswith("fluid_t"))
+    text = lambda n: open(os.path.join(out, n)).read()
+    np.savez_compressed(
+        os.path.join(OUT, "bodyforce_L4.npz"), input=BODYFORCE_INPUT,
+        parameters=text("parameters.datout"),
+        force=np.fromfile(os.path.join(out, "data_force.datout")),
+        ux=np.fromfile(os.path.join(out, "ux_t.datout")),
+        pops=np.fromfile(os.path.join(out, "pops.datout")),
+        vtk_names=np.array(vtk), vtk=np.array([text(n) for n in vtk]))
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libref.so missing: run `make -C oracle ref` "
@@ -98,6 +129,7 @@ def main():
     steps_fixture(8, 6, 8, 0.515, 0.05)
     perturb_fixture()
     glibc_fixture()
+    bodyforce_fixture()
     print("fixtures written to", OUT)
 
 

@@ -139,3 +139,55 @@ def test_driver_matches_reference_loop(tmp_path, O, Nc, alpha):
         got = np.fromfile(tmp_path / "traj" / f"clone_{j}" /
                           "data_force.datout").reshape(nt, 20)
         np.testing.assert_allclose(got, np.array(want), rtol=1e-10, atol=1e-12)
+
+
+WARMUP = os.path.join(PKG, "driver", "lbm_warmup")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_warmup_matches_reference_program(tmp_path):
+    """lbm_warmup against every output of the reference's run_bodyforce
+    (src/lbm_src/main_bodyforce.cpp) on the same input file: force and
+    velocity-probe series and final populations bit for bit, parameters
+    and VTK frames byte for byte."""
+    ref = np.load(os.path.join(GOLDEN, "bodyforce_L4.npz"))
+    (tmp_path / "input.datin").write_text(str(ref["input"]))
+    print("case: warm-up L=4, 100 steps from rest", flush=True)
+    r = subprocess.run([WARMUP], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    out = tmp_path / "out"
+    assert (out / "parameters.datout").read_text() == str(ref["parameters"])
+    assert np.array_equal(np.fromfile(out / "data_force.datout"), ref["force"])
+    assert np.array_equal(np.fromfile(out / "ux_t.datout"), ref["ux"])
+    assert np.array_equal(np.fromfile(out / "pops.datout"), ref["pops"])
+    for name, text in zip(ref["vtk_names"], ref["vtk"]):
+        assert (out / str(name)).read_text() == str(text), name
+
+
+def test_warmup_snapshot_set_feeds_the_cloning_driver(tmp_path):
+    """The warm-up's snapshot set (pops_<i>.datout + popfiles_list.dat) is
+    the cloning driver's initial-state input: the last snapshot is the final
+    state, and a driver run starting from the set completes."""
+    (tmp_path / "input.datin").write_text(
+        "1\n120\n4\n0.53\n0.1\nwarm/\n0\n1000\n1000\n1000\n"
+        "snap_every = 5\nsnap_count = 10\n")
+    print("case: warm-up snapshot set L=4", flush=True)
+    r = subprocess.run([WARMUP], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    warm = tmp_path / "warm"
+    names = (warm / "popfiles_list.dat").read_text().split()
+    assert names == [f"pops_{i}.datout" for i in range(10)]
+    assert np.array_equal(np.fromfile(warm / names[-1]),
+                          np.fromfile(warm / "pops.datout"))
+    assert not np.array_equal(np.fromfile(warm / names[0]),
+                              np.fromfile(warm / names[-1]))
+    inp = tmp_path / "input_tlgk"
+    inp.write_text(f"4\n0.5\n0.25\n0.01\n4\n0.53\n0.05\n5.0\n{warm}/\n"
+                   f"{tmp_path}/clones/\n1000\n")
+    r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert np.fromfile(tmp_path / "clones" / "labels.dat",
+                       dtype=np.int32).size == 4 * 2
