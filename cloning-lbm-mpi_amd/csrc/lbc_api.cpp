@@ -1,7 +1,10 @@
 // Host side of the C ABI declared in include/lbc.h: context and device
 // buffer management, launch sequencing on the context's stream, HIP-graph
-// capture of a whole cloning step, and event timing of the step kernel.
+// capture of a whole cloning step, event timing of the step kernel, and the
+// random fields of the forcing mode (embedding spectrum, per-interval draws
+// through hipFFT).
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cmath>
@@ -23,6 +26,9 @@ thread_local std::string g_err;
 
 constexpr int kMaxClones = 8192;
 constexpr int kMaxIntervals = 1 << 16;
+// forcing mode: largest embedding torus, and the FFT work buffer's budget
+constexpr long long kMaxFieldEntries = 1LL << 26;
+constexpr size_t kFieldWorkBytes = size_t(1) << 30;
 
 size_t elem_bytes(int storage) {
   switch (storage) {
@@ -65,6 +71,11 @@ struct lbc_ctx {
   hipGraph_t graphs[2] = {nullptr, nullptr};
   bool timing_from_graph = false;
   std::vector<float> graph_ms;
+  // forcing mode, per instance
+  bool forced = false;                // some instance has forced steps
+  std::vector<hipfftHandle> fft;      // batched in-place 2-D Z2Z plans
+  std::vector<int> fchunk;            // clones per FFT batch
+  std::vector<char> maps_set;         // next interval uses injected maps
 };
 
 namespace {
@@ -130,6 +141,17 @@ int validate(const lbc_config* cfg, std::string& why) {
         !std::isfinite(s.dt) || !std::isfinite(s.bias) ||
         !std::isfinite(s.beta) || !std::isfinite(s.u_in))
       return bad("non-finite parameter");
+    if (s.forced_steps < 0 || s.forced_steps > s.steps)
+      return bad("forced_steps must be in [0, steps]");
+    if (s.forced_steps > 0) {
+      if (cfg->mode != LBC_MODE_REF)
+        return bad("forced steps need LBC_MODE_REF (the body-force mode)");
+      if (!std::isfinite(s.field_eps) || !std::isfinite(s.field_sigma) ||
+          s.field_sigma < 0)
+        return bad("field_eps must be finite and field_sigma finite and >= 0");
+      if ((long long)(2 * s.Lx - 1) * (2 * s.Ly - 1) > kMaxFieldEntries)
+        return bad("lattice too large for the random-field embedding");
+    }
   }
   return 0;
 }
@@ -162,6 +184,7 @@ lbc::Launch make_launch(lbc_ctx* c) {
   L.nt = c->nt;
   L.passes = c->passes;
   L.rch = c->rch;
+  L.forced = c->forced ? 1 : 0;
   L.stream = c->stream;
   return L;
 }
@@ -183,6 +206,117 @@ int ensure_staging(lbc_ctx* c, size_t bytes) {
   return LBC_OK;
 }
 
+#define FFTC(ctx, expr)                                                      \
+  do {                                                                       \
+    hipfftResult r_ = (expr);                                                \
+    if (r_ != HIPFFT_SUCCESS)                                                \
+      return fail((ctx), LBC_E_HIP,                                          \
+                  std::string(#expr) + ": hipFFT error " +                   \
+                      std::to_string((int)r_));                              \
+  } while (0)
+
+// Random-field set-up of forced instance i: map and FFT buffers, the square
+// roots of the circulant embedding's eigenvalues, and the batched plan.
+// generate_random_field.cpp:44-106 builds the wrapped covariance
+// G[j + fmy*r] = f(min(r, fmx-r), min(j, fmy-j)) / (fmx*fmy) with
+// f(x, y) = exp(-(x*x + y*y) / (2 sigma^2)) (covariance_fctn.cpp:3-10);
+// :112-158 take Re of its forward 2-D DFT as the eigenvalues and zero the
+// negative ones.  The test there, abs(a) > 1e-15, calls the int abs of
+// <cstdlib> on the double, so every negative eigenvalue above -1 is zeroed
+// and only a <= -1 is flagged (and turns the field into NaN); such an
+// embedding is rejected here.
+int field_setup(lbc_ctx* c, int i) {
+  InstDev& d = c->idh[i];
+  const lbc_instance& s = c->inst[i];
+  const double sigma = s.field_sigma > 0 ? s.field_sigma : 32.0;
+  const int fmx = 2 * s.Lx - 1, fmy = 2 * s.Ly - 1;
+  const size_t fmn = (size_t)fmx * fmy;
+  d.forced_steps = s.forced_steps;
+  d.field_eps = s.field_eps;
+  d.fmx = fmx;
+  d.fmy = fmy;
+  const size_t per = 2 * sizeof(double) * fmn;
+  const int chunk = (int)std::max<size_t>(
+      1, std::min<size_t>((size_t)s.n_clones, kFieldWorkBytes / per));
+  c->fchunk[i] = chunk;
+  const size_t map_bytes = sizeof(double) * (size_t)s.n_clones * d.plane;
+  if (dalloc(c, &d.field_map, map_bytes) ||
+      dalloc(c, &d.field_sqrt_ev, sizeof(double) * fmn) ||
+      dalloc(c, &d.field_work, per * chunk))
+    return LBC_E_NOMEM;
+  HIPC(c, hipMemsetAsync(d.field_map, 0, map_bytes, c->stream));
+  HIPC(c, hipMemsetAsync(d.field_work, 0, per * chunk, c->stream));
+
+  std::vector<double> G(2 * fmn, 0.0);
+  const double N = (double)fmn;
+  for (int r = 0; r < fmx; r++) {
+    const int dx = std::min(r, fmx - r);
+    for (int j = 0; j < fmy; j++) {
+      const int dy = std::min(j, fmy - j);
+      const double dd = (double)(dx * dx + dy * dy);
+      G[2 * (j + (size_t)fmy * r)] = std::exp(-dd / (2 * sigma * sigma)) / N;
+    }
+  }
+  auto* work = reinterpret_cast<hipfftDoubleComplex*>(d.field_work);
+  HIPC(c, hipMemcpyAsync(work, G.data(), sizeof(double) * G.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  hipfftHandle one = nullptr;
+  FFTC(c, hipfftPlan2d(&one, fmx, fmy, HIPFFT_Z2Z));
+  hipfftResult fr = hipfftSetStream(one, c->stream);
+  if (fr == HIPFFT_SUCCESS) fr = hipfftExecZ2Z(one, work, work, HIPFFT_FORWARD);
+  const hipError_t he = hipMemcpyAsync(G.data(), work, sizeof(double) * G.size(),
+                                       hipMemcpyDeviceToHost, c->stream);
+  const hipError_t hs = hipStreamSynchronize(c->stream);
+  (void)hipfftDestroy(one);
+  FFTC(c, fr);
+  HIPC(c, he);
+  HIPC(c, hs);
+  std::vector<double> sq(fmn);
+  for (size_t m = 0; m < fmn; m++) {
+    double a = G[2 * m];
+    if (a < 0) {
+      if (std::trunc(-a) >= 1.0)
+        return fail(c, LBC_E_ARG,
+                    "random field: the circulant embedding has an eigenvalue "
+                    "<= -1");
+      a = 0.0;
+    }
+    sq[m] = std::sqrt(a);
+  }
+  HIPC(c, hipMemcpyAsync(d.field_sqrt_ev, sq.data(), sizeof(double) * fmn,
+                         hipMemcpyHostToDevice, c->stream));
+  int dims[2] = {fmx, fmy};
+  FFTC(c, hipfftPlanMany(&c->fft[i], 2, dims, nullptr, 1, (int)fmn, nullptr, 1,
+                         (int)fmn, HIPFFT_Z2Z, chunk));
+  FFTC(c, hipfftSetStream(c->fft[i], c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
+// Random fields of the coming interval for every forced instance: Philox
+// normals, the batched FFT and the lattice block, chunk by chunk; an
+// instance whose maps the caller set (lbc_set_field_maps) keeps them once.
+int enqueue_fields(lbc_ctx* c) {
+  for (int i = 0; i < c->cfg.n_instances; i++) {
+    const InstDev& d = c->idh[i];
+    if (d.forced_steps <= 0) continue;
+    if (c->maps_set[i]) {
+      c->maps_set[i] = 0;
+      continue;
+    }
+    auto* work = reinterpret_cast<hipfftDoubleComplex*>(d.field_work);
+    for (int c0 = 0; c0 < d.n_clones; c0 += c->fchunk[i]) {
+      const int n = std::min(c->fchunk[i], d.n_clones - c0);
+      HIPC(c, lbc::launch_field_draw(c->idd, c->counters, i, c0, n,
+                                     d.fmx * d.fmy, seed_lo(c), seed_hi(c),
+                                     c->stream));
+      FFTC(c, hipfftExecZ2Z(c->fft[i], work, work, HIPFFT_FORWARD));
+      HIPC(c, lbc::launch_field_extract(c->idd, i, c0, n, d.plane, c->stream));
+    }
+  }
+  return LBC_OK;
+}
+
 int check_inst(lbc_ctx* c, int inst) {
   if (!c) return fail(nullptr, LBC_E_ARG, "null context");
   if (inst < 0 || inst >= c->cfg.n_instances)
@@ -194,7 +328,8 @@ int check_inst(lbc_ctx* c, int inst) {
 // pass per sweep).
 int interval_flip(const lbc_ctx* c, int i) {
   const int passes = c->inst[i].steps + 1;
-  return c->passes > 1 ? lbc::fused_sweeps(c->inst[i].steps, c->passes) & 1
+  return c->passes > 1 ? lbc::fused_sweeps(c->inst[i].steps, c->passes,
+                                          c->inst[i].forced_steps) & 1
                        : passes & 1;
 }
 
@@ -243,7 +378,10 @@ int autotune(lbc_ctx* c) {
   // Time short intervals at each (workgroup size, sites per thread) on the
   // freshly initialised state; the caller re-initialises the state after.
   std::vector<InstDev> tmp = c->idh;
-  for (auto& d : tmp) d.steps = std::min(d.steps, 16);
+  for (auto& d : tmp) {
+    d.steps = std::min(d.steps, 16);
+    d.forced_steps = std::min(d.forced_steps, d.steps);
+  }
   HIPC(c, hipMemcpyAsync(c->idd, tmp.data(), sizeof(InstDev) * tmp.size(),
                          hipMemcpyHostToDevice, c->stream));
   const int tcands[3] = {256, 512, 1024};
@@ -328,6 +466,8 @@ void lbc_destroy(lbc_ctx* c) {
     if (c->graphs[i]) (void)hipGraphDestroy(c->graphs[i]);
   }
   for (auto e : c->ev) (void)hipEventDestroy(e);
+  for (auto p : c->fft)
+    if (p) (void)hipfftDestroy(p);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->staging) (void)hipFree(c->staging);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -362,6 +502,9 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
   c->idh.resize(ni);
   c->parity.assign(ni, 0);
   c->selected.assign(ni, 0);
+  c->fft.assign(ni, nullptr);
+  c->fchunk.assign(ni, 0);
+  c->maps_set.assign(ni, 0);
   int clone_base = 0;
   for (int i = 0; i < ni; i++) {
     const lbc_instance& s = c->inst[i];
@@ -447,6 +590,10 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
                   hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(c, LBC_E_HIP, "wave table upload failed"));
     d.wave_info = wt;
+    if (s.forced_steps > 0) {
+      c->forced = true;
+      if (int rc2 = field_setup(c, i)) return bail(rc2);
+    }
   }
   if (dalloc(c, &c->idd, sizeof(InstDev) * ni) ||
       dalloc(c, &c->counters, sizeof(Counters)) ||
@@ -604,6 +751,7 @@ int lbc_run_interval(lbc_ctx* c, const double* prefactors) {
   c->ev_used = 0;
   c->timing_from_graph = false;
   if (int rc = enqueue_perturb(c, philox)) return rc;
+  if (int rc = enqueue_fields(c)) return rc;
   if (int rc = enqueue_interval(c, true)) return rc;
   return LBC_OK;
 }
@@ -758,6 +906,8 @@ int lbc_advance(lbc_ctx* c, int n_intervals) {
         c->ev.push_back(e);
       }
     }
+    if (c->forced)
+      if (int rc = enqueue_fields(c)) return rc;
     HIPC(c, hipEventRecord(c->ev[c->ev_used], c->stream));
     HIPC(c, hipGraphLaunch(c->gexec[0], c->stream));
     HIPC(c, hipEventRecord(c->ev[c->ev_used + 1], c->stream));
@@ -806,6 +956,45 @@ int lbc_get_scgf(lbc_ctx* c, int inst, double dT, double* lam, int max_n,
   return LBC_OK;
 }
 
+int lbc_set_field_maps(lbc_ctx* c, int inst, const double* maps) {
+  if (int rc = check_inst(c, inst)) return rc;
+  const InstDev& d = c->idh[inst];
+  if (d.forced_steps <= 0)
+    return fail(c, LBC_E_STATE, "instance has no forced steps");
+  if (!maps) return fail(c, LBC_E_ARG, "null maps");
+  const size_t plane = (size_t)d.plane;
+  std::vector<double> tmp(plane * d.n_clones);
+  for (int j = 0; j < d.n_clones; j++)
+    for (int x = 0; x < d.Lx; x++)
+      for (int y = 0; y < d.Ly; y++)
+        tmp[j * plane + (size_t)y * d.Lx + x] =
+            maps[j * plane + y + (size_t)d.Ly * x];
+  HIPC(c, hipMemcpyAsync(d.field_map, tmp.data(), sizeof(double) * tmp.size(),
+                         hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->maps_set[inst] = 1;
+  return LBC_OK;
+}
+
+int lbc_get_field_maps(lbc_ctx* c, int inst, double* maps) {
+  if (int rc = check_inst(c, inst)) return rc;
+  const InstDev& d = c->idh[inst];
+  if (d.forced_steps <= 0)
+    return fail(c, LBC_E_STATE, "instance has no forced steps");
+  if (!maps) return fail(c, LBC_E_ARG, "null maps");
+  const size_t plane = (size_t)d.plane;
+  std::vector<double> tmp(plane * d.n_clones);
+  HIPC(c, hipMemcpyAsync(tmp.data(), d.field_map, sizeof(double) * tmp.size(),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  for (int j = 0; j < d.n_clones; j++)
+    for (int x = 0; x < d.Lx; x++)
+      for (int y = 0; y < d.Ly; y++)
+        maps[j * plane + y + (size_t)d.Ly * x] =
+            tmp[j * plane + (size_t)y * d.Lx + x];
+  return LBC_OK;
+}
+
 int lbc_sync(lbc_ctx* c) {
   if (!c) return fail(nullptr, LBC_E_ARG, "null context");
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -846,6 +1035,7 @@ int lbc_step_host(int mode, const lbc_instance* geom, double* fin, double* fout,
   s.dt = 1.0;
   s.rank_blocks = 1;
   s.n_snapshots = 0;
+  s.forced_steps = 0;
   lbc_config cfg{};
   cfg.mode = mode;
   cfg.storage = LBC_STORE_F64;

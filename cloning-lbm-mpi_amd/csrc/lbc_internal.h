@@ -42,6 +42,13 @@ struct InstDev {
   double* snapshots;       // [n_snapshots][9*plane] FP64 SoA
   double* norms;           // [n_clones][2] REF perturbation norms
   const uint32_t* wave_info;  // one word per block of 64 sites (wave_word)
+  // random-field body force (forcing mode); forced_steps = 0: off
+  int forced_steps;        // leading steps of an interval that use it
+  int fmx, fmy;            // circulant embedding (2Lx-1) x (2Ly-1)
+  double field_eps;        // beta(x,y) = (1 + field_eps*map(x,y))*beta
+  double* field_map;       // [n_clones][plane] fields, site order y*Lx + x
+  double* field_sqrt_ev;   // [fmx*fmy] square roots of the eigenvalues
+  double* field_work;      // [chunk][fmx*fmy] complex (re, im), FFT in place
   int max_intervals;
   int pad_;
 };
@@ -75,11 +82,15 @@ struct Launch {
   int nt;            // non-temporal population stores
   int passes;        // LBM passes per HBM sweep (1, 2 or 3)
   int rch;           // LDS ring sites per direction (fused)
+  int forced;        // some instance has forced steps (random-field force)
   hipStream_t stream;
 };
 
 // Kernel launchers (lbc_kernels.hip).
 hipError_t launch_interval(const Launch& L);
+// The step kernel with the random-field body force (lbc_forced.hip; the
+// reference mode only).
+hipError_t launch_interval_forced(const Launch& L);
 // LDS ring sites per direction the fused step kernel needs with `threads`
 // threads for lattices of row length up to max_lx in physics mode `mode`,
 // and the LDS bytes of the whole launch (0 if it does not fit in one CU's
@@ -113,17 +124,39 @@ inline uint32_t wave_word(int Lx, int Ly, int x0, int x1, int y0, int y1,
          ((uint32_t)special << kWaveSpecialBit);
 }
 
-// HBM sweeps of one interval of M steps with P passes per sweep (P >= 2):
-// the collide-only pass alone, passes 1..M-1 P at a time (the remainder
-// alone), the stream-only pass alone.  Each sweep swaps the ping-pong
-// buffers.
-constexpr int fused_sweeps(int M, int P) {
-  return M < 1 ? 1 : 2 + (M - 1) / P + (M - 1) % P;
+// Grouping of the M+1 passes of an interval into HBM sweeps with P passes
+// per sweep (P >= 2): pass n starts a group of P when it is a
+// stream-and-collide pass (1 <= n, n+P-1 <= M-1) and the group does not
+// straddle the last forced step M1 (all its passes collide steps < M1 or
+// all >= M1); every other pass runs alone.  The step kernel walks the same
+// rule.
+__host__ __device__ constexpr bool fused_group_at(int n, int M, int P, int M1) {
+  return n >= 1 && n + P - 1 <= M - 1 && (n + P - 1 < M1 || n >= M1);
+}
+
+// HBM sweeps of that grouping; each sweep swaps the ping-pong buffers.
+constexpr int fused_sweeps(int M, int P, int M1 = 0) {
+  int s = 0;
+  for (int n = 0; n <= M; s++) n += fused_group_at(n, M, P, M1) ? P : 1;
+  return s;
 }
 size_t fused_lds_bytes(int storage, int ring_max, int rch, int passes);
 hipError_t launch_flip(const Launch& L, int n_inst, const int* flip_mask_dev);
 hipError_t launch_scgf(const InstDev* insts, int inst, int n, double dT,
                        double* lam, hipStream_t stream);
+// Random fields of the forcing mode (lbc_forced.hip), for clones
+// [clone0, clone0 + n) of one instance through its FFT work slots 0..n-1:
+// draw fills each slot with complex standard normals (Philox4x32-10 counter
+// (pair, clone, interval, kFieldStream + inst)) times the square-rooted
+// eigenvalues; after the caller's in-place 2-D forward FFT, extract writes
+// the real part of the lattice block to the clones' maps.  fmn = fmx*fmy.
+constexpr uint32_t kFieldStream = 0xF1E1D000u;
+hipError_t launch_field_draw(const InstDev* insts, const Counters* counters,
+                             int inst, int clone0, int n, int fmn,
+                             uint32_t seed_lo, uint32_t seed_hi,
+                             hipStream_t stream);
+hipError_t launch_field_extract(const InstDev* insts, int inst, int clone0,
+                                int n, int plane, hipStream_t stream);
 hipError_t launch_perturb(const Launch& L, int philox_prefactors,
                           uint32_t seed_lo, uint32_t seed_hi);
 hipError_t launch_select(const InstDev* insts, Counters* counters, int inst0,

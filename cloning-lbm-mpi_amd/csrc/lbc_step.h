@@ -1,0 +1,1303 @@
+// Step-kernel device code of the cloning-sampler inner loop for gfx950
+// (MI355X), shared by the translation units that instantiate it:
+// storage traits, Philox, the collision, HBM and LDS access, the site
+// passes, the multi-pass sweeps and the interval kernels with their
+// launchers.  Internal; included by lbc_kernels.hip and lbc_forced.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lbc_internal.h"
+
+namespace lbc {
+namespace {
+
+// D2Q9 velocity set and weights in the reference's ordering
+// (src/mpi_LBM_GK_pops_perturb.cpp:31-32).
+constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+constexpr int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+constexpr int OPP[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+constexpr double WD[9] = {4.0 / 9.0,  1.0 / 9.0,  1.0 / 9.0,
+                          1.0 / 9.0,  1.0 / 9.0,  1.0 / 36.0,
+                          1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+constexpr float WF[9] = {4.0f / 9.0f,  1.0f / 9.0f,  1.0f / 9.0f,
+                         1.0f / 9.0f,  1.0f / 9.0f,  1.0f / 36.0f,
+                         1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f};
+
+constexpr int kModeRef = 0;
+constexpr int kModeChannel = 1;
+
+enum Phase { kCollide = 0, kBoth = 1, kStream = 2 };
+
+// ---------------------------------------------------------------- storage
+template <int ST>
+struct Traits;
+
+template <>
+struct Traits<kF64> {
+  using S = double;
+  using C = double;
+  static constexpr bool shifted = false;
+  __device__ static C ld(const S* p) { return *p; }
+  __device__ static void st(S* p, C v) { *p = v; }
+  __device__ static S pack(C v) { return v; }
+  __device__ static C rest(int q) { return WD[q]; }
+};
+
+template <>
+struct Traits<kF32> {
+  using S = float;
+  using C = float;
+  static constexpr bool shifted = true;
+  __device__ static C ld(const S* p) { return *p; }
+  __device__ static void st(S* p, C v) { *p = v; }
+  __device__ static S pack(C v) { return v; }
+  __device__ static C rest(int) { return 0.0f; }
+};
+
+template <>
+struct Traits<kF16> {
+  using S = _Float16;
+  using C = float;
+  static constexpr bool shifted = true;
+  __device__ static C ld(const S* p) { return (float)*p; }
+  __device__ static void st(S* p, C v) { *p = (_Float16)v; }
+  __device__ static S pack(C v) { return (_Float16)v; }
+  __device__ static C rest(int) { return 0.0f; }
+};
+
+// bfloat16 bits, round-to-nearest-even on store.
+__device__ inline uint16_t f32_to_bf16(float v) {
+  uint32_t b = __float_as_uint(v);
+  if ((b & 0x7f800000u) == 0x7f800000u) return (uint16_t)((b >> 16) | ((b & 0xffffu) ? 0x40u : 0u));
+  b += 0x7fffu + ((b >> 16) & 1u);
+  return (uint16_t)(b >> 16);
+}
+
+template <>
+struct Traits<kBF16> {
+  using S = uint16_t;
+  using C = float;
+  static constexpr bool shifted = true;
+  __device__ static C ld(const S* p) { return __uint_as_float((uint32_t)*p << 16); }
+  __device__ static void st(S* p, C v) { *p = f32_to_bf16(v); }
+  __device__ static S pack(C v) { return f32_to_bf16(v); }
+  __device__ static C rest(int) { return 0.0f; }
+};
+
+// ---------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon et al., SC'11).
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ inline U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                            uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+  return U4{c0, c1, c2, c3};
+}
+
+// 53-bit uniform in [0,1) from two words.
+__device__ inline double u53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) *
+         (1.0 / 9007199254740992.0);
+}
+
+__device__ inline void box_muller_d(U4 r, double n[4]) {
+  const double two32inv = 2.3283064365386963e-10;
+  const double twopi = 6.283185307179586;
+  const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    double u1 = ((double)rr[2 * h] + 1.0) * two32inv;
+    double u2 = (double)rr[2 * h + 1] * two32inv;
+    double rad = sqrt(-2.0 * log(u1));
+    double ang = twopi * u2;
+    n[2 * h] = rad * cos(ang);
+    n[2 * h + 1] = rad * sin(ang);
+  }
+}
+
+__device__ inline void box_muller_f(U4 r, float n[4]) {
+  const float two32inv = 2.3283064365386963e-10f;
+  const float twopi = 6.2831853071795865f;
+  const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    float u1 = ((float)(rr[2 * h] >> 8) + 1.0f) * (two32inv * 256.0f);
+    float u2 = (float)(rr[2 * h + 1] >> 8) * (two32inv * 256.0f);
+    float rad = sqrtf(-2.0f * logf(u1));
+    float s, c;
+    sincosf(twopi * u2, &s, &c);
+    n[2 * h] = rad * c;
+    n[2 * h + 1] = rad * s;
+  }
+}
+
+// ---------------------------------------------------------------- physics
+// BGK collision with Guo forcing in the reference's operation order
+// (src/lbm_src/streamCollCompute.cpp:34-52), contraction off.
+__device__ inline void collide_f64(double (&f)[9], double beta, double tau) {
+#pragma clang fp contract(off)
+  const double omega = 1. / tau;
+  const double omega1 = 1.0 - omega;
+  const double cforce = 1.0 - 0.5 * omega;
+  double rho = 0.0, ux = 0.0, uy = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const double ft = f[k];
+    rho += ft;
+    ux += ft * (double)CX[k];
+    uy += ft * (double)CY[k];
+  }
+  ux = ux / rho + 0.5 * beta / rho;
+  uy /= rho;
+  const double u2 = -1.5 * (ux * ux + uy * uy);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const double eu = (double)CX[k] * ux + (double)CY[k] * uy;
+    const double eueu = 4.5 * eu * eu;
+    const double feq = WD[k] * rho * (1.0 + 3.0 * eu + eueu + u2);
+    const double fd = WD[k] * cforce * beta *
+                      (3. * ((double)CX[k] - ux) + 9. * (double)CX[k] * eu);
+    f[k] = f[k] * omega1 + feq * omega + fd;
+  }
+}
+
+// Same collision on shifted populations d = f - w, FP32, written pairwise:
+// for each pair of opposite directions (+e, -e) the equilibrium splits into
+// a common part t = drho + rho(4.5 eu^2 - 1.5 u^2) and an odd part
+// s = 3 rho eu, so feq_q - w_q = w_q (t +/- s).
+// a*b + c as a storage element of precision ST.  FP16: the exact a*b + c
+// rounded once to half (v_fma_mixlo_f16); FP32: fmaf; BF16: fmaf, then
+// round-to-nearest-even to bfloat16.  Spelling out the half case keeps the
+// compiler from choosing between one rounding (fma_mix) and two (fma, then
+// convert) differently in different kernels.
+template <int ST>
+__device__ inline typename Traits<ST>::S fma_store(float a, float b, float c) {
+  if constexpr (ST == kF16) {
+    unsigned r;
+    asm("v_fma_mixlo_f16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return __builtin_bit_cast(_Float16, (unsigned short)r);
+  } else {
+    return Traits<ST>::pack(fmaf(a, b, c));
+  }
+}
+
+template <int MODE, int ST>
+__device__ inline void collide_shifted(const float (&d)[9],
+                                       typename Traits<ST>::S (&o)[9],
+                                       float omega, float beta) {
+  // Every fused multiply-add is written out (the file compiles with
+  // contraction off), so the rounding is fixed by this source and every
+  // kernel that inlines it produces the same bits.
+  const float drho = ((d[0] + d[1]) + (d[2] + d[3])) +
+                     ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
+  const float jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
+  const float jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
+  const float rho = 1.0f + drho;
+  const float inv = __builtin_amdgcn_rcpf(rho);
+  float ux = jx * inv;
+  if (MODE == kModeRef) ux = fmaf(0.5f * beta, inv, ux);
+  const float uy = jy * inv;
+  const float usq = fmaf(ux, ux, uy * uy);
+  const float base = fmaf(-1.5f * rho, usq, drho);
+  const float r45 = 4.5f * rho, r3 = 3.0f * rho;
+  const float om1 = 1.0f - omega;
+  const float o0 = omega * (4.0f / 9.0f), o1 = omega * (1.0f / 9.0f),
+              o2 = omega * (1.0f / 36.0f);
+  // eu of directions 1, 2, 5, 6 (their opposites 3, 4, 7, 8 have -eu)
+  const float e1 = ux, e2 = uy, e5 = ux + uy, e6 = uy - ux;
+  const float t1 = fmaf(r45 * e1, e1, base), s1 = r3 * e1;
+  const float t2 = fmaf(r45 * e2, e2, base), s2 = r3 * e2;
+  const float t5 = fmaf(r45 * e5, e5, base), s5 = r3 * e5;
+  const float t6 = fmaf(r45 * e6, e6, base), s6 = r3 * e6;
+  // d_q' = omega_q (feq_q - w_q) + (1 - omega) d_q, omega_q = omega w_q
+  const float oq[9] = {o0, o1, o1, o1, o1, o2, o2, o2, o2};
+  const float tq[9] = {base,    t1 + s1, t2 + s2, t1 - s1, t2 - s2,
+                       t5 + s5, t6 + s6, t5 - s5, t6 - s6};
+  if (MODE != kModeRef) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) o[q] = fma_store<ST>(oq[q], tq[q], om1 * d[q]);
+  } else {
+    float n[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) n[q] = fmaf(oq[q], tq[q], om1 * d[q]);
+    // Guo forcing w_q (1 - omega/2) beta (3 (cx - ux) + 9 cx eu)
+    const float G = (1.0f - 0.5f * omega) * beta;
+    const float g0 = (4.0f / 9.0f) * G, g1 = (1.0f / 9.0f) * G,
+                g2 = (1.0f / 36.0f) * G;
+    const float a = 6.0f * ux, c = 9.0f * uy, m3ux = -3.0f * ux;
+    o[0] = fma_store<ST>(g0, m3ux, n[0]);
+    o[1] = fma_store<ST>(g1, 3.0f + a, n[1]);
+    o[3] = fma_store<ST>(g1, a - 3.0f, n[3]);
+    o[2] = fma_store<ST>(g1, m3ux, n[2]);
+    o[4] = fma_store<ST>(g1, m3ux, n[4]);
+    o[5] = fma_store<ST>(g2, (3.0f + a) + c, n[5]);
+    o[7] = fma_store<ST>(g2, (a - 3.0f) + c, n[7]);
+    o[6] = fma_store<ST>(g2, (a - 3.0f) - c, n[6]);
+    o[8] = fma_store<ST>(g2, (3.0f + a) - c, n[8]);
+  }
+}
+
+// Moments and non-equilibrium stress of one node in the order of
+// src/lbm_src/force.cpp:26-46 (FP64, contraction off).
+__device__ inline void node_stress_f64(const double (&f)[9], double& rho_o,
+                                       double& pxx_o, double& pxy_o) {
+#pragma clang fp contract(off)
+  double rho = 0.0, ux = 0.0, uy = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    rho += f[k];
+    ux += f[k] * (double)CX[k];
+    uy += f[k] * (double)CY[k];
+  }
+  uy /= rho;
+  ux /= rho;
+  double pxx = 0.0, pxy = 0.0;
+  const double u2 = -1.5 * (ux * ux + uy * uy);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const double eu = (double)CX[k] * ux + (double)CY[k] * uy;
+    const double eueu = 4.5 * eu * eu;
+    const double feq = WD[k] * rho * (1.0 + 3.0 * eu + eueu + u2);
+    const double fneq = f[k] - feq;
+    pxx += fneq * (double)CX[k] * (double)CX[k];
+    pxy += fneq * (double)CX[k] * (double)CY[k];
+  }
+  rho_o = rho;
+  pxx_o = pxx;
+  pxy_o = pxy;
+}
+
+// Shifted FP32 version: returns drho and the non-equilibrium stresses.
+__device__ inline void node_stress_shifted(const float (&d)[9], float& drho_o,
+                                           float& pxx_o, float& pxy_o) {
+  const float drho = ((d[0] + d[1]) + (d[2] + d[3])) +
+                     ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
+  const float jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
+  const float jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
+  const float rho = 1.0f + drho;
+  const float inv = 1.0f / rho;
+  const float ux = jx * inv, uy = jy * inv;
+  const float usq = 1.5f * (ux * ux + uy * uy);
+  float pxx = 0.0f, pxy = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const float eu = (float)CX[k] * ux + (float)CY[k] * uy;
+    const float deq = WF[k] * (drho + rho * (3.0f * eu + 4.5f * eu * eu - usq));
+    const float ne = d[k] - deq;
+    pxx += ne * (float)(CX[k] * CX[k]);
+    pxy += ne * (float)(CX[k] * CY[k]);
+  }
+  drho_o = drho;
+  pxx_o = pxx;
+  pxy_o = pxy;
+}
+
+// Bounce-back directions of the square ring at (x,y) (the writes of
+// computeSquareBounceBack_TEST, src/lbm_src/square.cpp:24-50).
+__device__ inline int square_mask(int x, int y, int x0, int x1, int y0, int y1) {
+  if (x < x0 || x > x1 || y < y0 || y > y1) return 0;
+  const bool inx = (x > x0 && x < x1), iny = (y > y0 && y < y1);
+  if (inx && iny) return 0;
+  if (y == y0 && inx) return (1 << 4) | (1 << 7) | (1 << 8);
+  if (y == y1 && inx) return (1 << 2) | (1 << 5) | (1 << 6);
+  if (x == x0 && iny) return (1 << 3) | (1 << 6) | (1 << 7);
+  if (x == x1 && iny) return (1 << 1) | (1 << 5) | (1 << 8);
+  if (x == x0) return (y == y0) ? (1 << 7) : (1 << 6);
+  return (y == y1) ? (1 << 5) : (1 << 8);
+}
+
+// Wall bounce-back directions (src/lbm_src/domain_noSlipWalls.cpp:10-21).
+__device__ inline int wall_mask(int y, int Ly) {
+  int m = 0;
+  if (y == 0) m |= (1 << 2) | (1 << 5) | (1 << 6);
+  if (y == Ly - 1) m |= (1 << 4) | (1 << 7) | (1 << 8);
+  return m;
+}
+
+// Channel x boundaries on the post-stream populations of one site
+// (no reference counterpart; the oracle states the same rules):
+//  inlet  x = 0:    feq(rho_in, (u_in, 0)), rho_in = (f0 + f2 + f4 +
+//                   2 (f3 + f6 + f7)) / (1 - u_in) from the populations that
+//                   streamed in from the domain;
+//  outlet x = Lx-1: after the zero-gradient ghost copy, feq(1, u) with u the
+//                   site's own velocity (the outlet holds the reference
+//                   density).
+template <int ST>
+__device__ inline void channel_x_boundary(typename Traits<ST>::C (&f)[9],
+                                          bool inlet, double u_in,
+                                          double inlet_k) {
+  using T = Traits<ST>;
+  if constexpr (!T::shifted) {
+#pragma clang fp contract(off)
+    double rho, ux, uy;
+    if (inlet) {
+      rho = (f[0] + f[2] + f[4] + 2.0 * (f[3] + f[6] + f[7])) / (1.0 - u_in);
+      ux = u_in;
+      uy = 0.0;
+    } else {
+      double r = 0.0, jx = 0.0, jy = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        r += f[k];
+        jx += f[k] * (double)CX[k];
+        jy += f[k] * (double)CY[k];
+      }
+      ux = jx / r;
+      uy = jy / r;
+      rho = 1.0;
+    }
+    const double u2 = -1.5 * (ux * ux + uy * uy);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const double eu = (double)CX[k] * ux + (double)CY[k] * uy;
+      const double eueu = 4.5 * eu * eu;
+      f[k] = WD[k] * rho * (1.0 + 3.0 * eu + eueu + u2);
+    }
+  } else {
+    // Shifted storage (f = w + d): both rules in one branch-free evaluation,
+    // so a wave holding inlet and outlet lanes runs it once.
+    const float ui = (float)u_in;
+    const float kin = (float)inlet_k;  // 1 / (1 - u_in)
+    const float D = (f[0] + f[2] + f[4]) + 2.0f * (f[3] + f[6] + f[7]);
+    const float dr = ((f[0] + f[1]) + (f[2] + f[3])) +
+                     ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
+    const float jx = (f[1] - f[3]) + (f[5] - f[6]) + (f[8] - f[7]);
+    const float jy = (f[2] - f[4]) + (f[5] + f[6]) - (f[7] + f[8]);
+    const float inv = __builtin_amdgcn_rcpf(1.0f + dr);
+    // inlet: rho_in - 1 = (D + u_in) / (1 - u_in), since the weights of the
+    // known directions sum to 1 - (w1 + w5 + w8) and w-shifts cancel
+    const float drho = inlet ? (D + ui) * kin : 0.0f;
+    const float ux = inlet ? ui : jx * inv;
+    const float uy = inlet ? 0.0f : jy * inv;
+    const float rho = 1.0f + drho;
+    const float usq = 1.5f * fmaf(ux, ux, uy * uy);
+    const float A = fmaf(-rho, usq, drho);
+    const float r45 = 4.5f * rho, r3 = 3.0f * rho;
+    const float e1 = ux, e2 = uy, e5 = ux + uy, e6 = uy - ux;
+    const float t1 = fmaf(r45 * e1, e1, A), s1 = r3 * e1;
+    const float t2 = fmaf(r45 * e2, e2, A), s2 = r3 * e2;
+    const float t5 = fmaf(r45 * e5, e5, A), s5 = r3 * e5;
+    const float t6 = fmaf(r45 * e6, e6, A), s6 = r3 * e6;
+    f[0] = WF[0] * A;
+    f[1] = WF[1] * (t1 + s1);
+    f[3] = WF[3] * (t1 - s1);
+    f[2] = WF[2] * (t2 + s2);
+    f[4] = WF[4] * (t2 - s2);
+    f[5] = WF[5] * (t5 + s5);
+    f[7] = WF[7] * (t5 - s5);
+    f[6] = WF[6] * (t6 + s6);
+    f[8] = WF[8] * (t6 - s6);
+  }
+}
+
+// Per-instance scalars held in registers by the step kernel.
+struct Geo {
+  int Lx, Ly, x0, x1, y0, y1, plane;
+  int sx, sy;
+};
+
+// Buffer-resource access to one clone's populations: the per-direction
+// plane offsets are wave-uniform (SGPR soffset), so a load or store costs no
+// vector address arithmetic.  dword3 0x00020000 is the raw-buffer format of
+// gfx9-family targets.
+__device__ inline __amdgpu_buffer_rsrc_t clone_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, bytes,
+                                           0x00020000);
+}
+
+template <int ST>
+struct BufIO;
+
+template <>
+struct BufIO<kF64> {
+  using R = double;  // raw element as loaded
+  __device__ static R raw(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+  }
+  __device__ static double cvt(R v) { return v; }
+  __device__ static double ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return raw(r, vo, so);
+  }
+  template <bool NT>
+  __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(
+        __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
+        vo, so, NT ? 2 : 0);
+  }
+  template <bool NT>
+  __device__ static void st_raw(__amdgpu_buffer_rsrc_t r, int vo, int so,
+                                double v) {
+    st<NT>(r, vo, so, v);
+  }
+};
+
+template <>
+struct BufIO<kF32> {
+  using R = float;
+  __device__ static R raw(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+  }
+  __device__ static float cvt(R v) { return v; }
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return raw(r, vo, so);
+  }
+  template <bool NT>
+  __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo,
+                                          so, NT ? 2 : 0);
+  }
+  template <bool NT>
+  __device__ static void st_raw(__amdgpu_buffer_rsrc_t r, int vo, int so,
+                                float v) {
+    st<NT>(r, vo, so, v);
+  }
+};
+
+template <>
+struct BufIO<kF16> {
+  using R = unsigned short;
+  __device__ static R raw(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
+  }
+  __device__ static float cvt(R v) { return (float)__builtin_bit_cast(_Float16, v); }
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return cvt(raw(r, vo, so));
+  }
+  template <bool NT>
+  __device__ static void st_raw(__amdgpu_buffer_rsrc_t r, int vo, int so,
+                                _Float16 v) {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v),
+                                          r, vo, so, NT ? 2 : 0);
+  }
+  template <bool NT>
+  __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float v) {
+    st_raw<NT>(r, vo, so, (_Float16)v);
+  }
+};
+
+template <>
+struct BufIO<kBF16> {
+  using R = unsigned short;
+  __device__ static R raw(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
+  }
+  __device__ static float cvt(R v) { return __uint_as_float((uint32_t)v << 16); }
+  __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return cvt(raw(r, vo, so));
+  }
+  template <bool NT>
+  __device__ static void st_raw(__amdgpu_buffer_rsrc_t r, int vo, int so,
+                                uint16_t v) {
+    __builtin_amdgcn_raw_buffer_store_b16(v, r, vo, so, NT ? 2 : 0);
+  }
+  template <bool NT>
+  __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float v) {
+    st_raw<NT>(r, vo, so, f32_to_bf16(v));
+  }
+};
+
+// Population storage of one clone as seen by one pass: HBM through a
+// buffer resource (soffset carries the wave-uniform plane offset), or the
+// LDS ring of the two-level sweep, which holds `rch` consecutive sites
+// (a power of two) per direction.
+template <int ST>
+struct HbmIO {
+  __amdgpu_buffer_rsrc_t r;
+  int plane;
+  static constexpr int ES = (int)sizeof(typename Traits<ST>::S);
+  __device__ typename Traits<ST>::C ld(int q, int site) const {
+    return BufIO<ST>::ld(r, (q * plane + site) * ES, 0);
+  }
+  template <bool NT>
+  __device__ void st(int q, int site, typename Traits<ST>::S v) const {
+    BufIO<ST>::template st_raw<NT>(r, (q * plane + site) * ES, 0, v);
+  }
+};
+
+//
+// The rings sit from LDS address 0 (the step kernels declare no static LDS,
+// so their dynamic LDS starts there), ring k at k * BYTES, and are addressed
+// absolutely: ring and direction plane go into the instruction's offset
+// field and no base is added.
+template <int ST, int RCH, int RING = 0>
+struct LdsIO {
+  typedef __attribute__((address_space(3))) typename Traits<ST>::S LS;
+  static constexpr int ES = (int)sizeof(typename Traits<ST>::S);
+  static constexpr int BMASK = RCH * ES - 1;  // ring bytes per direction - 1
+  static constexpr int BYTES = 9 * RCH * ES;  // one ring
+  __device__ static LS* at(int q, int boff) {
+    return (LS*)(size_t)(unsigned)(RING * BYTES + q * RCH * ES + (boff & BMASK));
+  }
+  __device__ typename Traits<ST>::C ld_byte(int q, int boff) const {
+    typename Traits<ST>::S e = *at(q, boff);
+    return Traits<ST>::ld(&e);
+  }
+  __device__ typename Traits<ST>::C ld(int q, int site) const {
+    return ld_byte(q, site * ES);
+  }
+  template <bool NT>
+  __device__ void st(int q, int site, typename Traits<ST>::S v) const {
+    *at(q, site * ES) = v;
+  }
+};
+
+// Collision of one site's populations in the storage's compute type; the
+// result comes out as storage elements.
+template <int MODE, int ST>
+__device__ inline void collide_any(const typename Traits<ST>::C (&f)[9],
+                                   typename Traits<ST>::S (&o)[9], double beta,
+                                   double tau) {
+  using T = Traits<ST>;
+  if constexpr (!T::shifted) {
+    double ff[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) ff[q] = (double)f[q];
+    collide_f64(ff, MODE == kModeRef ? beta : 0.0, tau);
+#pragma unroll
+    for (int q = 0; q < 9; q++) o[q] = ff[q];
+  } else {
+    collide_shifted<MODE, ST>(f, o, (float)(1.0 / tau), (float)beta);
+  }
+}
+
+// Body force of a pass.  Drive<false>: the instance's uniform beta.
+// Drive<true>: in a forced pass (map set) the per-site force
+// beta(x, y) = (1 + eps*map[site])*beta of
+// src/lbm_src/streamCollCompute.cpp:63, in FP64 with the reference's
+// operation order; map holds the clone's random field in site order.
+template <bool FRC>
+struct Drive {
+  double beta;
+  __device__ double at(int) const { return beta; }
+};
+
+template <>
+struct Drive<true> {
+  double beta, eps;
+  const double* __restrict__ map;  // null outside the forced passes
+  __device__ double at(int site) const {
+#pragma clang fp contract(off)
+    return map ? (1.0 + eps * map[site]) * beta : beta;
+  }
+};
+
+template <bool FRC>
+__device__ inline Drive<FRC> drive_of(const InstDev* __restrict__ I, int clone,
+                                      bool forced) {
+  if constexpr (FRC)
+    return Drive<true>{I->beta, I->field_eps,
+                       forced ? I->field_map + (long long)clone * I->plane
+                              : nullptr};
+  else
+    return Drive<false>{I->beta};
+}
+
+// Storage elements of a site: collided (any pass but stream-only) or packed
+// as they are.
+template <int MODE, int ST>
+__device__ inline void finish_site(const typename Traits<ST>::C (&f)[9],
+                                   typename Traits<ST>::S (&o)[9], int PH,
+                                   double beta, double tau) {
+  if (PH != kStream) {
+    collide_any<MODE, ST>(f, o, beta, tau);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 9; q++) o[q] = Traits<ST>::pack(f[q]);
+  }
+}
+
+// Drag term of a ring node from its post-stream populations, written to
+// the slot(s) of its face(s) in the reference's face order.
+template <int MODE, int ST>
+__device__ inline void ring_drag(const Geo& g, int x, int y,
+                                 const typename Traits<ST>::C (&f)[9],
+                                 double* __restrict__ ring, double tau) {
+  using T = Traits<ST>;
+  if (MODE == kModeRef) {
+    double tW, tE, tN, tS;
+    if (!T::shifted) {
+#pragma clang fp contract(off)
+      double rho, pxx, pxy;
+      double ff[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) ff[q] = (double)f[q];
+      node_stress_f64(ff, rho, pxx, pxy);
+      const double ot = 1. / 3.;
+      const double cf = 1.0 - .5 * (1. / tau);
+      tW = rho * ot + cf * pxx;
+      tE = -rho * ot - cf * pxx;
+      tN = -cf * pxy;
+      tS = +cf * pxy;
+    } else {
+      float dr, pxx, pxy;
+      float dd[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) dd[q] = (float)f[q];
+      node_stress_shifted(dd, dr, pxx, pxy);
+      const float cf = 1.0f - 0.5f / (float)tau;
+      // the constant 1/3 of each node cancels between the west and east
+      // faces, which have the same number of nodes
+      tW = (double)(dr * (1.0f / 3.0f) + cf * pxx);
+      tE = -tW;
+      tN = (double)(-cf * pxy);
+      tS = -tN;
+    }
+    if (x == g.x0) ring[y - g.y0] = tW;
+    if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = tE;
+    if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = tN;
+    if (y == g.y0) ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = tS;
+  } else {
+    const int sm = square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
+    double t = 0.0;
+    if (!T::shifted) {
+#pragma clang fp contract(off)
+#pragma unroll
+      for (int q = 0; q < 9; q++)
+        if ((sm >> q) & 1) t += (double)(-2 * CX[q]) * (double)f[q];
+    } else {
+      float tf = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 9; q++)
+        if ((sm >> q) & 1) tf += (float)(-2 * CX[q]) * (float)f[q];
+      t = (double)tf;
+    }
+    if (x == g.x0) ring[y - g.y0] = t;
+    else if (x == g.x1) ring[(g.sy + 1) + (y - g.y0)] = t;
+    else if (y == g.y1) ring[2 * (g.sy + 1) + (x - g.x0)] = t;
+    else ring[2 * (g.sy + 1) + (g.sx + 1) + (x - g.x0)] = t;
+  }
+}
+
+// Uses all nine loaded registers at one point, so the compiler issues the
+// nine loads back to back and waits once, instead of interleaving a
+// conversion (and a wait) after each load.  Emits no instruction.
+template <class R>
+__device__ inline void join9(R (&v)[9]) {
+  asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]),
+               "+v"(v[5]), "+v"(v[6]), "+v"(v[7]), "+v"(v[8]));
+}
+
+// Pulls of a fast-path site.  From HBM: one buffer load per direction at
+// the site's byte offset with the wave-uniform soffset q*plane - cy*Lx - cx;
+// from the LDS ring: the ring slot of site - cy*Lx - cx.  voL / voR (sL / sR)
+// fold in the x boundary for directions with cx = +1 / -1.
+template <int ST, int RCH, int RING>
+__device__ inline void pull9(const LdsIO<ST, RCH, RING>& io,
+                             typename Traits<ST>::C (&f)[9], int site, int adjL,
+                             int adjR, int rowb) {
+  constexpr int ES = LdsIO<ST, RCH, RING>::ES;
+  // source byte offsets: column x - cx (x boundary folded in), then the row
+  // y - cy one row-length (rowb bytes) away
+  const int b0 = site * ES;
+  const int bP = b0 + (adjL - 1) * ES, bM = b0 + (adjR + 1) * ES;
+  const int col[9] = {b0, bP, b0, bM, b0, bP, bM, bM, bP};
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+    f[q] = io.ld_byte(q, CY[q] > 0 ? col[q] - rowb
+                                   : (CY[q] < 0 ? col[q] + rowb : col[q]));
+}
+
+template <int ST>
+__device__ inline void own9(const HbmIO<ST>& io, typename Traits<ST>::C (&f)[9],
+                            int site, const int (&sown)[9]) {
+  constexpr int ES = HbmIO<ST>::ES;
+  typename BufIO<ST>::R v[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) v[q] = BufIO<ST>::raw(io.r, site * ES, sown[q]);
+  join9(v);
+#pragma unroll
+  for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(v[q]);
+}
+
+template <int ST, int RCH, int RING>
+__device__ inline void own9(const LdsIO<ST, RCH, RING>& io,
+                            typename Traits<ST>::C (&f)[9], int site,
+                            const int (&)[9]) {
+#pragma unroll
+  for (int q = 0; q < 9; q++) f[q] = io.ld(q, site);
+}
+
+template <int ST, bool NT>
+__device__ inline void put9(const HbmIO<ST>& io, const typename Traits<ST>::S (&o)[9],
+                            int site, const int (&sown)[9]) {
+  constexpr int ES = HbmIO<ST>::ES;
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+    BufIO<ST>::template st_raw<NT>(io.r, site * ES, sown[q], o[q]);
+}
+
+template <int ST, bool NT, int RCH, int RING>
+__device__ inline void put9(const LdsIO<ST, RCH, RING>& io,
+                            const typename Traits<ST>::S (&o)[9], int site,
+                            const int (&)[9]) {
+#pragma unroll
+  for (int q = 0; q < 9; q++) io.template st<NT>(q, site, o[q]);
+}
+
+// Per-pass constants of the pull and store addressing.
+template <int ST>
+struct PassConst {
+  int rowb;     // one lattice row in bytes (LDS source)
+  int soff[9];  // pull soffset of direction q, in bytes (HBM source; one
+                // row less for cx = -1, whose voffset is one row ahead)
+  int sown[9];  // own-site soffset of direction q, in bytes
+  int adjL, adjR;
+  __device__ PassConst(const Geo& g, int mode) {
+    constexpr int ES = (int)sizeof(typename Traits<ST>::S);
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      const int off = CY[q] * g.Lx + CX[q];
+      sown[q] = q * g.plane * ES;
+      soff[q] = (q * g.plane - off - (CX[q] < 0 ? g.Lx : 0)) * ES;
+    }
+    rowb = g.Lx * ES;
+    adjL = (mode == kModeRef) ? g.Lx : 1;
+    adjR = (mode == kModeRef) ? -g.Lx : -1;
+  }
+};
+
+// What one wave's 64 consecutive sites touch (decoded from the instance's
+// wave table, one word per block of 64 sites, read with a scalar load):
+// wave-uniform, so the per-site branches on it are scalar and most waves
+// compute no lane coordinates at all.
+struct WaveGeo {
+  int x, y;      // first site
+  bool edge;     // holds an x = 0 or x = Lx-1 site
+  bool special;  // holds a wall-row site or a site of the square's box
+};
+
+__device__ inline WaveGeo wave_of(const InstDev* __restrict__ I, int block) {
+  const __attribute__((address_space(4))) uint32_t* t =
+      (const __attribute__((address_space(4))) uint32_t*)I->wave_info;
+  const uint32_t w = t[block];
+  WaveGeo g;
+  g.x = (int)(w & kWaveCoordMask);
+  g.edge = (w >> kWaveEdgeBit) & 1u;
+  g.y = (int)((w >> 16) & kWaveCoordMask);
+  g.special = (w >> kWaveSpecialBit) & 1u;
+  return g;
+}
+
+// Lane coordinates inside a wave (edge / special waves only).
+__device__ inline void lane_xy(const Geo& g, const WaveGeo& w, int lane, int& x,
+                               int& y) {
+  x = w.x + lane;
+  y = w.y;
+  if (g.Lx >= 64) {
+    if (x >= g.Lx) {
+      x -= g.Lx;
+      y += 1;
+    }
+  } else {
+    const int r = x / g.Lx;
+    y += r;
+    x -= r * g.Lx;
+  }
+}
+
+// x-boundary offsets of a lane for the pulls of directions with cx = +1 / -1
+// (periodic wrap in REF mode, inlet/outlet ghost columns in channel mode);
+// zero away from the x boundaries.
+template <int ST>
+__device__ inline void x_adjust(const Geo& g, const WaveGeo& w, int lane,
+                                const PassConst<ST>& pc, int& aL, int& aR) {
+  aL = aR = 0;
+  if (w.edge) {
+    int x, y;
+    lane_xy(g, w, lane, x, y);
+    aL = (x == 0) ? pc.adjL : 0;
+    aR = (x == g.Lx - 1) ? pc.adjR : 0;
+  }
+}
+
+// Issues the nine HBM pulls of a site (raw elements; their wait comes at
+// the first use, so a caller can issue them a chunk ahead).
+template <int ST>
+__device__ inline void pull_issue(const Geo& g, const WaveGeo& w, int lane,
+                                  int site, const HbmIO<ST>& src,
+                                  const PassConst<ST>& pc,
+                                  typename BufIO<ST>::R (&v)[9]) {
+  constexpr int ES = HbmIO<ST>::ES;
+  int aL, aR;
+  x_adjust(g, w, lane, pc, aL, aR);
+  // A raw buffer access is range-checked on voffset alone, so voffset must
+  // stay non-negative: directions with cx = -1 carry +Lx sites in voffset
+  // and -Lx in their soffset (adjR >= -Lx).
+  const int vo = site * ES, voL = vo + aL * ES, voR = vo + (aR + g.Lx) * ES;
+#pragma unroll
+  for (int q = 0; q < 9; q++)
+    v[q] = BufIO<ST>::raw(src.r, CX[q] > 0 ? voL : (CX[q] < 0 ? voR : vo),
+                          pc.soff[q]);
+}
+
+// The nine pulled populations of a site.
+template <int ST>
+__device__ inline void pull_site(const Geo& g, const WaveGeo& w, int lane,
+                                 int site, const HbmIO<ST>& src,
+                                 const PassConst<ST>& pc,
+                                 typename Traits<ST>::C (&f)[9]) {
+  typename BufIO<ST>::R v[9];
+  pull_issue(g, w, lane, site, src, pc, v);
+  join9(v);
+#pragma unroll
+  for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(v[q]);
+}
+
+template <int ST, int RCH, int RING>
+__device__ inline void pull_site(const Geo& g, const WaveGeo& w, int lane,
+                                 int site, const LdsIO<ST, RCH, RING>& src,
+                                 const PassConst<ST>& pc,
+                                 typename Traits<ST>::C (&f)[9]) {
+  int aL, aR;
+  x_adjust(g, w, lane, pc, aL, aR);
+  pull9(src, f, site, aL, aR, pc.rowb);
+}
+
+// The rest of one site of one pass, given its nine pulled populations f
+// (own populations in the collide-only pass).  Waves holding wall or square
+// lanes load each lane's own populations and take the bounce-back ones
+// from there (ring lanes add their drag term, solid lanes store the rest
+// state); channel x-boundary lanes apply the boundary rule.  Collision
+// (except in the stream-only pass) and nine stores follow.
+template <int MODE, int ST, bool NT, class SRC, class DST, class DV>
+__device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
+                                   int site, typename Traits<ST>::C (&f)[9],
+                                   const SRC& src, const DST& dst,
+                                   const PassConst<ST>& pc,
+                                   double* __restrict__ ring, const DV& dv,
+                                   double tau, const InstDev* __restrict__ I,
+                                   int PH) {
+  using T = Traits<ST>;
+  using C = typename T::C;
+  bool solid = false;
+  if (PH != kCollide && (w.edge || w.special)) {
+    int x, y;
+    lane_xy(g, w, lane, x, y);
+    if (w.special) {
+      C own[9];
+      own9(src, own, site, pc.sown);
+      const int m = wall_mask(y, g.Ly) |
+                    square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
+#pragma unroll
+      for (int q = 0; q < 9; q++)
+        if ((m >> q) & 1) f[q] = own[OPP[q]];
+      const bool in_box = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1;
+      solid = in_box && x > g.x0 && x < g.x1 && y > g.y0 && y < g.y1;
+      if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1))
+        channel_x_boundary<ST>(f, x == 0, I->u_in, I->inlet_k);
+      if (in_box && !solid) ring_drag<MODE, ST>(g, x, y, f, ring, tau);
+    } else if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1)) {
+      channel_x_boundary<ST>(f, x == 0, I->u_in, I->inlet_k);
+    }
+  }
+  typename T::S o[9];
+  finish_site<MODE, ST>(f, o, PH, dv.at(site), tau);
+  if (w.special && solid) {
+    // the reference resets the solid interior to w every step; its
+    // post-collision values never reach a fluid node
+#pragma unroll
+    for (int q = 0; q < 9; q++) o[q] = T::pack(T::rest(q));
+  }
+  put9<ST, NT>(dst, o, site, pc.sown);
+}
+
+// One site of one pass: the collide-only pass loads the site's own
+// populations, every other pass pulls its nine populations (x boundary
+// folded in), then site_finish.
+template <int MODE, int ST, bool NT, class SRC, class DST, class DV>
+__device__ inline void site_step(const Geo& g, const WaveGeo& w, int lane,
+                                 int site, const SRC& src, const DST& dst,
+                                 const PassConst<ST>& pc,
+                                 double* __restrict__ ring, const DV& dv,
+                                 double tau, const InstDev* __restrict__ I,
+                                 int PH) {
+  typename Traits<ST>::C f[9];
+  if (PH == kCollide)
+    own9(src, f, site, pc.sown);
+  else
+    pull_site(g, w, lane, site, src, pc, f);
+  site_finish<MODE, ST, NT>(g, w, lane, site, f, src, dst, pc, ring, dv, tau,
+                            I, PH);
+}
+
+// One HBM-to-HBM pass of one thread over its sites tid, tid + T, ...
+template <int MODE, int ST, bool NT, class DV>
+__device__ inline void site_pass(const Geo& g, int tid, int nthr,
+                                 const HbmIO<ST>& src, const HbmIO<ST>& dst,
+                                 double* __restrict__ ring, const DV& dv,
+                                 double tau, const InstDev* __restrict__ I,
+                                 int PH) {
+  const PassConst<ST> pc(g, MODE);
+  const int lane = tid & 63;
+  const int wblk = __builtin_amdgcn_readfirstlane(tid) >> 6;  // wave in chunk
+  const int bpc = nthr >> 6;                                  // waves per chunk
+  const int nch = (g.plane + nthr - 1) / nthr;
+  int site = tid;
+  for (int c = 0; c < nch; c++) {
+    if (site < g.plane)
+      site_step<MODE, ST, NT>(g, wave_of(I, c * bpc + wblk), lane, site, src,
+                              dst, pc, ring, dv, tau, I, PH);
+    site += nthr;
+  }
+}
+
+// Reach of a pull in sites: Lx + 1 with clamped x boundaries; with the
+// periodic x wrap a site at x = 0 pulls from x = Lx-1 one row away, up to
+// 2*Lx - 1 sites ahead (and symmetrically behind).
+__host__ __device__ inline int pull_reach(int mode, int Lx) {
+  return mode == kModeRef ? 2 * Lx : Lx + 1;
+}
+
+// Workgroup barrier that orders LDS traffic only: LDS writes before it are
+// visible to LDS reads after it, while HBM loads and stores in flight keep
+// going (__syncthreads would wait for them too).  Inside one sweep nothing
+// reads HBM that the sweep writes, so this is all the chunk steps need.  The
+// memory clobber keeps the compiler from moving memory accesses across.
+__device__ inline void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// NP (2 or 3) stream-and-collide passes in one sweep over the lattice.  The
+// first pass reads HBM and writes its post-collision populations into LDS
+// ring 0; each later pass runs L chunks behind the one before, reads the
+// previous ring, and writes the next ring or (the last pass) HBM.  A chunk
+// is T consecutive sites (one per thread); L*T >= the pull reach, so every
+// pull finds its source already in the ring, and a ring (>= 2L + 2 chunks)
+// is never overwritten while a later pass still needs it.  One LDS barrier
+// between consecutive passes of a chunk; the sweep's caller ends it with a
+// full barrier.  drag[k*ring_max] holds pass k's drag slots.
+template <int MODE, int ST, bool NT, int RCH, int NP, class DV>
+__device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
+                                   const HbmIO<ST>& src, const HbmIO<ST>& dst,
+                                   double* drag, int ring_max, const DV& dv,
+                                   double tau, const InstDev* __restrict__ I) {
+  static_assert(NP == 2 || NP == 3, "two or three passes per sweep");
+  const PassConst<ST> pc(g, MODE);
+  const int CH = nthr;
+  const int L = (pull_reach(MODE, g.Lx) + CH - 1) / CH;
+  const int nch = (g.plane + CH - 1) / CH;
+  const int lane = tid & 63;
+  const int wblk = __builtin_amdgcn_readfirstlane(tid) >> 6;  // wave in chunk
+  const int bpc = CH >> 6;                                    // waves per chunk
+  const LdsIO<ST, RCH, 0> r0{};
+  const LdsIO<ST, RCH, 1> r1{};
+  // The first pass's HBM pulls are issued one chunk ahead, so they travel
+  // while the later passes of the chunk before compute.  They are issued
+  // for every lane: past the lattice they read padding or, past the clone,
+  // zeros (raw buffer range check), and nobody uses them; a conditional
+  // load would make the compiler carry the old values along and wait.
+  typename BufIO<ST>::R pre[9];
+  pull_issue(g, wave_of(I, wblk), lane, tid, src, pc, pre);
+  for (int c = 0; c < nch + (NP - 1) * L; c++) {
+    if (c < nch) {
+      const int s = c * CH + tid;
+      const WaveGeo w = wave_of(I, c * bpc + wblk);
+      typename Traits<ST>::C f[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(pre[q]);
+      // the converted values first, then the next chunk's loads into the
+      // same registers (loads cannot move above a memory clobber)
+      asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]), "+v"(f[3]),
+                   "+v"(f[4]), "+v"(f[5]), "+v"(f[6]), "+v"(f[7]), "+v"(f[8])
+                   :: "memory");
+      pull_issue(g, wave_of(I, (c + 1) * bpc + wblk), lane, s + CH, src, pc,
+                 pre);
+      if (s < g.plane)
+        site_finish<MODE, ST, false>(g, w, lane, s, f, src, r0, pc, drag, dv,
+                                     tau, I, kBoth);
+    }
+    lds_barrier();
+    if constexpr (NP == 3) {
+      const int cm = c - L;
+      if (cm >= 0 && cm < nch) {
+        const int s = cm * CH + tid;
+        if (s < g.plane)
+          site_step<MODE, ST, false>(g, wave_of(I, cm * bpc + wblk), lane, s, r0,
+                                     r1, pc, drag + ring_max, dv, tau, I,
+                                     kBoth);
+      }
+      lds_barrier();
+    }
+    const int cz = c - (NP - 1) * L;
+    if (cz >= 0) {
+      const int s = cz * CH + tid;
+      if (s < g.plane) {
+        double* dz = drag + (NP - 1) * ring_max;
+        if constexpr (NP == 3)
+          site_step<MODE, ST, NT>(g, wave_of(I, cz * bpc + wblk), lane, s, r1,
+                                  dst, pc, dz, dv, tau, I, kBoth);
+        else
+          site_step<MODE, ST, NT>(g, wave_of(I, cz * bpc + wblk), lane, s, r0,
+                                  dst, pc, dz, dv, tau, I, kBoth);
+      }
+    }
+  }
+}
+
+// Sum of the drag slots in the reference's order: west face (y ascending),
+// east face, north face (x ascending), south face
+// (src/lbm_src/force.cpp:23,54,84,114,141).
+template <int MODE>
+__device__ inline double ring_sum(const double* ring, int sx, int sy) {
+#pragma clang fp contract(off)
+  double fW = 0.0, fE = 0.0, fN = 0.0, fS = 0.0;
+  for (int i = 0; i <= sy; i++) fW += ring[i];
+  for (int i = 0; i <= sy; i++) fE += ring[(sy + 1) + i];
+  const int lo = (MODE == kModeRef) ? 0 : 1;
+  const int hi = (MODE == kModeRef) ? sx : sx - 1;
+  for (int i = lo; i <= hi; i++) fN += ring[2 * (sy + 1) + i];
+  for (int i = lo; i <= hi; i++) fS += ring[2 * (sy + 1) + (sx + 1) + i];
+  return fW + fE + fN + fS;
+}
+
+struct KernelSetup {
+  Geo g;
+  int M;
+  double beta, tau, scale, shift;
+  void* A;
+  void* B;
+  double* drag;
+  int clone_bytes;
+};
+
+template <int ST>
+__device__ inline KernelSetup setup_clone(const InstDev* __restrict__ I,
+                                          const WorkItem& wi, int par) {
+  using S = typename Traits<ST>::S;
+  KernelSetup k;
+  k.g.Lx = I->Lx;
+  k.g.Ly = I->Ly;
+  k.g.x0 = I->xmin;
+  k.g.x1 = I->xmax;
+  k.g.y0 = I->ymin;
+  k.g.y1 = I->ymax;
+  k.g.plane = I->plane;
+  k.g.sx = k.g.x1 - k.g.x0;
+  k.g.sy = k.g.y1 - k.g.y0;
+  k.M = I->steps;
+  k.beta = I->beta;
+  k.tau = I->tau;
+  k.scale = I->scale;
+  k.shift = I->shift;
+  k.A = (S*)I->buf[par] + (long long)wi.clone * I->clone_stride;
+  k.B = (S*)I->buf[par ^ 1] + (long long)wi.clone * I->clone_stride;
+  k.drag = I->drag + (long long)wi.clone * k.M;
+  k.clone_bytes = (int)(I->clone_stride * (long long)sizeof(S));
+  return k;
+}
+
+// Drag of pass n (n >= 1) from its slots: F*scale - shift, accumulated in
+// FP64 in pass order (src/mpi_LBM_GK_pops_perturb.cpp:445-459).
+template <int MODE>
+__device__ inline void take_drag(const KernelSetup& k, const double* ring, int n,
+                                 double& acc) {
+#pragma clang fp contract(off)
+  double F = ring_sum<MODE>(ring, k.g.sx, k.g.sy);
+  F = F * k.scale;
+  F = F - k.shift;
+  k.drag[n - 1] = F;
+  acc += F;
+}
+
+template <int MODE>
+__device__ inline void finish_clone(const InstDev* __restrict__ I,
+                                    const WorkItem& wi, double acc) {
+#pragma clang fp contract(off)
+  const double o = acc * I->dt;
+  I->obs[wi.clone] = o;
+  I->logw[wi.clone] = I->bias * o;
+}
+
+// One cloning interval of one clone per workgroup, one pass per sweep.  The
+// post-stream state S_0 is in buf[parity]; pass 0 collides it, passes
+// 1..M-1 stream and collide, pass M streams only, so the interval ends with
+// S_M in buf[parity ^ ((M+1)&1)].  Drag F_n is taken on S_n in pass n.
+// Pass n collides step n; with FRC the passes n < forced_steps collide with
+// the clone's random-field force (src/mpi_LBM_GK_forcing.cpp:335-364).
+template <int MODE, int ST, bool NT, bool FRC>
+__global__ void __launch_bounds__(1024)
+    k_interval(const InstDev* __restrict__ insts,
+               const WorkItem* __restrict__ items,
+               const Counters* __restrict__ counters, int ring_max) {
+  extern __shared__ double ring_lds[];
+  const WorkItem wi = items[blockIdx.x];
+  const InstDev* __restrict__ I = insts + wi.inst;
+  const KernelSetup k = setup_clone<ST>(I, wi, counters->parity[wi.inst]);
+  const int nthr = blockDim.x, tid = threadIdx.x;
+  const HbmIO<ST> ioA{clone_rsrc(k.A, k.clone_bytes), k.g.plane};
+  const HbmIO<ST> ioB{clone_rsrc(k.B, k.clone_bytes), k.g.plane};
+  const int M1 = FRC ? I->forced_steps : 0;
+  double acc = 0.0;
+  for (int n = 0; n <= k.M; ++n) {
+    const HbmIO<ST>& src = (n & 1) ? ioB : ioA;
+    const HbmIO<ST>& dst = (n & 1) ? ioA : ioB;
+    double* ring = ring_lds + (n & 1) * ring_max;
+    const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
+    site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, ring,
+                            drive_of<FRC>(I, wi.clone, n < M1), k.tau, I, ph);
+    __syncthreads();
+    if (n >= 1 && tid == 0) take_drag<MODE>(k, ring, n, acc);
+  }
+  if (tid == 0) finish_clone<MODE>(I, wi, acc);
+}
+
+// The same interval with NP passes per HBM sweep where it can: the
+// collide-only pass 0 runs alone, passes 1..M-1 (stream and collide) run NP
+// at a time through the LDS rings (fused_sweep; the remainder alone), and
+// the stream-only pass M runs alone, fused_sweeps(M, NP, M1) sweeps in all,
+// so the interval ends with S_M in buf[parity ^ (fused_sweeps(M, NP, M1) & 1)].
+// With FRC no sweep mixes forced passes (n < M1 = forced_steps) with the
+// rest (fused_group_at), so the force is one choice per sweep.
+// LDS: NP-1 population rings of 9*rch sites, then 2*NP drag slot arrays
+// (NP per sweep, double-buffered over sweeps).
+template <int MODE, int ST, bool NT, int RCH, int NP, bool FRC>
+__device__ inline void interval_fused(const InstDev* __restrict__ insts,
+                                      const WorkItem* __restrict__ items,
+                                      const Counters* __restrict__ counters,
+                                      int ring_max) {
+  extern __shared__ double lds[];
+  const WorkItem wi = items[blockIdx.x];
+  const InstDev* __restrict__ I = insts + wi.inst;
+  const KernelSetup k = setup_clone<ST>(I, wi, counters->parity[wi.inst]);
+  const int nthr = blockDim.x, tid = threadIdx.x;
+  const HbmIO<ST> ioA{clone_rsrc(k.A, k.clone_bytes), k.g.plane};
+  const HbmIO<ST> ioB{clone_rsrc(k.B, k.clone_bytes), k.g.plane};
+  double* const slots = lds + (NP - 1) * (LdsIO<ST, RCH>::BYTES / 8);
+  const int M1 = FRC ? I->forced_steps : 0;
+  double acc = 0.0;
+  int sweep = 0;
+  for (int n = 0; n <= k.M; sweep++) {
+    const HbmIO<ST>& src = (sweep & 1) ? ioB : ioA;
+    const HbmIO<ST>& dst = (sweep & 1) ? ioA : ioB;
+    double* drag = slots + (sweep & 1) * NP * ring_max;
+    const Drive<FRC> dv = drive_of<FRC>(I, wi.clone, n < M1);
+    if (fused_group_at(n, k.M, NP, M1)) {
+      fused_sweep<MODE, ST, NT, RCH, NP>(k.g, tid, nthr, src, dst, drag,
+                                         ring_max, dv, k.tau, I);
+      __syncthreads();
+      if (tid == 0)
+        for (int p = 0; p < NP; p++)
+          take_drag<MODE>(k, drag + p * ring_max, n + p, acc);
+      n += NP;
+    } else {
+      const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
+      site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, drag, dv, k.tau, I, ph);
+      __syncthreads();
+      if (n >= 1 && tid == 0) take_drag<MODE>(k, drag, n, acc);
+      n += 1;
+    }
+  }
+  if (tid == 0) finish_clone<MODE>(I, wi, acc);
+}
+
+template <int MODE, int ST, bool NT, int RCH, int NP, bool FRC>
+__global__ void __launch_bounds__(1024)
+    k_interval_fused(const InstDev* __restrict__ insts,
+                     const WorkItem* __restrict__ items,
+                     const Counters* __restrict__ counters, int ring_max) {
+  interval_fused<MODE, ST, NT, RCH, NP, FRC>(insts, items, counters, ring_max);
+}
+
+// 2-byte storages, two passes per sweep: a 512-thread workgroup's ring
+// takes a quarter of the CU's LDS, so registers are capped for eight waves
+// per SIMD and four workgroups share a CU (uncapped, the SGPR count allows
+// seven waves).
+template <int MODE, int ST, bool NT, int RCH, bool FRC>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
+    k_interval_fused_w8(const InstDev* __restrict__ insts,
+                        const WorkItem* __restrict__ items,
+                        const Counters* __restrict__ counters, int ring_max) {
+  interval_fused<MODE, ST, NT, RCH, 2, FRC>(insts, items, counters, ring_max);
+}
+
+template <class K>
+hipError_t raise_lds_limit(K kernel, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  return hipFuncSetAttribute((const void*)kernel,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds);
+}
+
+// Launchers of the step kernel for one body-force kind (FRC), dispatched
+// on the context's shape: passes per sweep, ring size, non-temporal stores
+// and storage.
+template <int MODE, int ST, bool NT, int RCH, int NP, bool FRC>
+hipError_t launch_fused(const Launch& L) {
+  const size_t lds = fused_lds_bytes(ST, L.ring_max, RCH, NP);
+  if (!lds) return hipErrorInvalidValue;
+  if constexpr (sizeof(typename Traits<ST>::S) == 2 && NP == 2) {
+    hipError_t e =
+        raise_lds_limit(k_interval_fused_w8<MODE, ST, NT, RCH, FRC>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_interval_fused_w8<MODE, ST, NT, RCH, FRC>),
+                       dim3(L.n_items), dim3(L.threads), lds, L.stream, L.insts,
+                       L.items, L.counters, L.ring_max);
+  } else {
+    hipError_t e =
+        raise_lds_limit(k_interval_fused<MODE, ST, NT, RCH, NP, FRC>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_interval_fused<MODE, ST, NT, RCH, NP, FRC>),
+                       dim3(L.n_items), dim3(L.threads), lds, L.stream, L.insts,
+                       L.items, L.counters, L.ring_max);
+  }
+  return hipGetLastError();
+}
+
+template <int MODE, int ST, bool NT, int NP, bool FRC>
+hipError_t launch_fused_r(const Launch& L) {
+  switch (L.rch) {
+    case 1024: return launch_fused<MODE, ST, NT, 1024, NP, FRC>(L);
+    case 2048: return launch_fused<MODE, ST, NT, 2048, NP, FRC>(L);
+    case 4096: return launch_fused<MODE, ST, NT, 4096, NP, FRC>(L);
+    case 8192: return launch_fused<MODE, ST, NT, 8192, NP, FRC>(L);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int MODE, int ST, bool NT, bool FRC>
+hipError_t launch_interval_k(const Launch& L) {
+  if (L.passes == 2) return launch_fused_r<MODE, ST, NT, 2, FRC>(L);
+  if (L.passes == 3) return launch_fused_r<MODE, ST, NT, 3, FRC>(L);
+  const size_t lds = sizeof(double) * 2 * (size_t)L.ring_max;
+  hipLaunchKernelGGL((k_interval<MODE, ST, NT, FRC>), dim3(L.n_items),
+                     dim3(L.threads), lds, L.stream, L.insts, L.items,
+                     L.counters, L.ring_max);
+  return hipGetLastError();
+}
+
+template <int MODE, int ST, bool FRC>
+hipError_t launch_interval_t(const Launch& L) {
+  return L.nt ? launch_interval_k<MODE, ST, true, FRC>(L)
+              : launch_interval_k<MODE, ST, false, FRC>(L);
+}
+
+template <int MODE, bool FRC>
+hipError_t launch_interval_m(const Launch& L) {
+  switch (L.storage) {
+    case kF64: return launch_interval_t<MODE, kF64, FRC>(L);
+    case kF32: return launch_interval_t<MODE, kF32, FRC>(L);
+    case kF16: return launch_interval_t<MODE, kF16, FRC>(L);
+    case kBF16: return launch_interval_t<MODE, kBF16, FRC>(L);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+}  // namespace lbc

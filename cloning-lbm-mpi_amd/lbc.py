@@ -37,7 +37,8 @@ EXPORTS = (
     "lbc_store_states", "lbc_init_equilibrium", "lbc_set_snapshots",
     "lbc_set_marks", "lbc_get_marks", "lbc_run_interval", "lbc_get_drag",
     "lbc_get_logweights", "lbc_select", "lbc_gather", "lbc_advance",
-    "lbc_get_R_history", "lbc_get_scgf", "lbc_sync", "lbc_step_kernel_ms", "lbc_threads",
+    "lbc_get_R_history", "lbc_get_scgf", "lbc_set_field_maps",
+    "lbc_get_field_maps", "lbc_sync", "lbc_step_kernel_ms", "lbc_threads",
     "lbc_sites_per_thread", "lbc_step_shape", "lbc_step_host",
 )
 
@@ -53,6 +54,8 @@ class Instance(C.Structure):
         ("dt", C.c_double), ("bias", C.c_double), ("eps", C.c_double),
         ("noise_sigma", C.c_double),
         ("rank_blocks", C.c_int), ("n_snapshots", C.c_int),
+        ("forced_steps", C.c_int), ("field_eps", C.c_double),
+        ("field_sigma", C.c_double),
     ]
 
 
@@ -113,6 +116,8 @@ def lib() -> C.CDLL:
                                     C.POINTER(C.c_int)]
     L.lbc_get_scgf.argtypes = [vp, C.c_int, C.c_double, dp, C.c_int,
                                C.POINTER(C.c_int)]
+    L.lbc_set_field_maps.argtypes = [vp, C.c_int, dp]
+    L.lbc_get_field_maps.argtypes = [vp, C.c_int, dp]
     L.lbc_sync.argtypes = [vp]
     L.lbc_step_kernel_ms.argtypes = [vp, dp, C.POINTER(C.c_int)]
     L.lbc_threads.argtypes = [vp]
@@ -159,13 +164,17 @@ class InstanceSpec:
     noise_sigma: float = 0.0
     rank_blocks: int = 1
     n_snapshots: int = 0
+    forced_steps: int = 0
+    field_eps: float = 0.0
+    field_sigma: float = 0.0
 
     def c(self) -> Instance:
         return Instance(self.Lx, self.Ly, self.xmin, self.xmax, self.ymin,
                         self.ymax, self.n_clones, self.steps, self.tau,
                         self.beta, self.u_in, self.force_scale,
                         self.force_shift, self.dt, self.bias, self.eps,
-                        self.noise_sigma, self.rank_blocks, self.n_snapshots)
+                        self.noise_sigma, self.rank_blocks, self.n_snapshots,
+                        self.forced_steps, self.field_eps, self.field_sigma)
 
     @property
     def sites(self) -> int:
@@ -189,6 +198,23 @@ def ref_box(L: int, tau: float, U0: float, n_clones: int, steps: int,
                         force_shift=1.775 if zero_mean else 0.0,
                         dt=1.0 / T0, bias=alpha, eps=eps,
                         rank_blocks=rank_blocks, n_snapshots=n_snapshots)
+
+
+def forcing_box(L: int, tau: float, U0: float, n_clones: int, dT: float,
+                dT0: float, eps: float, alpha: float,
+                field_sigma: float = 0.0) -> InstanceSpec:
+    """The forcing driver's geometry, scalings and two-phase interval for
+    square side L (src/mpi_LBM_GK_forcing.cpp:131-148): floor(dT0*T0) forced
+    steps then floor((dT - dT0)*T0) uniform ones, T0 = L/U0, drag scaled by
+    1/F0 with no shift, random-field amplitude eps."""
+    T0 = L / U0
+    m1 = math.floor(dT0 * T0)
+    m2 = math.floor((dT - dT0) * T0)
+    spec = ref_box(L, tau, U0, n_clones, m1 + m2, alpha, zero_mean=False)
+    spec.forced_steps = m1
+    spec.field_eps = eps
+    spec.field_sigma = field_sigma
+    return spec
 
 
 def channel(Lx: int, Ly: int, side: int, cx: int, cy: int, tau: float,
@@ -402,6 +428,22 @@ class Sampler:
         self._check(lib().lbc_get_scgf(self._h, inst, dT, _dp(out), n.value,
                                        C.byref(n)), "lbc_get_scgf")
         return out[: n.value]
+
+    def set_field_maps(self, inst: int, maps: np.ndarray) -> None:
+        """Forcing mode: the next interval's random fields, shape
+        (n_clones, Lx, Ly) (the reference's map[y + Ly*x] layout)."""
+        s = self.specs[inst]
+        a = np.ascontiguousarray(maps, np.float64)
+        assert a.shape == (s.n_clones, s.Lx, s.Ly)
+        self._check(lib().lbc_set_field_maps(self._h, inst, _dp(a)),
+                    "lbc_set_field_maps")
+
+    def field_maps(self, inst: int) -> np.ndarray:
+        s = self.specs[inst]
+        out = np.zeros((s.n_clones, s.Lx, s.Ly))
+        self._check(lib().lbc_get_field_maps(self._h, inst, _dp(out)),
+                    "lbc_get_field_maps")
+        return out
 
     def sync(self) -> None:
         self._check(lib().lbc_sync(self._h), "lbc_sync")

@@ -68,6 +68,17 @@ typedef struct lbc_instance {
   double noise_sigma;         /* CHANNEL: population noise after a copy */
   int rank_blocks;            /* total_R summed in this many equal blocks (>=1) */
   int n_snapshots;            /* REF: snapshots mixed by the perturbation (NN) */
+  /* Forcing-mode perturbation (src/mpi_LBM_GK_forcing.cpp:333-395; REF
+   * mode): the first forced_steps steps of every interval (lbmTimeSteps1)
+   * run with the body force (1 + field_eps*map(x,y))*beta, map a fresh
+   * Gaussian random field per clone and interval drawn on the device by
+   * circulant embedding of the covariance exp(-r^2/(2 field_sigma^2)) in a
+   * (2Lx-1) x (2Ly-1) torus, negative eigenvalues set to zero
+   * (src/perturbation_routines/generate_random_field.cpp:9-181;
+   * field_sigma 0 = the reference's 32).  forced_steps = 0: off. */
+  int forced_steps;
+  double field_eps;
+  double field_sigma;
 } lbc_instance;
 
 typedef struct lbc_config {
@@ -187,6 +198,16 @@ int lbc_get_R_history(lbc_ctx* ctx, int inst, double* R, int max_n,
  * intervals; at most max_n values are written. */
 int lbc_get_scgf(lbc_ctx* ctx, int inst, double dT, double* lam, int max_n,
                  int* n_out);
+
+/* Forcing mode: the random fields of the next interval of instance inst,
+ * instead of drawing them on the device: n_clones maps of Lx*Ly doubles in
+ * the reference's field layout map[y + Ly*x] (the map argument of
+ * src/lbm_src/streamCollCompute.cpp:50-93).  Used once. */
+int lbc_set_field_maps(lbc_ctx* ctx, int inst, const double* maps);
+
+/* The random fields the last interval of instance inst ran with, in the same
+ * layout. */
+int lbc_get_field_maps(lbc_ctx* ctx, int inst, double* maps);
 
 /* Wait for the context's stream. */
 int lbc_sync(lbc_ctx* ctx);

@@ -99,7 +99,11 @@ static void collide_node(double* f, double beta, double tau) {
   }
 }
 
-double orc_step(const orc_geom* g, double* fin, double* fout) {
+/* One step; map (REF only, may be NULL) gives the forcing mode's per-site
+ * body force beta(x,y) = (1.0 + eps*map[y + Dy*x])*beta
+ * (src/lbm_src/streamCollCompute.cpp:63, idx(x,y) = y + Dy*x). */
+static double step_impl(const orc_geom* g, double* fin, double* fout,
+                        const double* map, double eps) {
   const int Dx = g->Lx, Dy = g->Ly;
   const int periodic = (g->mode == ORC_MODE_REF);
   const double beta = periodic ? g->beta : 0.0;
@@ -107,7 +111,9 @@ double orc_step(const orc_geom* g, double* fin, double* fout) {
   for (int x = 0; x < Dx; x++) {
     for (int y = 0; y < Dy; y++) {
       double* f = fin + IDX(g, x, y, 0);
-      collide_node(f, beta, g->tau);
+      const double b =
+          (periodic && map) ? (1.0 + eps * map[y + Dy * x]) * beta : beta;
+      collide_node(f, b, g->tau);
       for (int k = 0; k < 9; k++) {
         int nx = x + orc_c[k][0], ny = y + orc_c[k][1];
         if (periodic) {
@@ -186,6 +192,15 @@ double orc_step(const orc_geom* g, double* fin, double* fout) {
 
   return (g->mode == ORC_MODE_REF) ? orc_force_stress(g, fout)
                                    : orc_force_mea(g, fout);
+}
+
+double orc_step(const orc_geom* g, double* fin, double* fout) {
+  return step_impl(g, fin, fout, NULL, 0.0);
+}
+
+double orc_step_map(const orc_geom* g, double* fin, double* fout,
+                    const double* map, double eps) {
+  return step_impl(g, fin, fout, map, eps);
 }
 
 /* Moments and non-equilibrium stress at one node, in the order of
@@ -283,6 +298,54 @@ double orc_run_steps(const orc_geom* g, double* state, double* scratch, int M,
   if (cur != state) memcpy(state, cur, (size_t)N * sizeof(double));
   acc *= dt;
   return acc;
+}
+
+/* Forcing-mode interval (src/mpi_LBM_GK_forcing.cpp:335-400): M1 steps
+ * with the per-site force of map, then M - M1 with the uniform one; the
+ * drag of every step is scaled, shifted and summed as in orc_run_steps. */
+double orc_run_steps_forced(const orc_geom* g, double* state, double* scratch,
+                            int M1, int M, const double* map, double eps,
+                            double scale, double shift, double dt,
+                            double* A_out) {
+  long N = 9L * g->Lx * g->Ly;
+  double* cur = state;
+  double* oth = scratch;
+  double acc = 0.0;
+  for (int n = 0; n < M; n++) {
+    double F = step_impl(g, cur, oth, n < M1 ? map : NULL, eps);
+    double* t = cur;
+    cur = oth;
+    oth = t;
+    F = F * scale;
+    F = F - shift;
+    if (A_out) A_out[n] = F;
+    acc += F;
+  }
+  if (cur != state) memcpy(state, cur, (size_t)N * sizeof(double));
+  acc *= dt;
+  return acc;
+}
+
+/* Forcing-mode field draw: complex standard normals Z_m = z[2m] + i z[2m+1]
+ * for m < n, the pair m = 2p, 2p+1 from one Philox block with counter
+ * (p, clone, interval, 0xF1E1D000 + instance) and key (seed_lo, seed_hi)
+ * through orc_box_muller4. */
+void orc_field_normals(uint32_t seed_lo, uint32_t seed_hi, uint32_t clone,
+                       uint32_t interval, uint32_t instance, int n, double* z) {
+  const uint32_t key[2] = {seed_lo, seed_hi};
+  for (int p = 0; 2 * p < n; p++) {
+    uint32_t ctr[4] = {(uint32_t)p, clone, interval, 0xF1E1D000u + instance};
+    uint32_t r[4];
+    double v[4];
+    orc_philox4x32_10(ctr, key, r);
+    orc_box_muller4(r, v);
+    z[4 * p] = v[0];
+    z[4 * p + 1] = v[1];
+    if (2 * p + 1 < n) {
+      z[4 * p + 2] = v[2];
+      z[4 * p + 3] = v[3];
+    }
+  }
 }
 
 double orc_run_clones(const orc_geom* g, double* states, double* scratch,

@@ -56,6 +56,26 @@ int orc_bb_mask(const orc_geom* g, int x, int y);
  * REF mode, momentum exchange in CHANNEL mode). */
 double orc_step(const orc_geom* g, double* fin, double* fout);
 
+/* The same step with the forcing mode's per-site body force
+ * beta(x,y) = (1.0 + eps*map[y + Ly*x])*beta (REF mode;
+ * src/lbm_src/streamCollCompute.cpp:50-93); map NULL = orc_step. */
+double orc_step_map(const orc_geom* g, double* fin, double* fout,
+                    const double* map, double eps);
+
+/* Forcing-mode interval (src/mpi_LBM_GK_forcing.cpp:335-400): M1 steps with
+ * map, then M - M1 uniform ones; outputs as orc_run_steps. */
+double orc_run_steps_forced(const orc_geom* g, double* state, double* scratch,
+                            int M1, int M, const double* map, double eps,
+                            double scale, double shift, double dt,
+                            double* A_out);
+
+/* Complex normals of one clone's random-field draw: z[2m] + i z[2m+1],
+ * m < n; pair p = (2p, 2p+1) from Philox counter
+ * (p, clone, interval, 0xF1E1D000 + instance), key (seed_lo, seed_hi),
+ * orc_box_muller4. */
+void orc_field_normals(uint32_t seed_lo, uint32_t seed_hi, uint32_t clone,
+                       uint32_t interval, uint32_t instance, int n, double* z);
+
 /* Drag functionals on a post-stream state. */
 double orc_force_stress(const orc_geom* g, const double* f);
 double orc_force_mea(const orc_geom* g, const double* f);

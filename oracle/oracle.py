@@ -10,6 +10,7 @@ Arrays are numpy FP64 in the reference's AoS layout [x][y][k]
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -115,6 +116,14 @@ def lib() -> C.CDLL:
         L.orc_bb_mask.restype = C.c_int
         L.orc_scgf.argtypes = [dp, C.c_int, C.c_double, dp]
         L.orc_scgf.restype = None
+        L.orc_step_map.argtypes = [gp, dp, dp, dp, C.c_double]
+        L.orc_step_map.restype = C.c_double
+        L.orc_run_steps_forced.argtypes = [gp, dp, dp, C.c_int, C.c_int, dp,
+                                           C.c_double, C.c_double, C.c_double,
+                                           C.c_double, dp]
+        L.orc_run_steps_forced.restype = C.c_double
+        L.orc_field_normals.argtypes = [C.c_uint32] * 5 + [C.c_int, dp]
+        L.orc_field_normals.restype = None
         L.orc_glibc_srand.argtypes = [C.c_uint]
         L.orc_glibc_rand_n.argtypes = [C.c_int, ip]
         L.orc_glibc_seed48_default.argtypes = []
@@ -134,6 +143,8 @@ def ref() -> C.CDLL:
         dp = C.POINTER(C.c_double)
         R.ref_set_geometry.argtypes = [C.c_int] * 6
         R.ref_step.argtypes = [dp, dp, dp, dp, dp, C.c_double, C.c_double]
+        R.ref_step_spatial.argtypes = [dp, dp, dp, dp, dp, C.c_double, dp,
+                                       C.c_double, C.c_double]
         R.ref_force.argtypes = [dp, C.c_double]
         R.ref_force.restype = C.c_double
         R.ref_perturb.argtypes = [dp, dp, C.c_double, C.c_int, C.c_int]
@@ -189,6 +200,82 @@ def run_clones(g: Geom, states: np.ndarray, M: int, scale=1.0, shift=0.0,
     lib().orc_run_clones(C.byref(g), _dp(states), _dp(scratch), n, M, scale,
                          shift, dt, _dp(acc))
     return acc
+
+
+def step_map(g: Geom, fin: np.ndarray, fout: np.ndarray, fmap: np.ndarray,
+             eps: float) -> float:
+    """One REF step with the per-site force (1 + eps*map)*beta; fmap has
+    shape (Lx, Ly) (the reference's idx(x,y) = y + Ly*x layout)."""
+    m = np.ascontiguousarray(fmap, np.float64)
+    return lib().orc_step_map(C.byref(g), _dp(fin), _dp(fout), _dp(m), eps)
+
+
+def run_steps_forced(g: Geom, state: np.ndarray, M1: int, M: int,
+                     fmap: np.ndarray, eps: float, scale=1.0, shift=0.0,
+                     dt=1.0):
+    """Forcing-mode interval in place: M1 forced steps, then M - M1 uniform
+    ones; returns (acc, A[M])."""
+    scratch = np.empty_like(state)
+    A = np.zeros(M)
+    m = np.ascontiguousarray(fmap, np.float64)
+    acc = lib().orc_run_steps_forced(C.byref(g), _dp(state), _dp(scratch), M1,
+                                     M, _dp(m), eps, scale, shift, dt, _dp(A))
+    return acc, A
+
+
+# --------------------------------------------------------------------------
+# random fields of the forcing mode
+# (src/perturbation_routines/generate_random_field.cpp:9-181)
+
+def field_embedding(Lx: int, Ly: int, sigma: float = 32.0) -> np.ndarray:
+    """Wrapped covariance of the circulant embedding, shape (2Lx-1, 2Ly-1):
+    G[r, j] = f(min(r, fmx-r), min(j, fmy-j)) / (fmx*fmy) with
+    f(x, y) = exp(-(x*x + y*y) / (2 sigma^2)) (covariance_fctn.cpp:3-10;
+    generate_random_field.cpp:44-106 for Lx = Ly).  The exponentials come
+    from math.exp (the C library's exp), one per distinct squared lag."""
+    fmx, fmy = 2 * Lx - 1, 2 * Ly - 1
+    r = np.arange(fmx)
+    j = np.arange(fmy)
+    dx = np.minimum(r, fmx - r)
+    dy = np.minimum(j, fmy - j)
+    d = dx[:, None] ** 2 + dy[None, :] ** 2
+    uniq, inv = np.unique(d, return_inverse=True)
+    vals = np.array([math.exp(-float(v) / (2 * sigma * sigma)) for v in uniq])
+    return vals[inv].reshape(d.shape) / float(fmx * fmy)
+
+
+def field_eigenvalues(Lx: int, Ly: int, sigma: float = 32.0) -> np.ndarray:
+    """Eigenvalues of the embedding, negative ones set to zero
+    (generate_random_field.cpp:127-158; its abs() is the int one, so every
+    negative eigenvalue above -1 is zeroed)."""
+    lam = np.fft.fft2(field_embedding(Lx, Ly, sigma)).real
+    if (lam <= -1.0).any():
+        raise ValueError("embedding eigenvalue <= -1")
+    return np.where(lam < 0, 0.0, lam)
+
+
+def field_normals(seed: int, clone: int, interval: int, instance: int,
+                  shape) -> np.ndarray:
+    """The complex normals Z of one clone's draw, shape (fmx, fmy)."""
+    n = int(shape[0] * shape[1])
+    z = np.zeros(2 * n)
+    lib().orc_field_normals(seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF,
+                            clone, interval, instance, n, _dp(z))
+    return (z[0::2] + 1j * z[1::2]).reshape(shape)
+
+
+def field_from_normals(lam: np.ndarray, Z: np.ndarray, Lx: int,
+                       Ly: int) -> np.ndarray:
+    """Field on the lattice, shape (Lx, Ly): Re of the forward 2-D DFT of
+    sqrt(lam)*Z on the block x < Lx, y < Ly
+    (generate_random_field.cpp:158-178)."""
+    return np.fft.fft2(np.sqrt(lam) * Z).real[:Lx, :Ly].copy()
+
+
+def field_covariance(lam: np.ndarray, Lx: int, Ly: int) -> np.ndarray:
+    """Covariance of the field at lag (dx, dy), dx < Lx, dy < Ly, implied by
+    the (clipped) eigenvalues: sum_k lam_k cos(2 pi k.d / M)."""
+    return np.fft.fft2(lam).real[:Lx, :Ly].copy()
 
 
 def scgf(logR: np.ndarray, dT: float) -> np.ndarray:
@@ -301,6 +388,22 @@ def ref_step(g: Geom, fin: np.ndarray, fout: np.ndarray) -> float:
     ux = np.zeros(S)
     uy = np.zeros(S)
     R.ref_step(_dp(fin), _dp(fout), _dp(rho), _dp(ux), _dp(uy), g.beta, g.tau)
+    return R.ref_force(_dp(fout), 1.0 / g.tau)
+
+
+def ref_step_spatial(g: Geom, fin: np.ndarray, fout: np.ndarray,
+                     fmap: np.ndarray, eps: float) -> float:
+    """One reference forced step (src/mpi_LBM_GK_forcing.cpp:337-351) with
+    map fmap of shape (Lx, Ly); returns the raw force on fout."""
+    R = ref()
+    R.ref_set_geometry(g.Lx, g.Ly, g.xmin, g.xmax, g.ymin, g.ymax)
+    S = g.Lx * g.Ly
+    rho = np.zeros(S)
+    ux = np.zeros(S)
+    uy = np.zeros(S)
+    m = np.ascontiguousarray(fmap, np.float64).copy()
+    R.ref_step_spatial(_dp(fin), _dp(fout), _dp(rho), _dp(ux), _dp(uy), g.beta,
+                       _dp(m), g.tau, eps)
     return R.ref_force(_dp(fout), 1.0 / g.tau)
 
 

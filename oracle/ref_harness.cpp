@@ -13,6 +13,11 @@
 void streamingAndCollisionComputeMacroBodyForce(double*, double*, double*,
                                                 double*, double*, double,
                                                 double);
+void streamingAndCollisionComputeMacroBodyForceSpatial(double*, double*,
+                                                       double*, double*,
+                                                       double*, double,
+                                                       double*, double,
+                                                       double);
 void computeDomainNoSlipWalls_BB(double*, double*);
 void computeSquareBounceBack_TEST(double*, double*);
 double computeForceOnSquare(double*, double);
@@ -40,6 +45,20 @@ void ref_set_geometry(int dx, int dy, int x0, int x1, int y0, int y1) {
 void ref_step(double* fin, double* fout, double* rho, double* ux, double* uy,
               double beta, double tau) {
   streamingAndCollisionComputeMacroBodyForce(fin, fout, rho, ux, uy, beta, tau);
+  computeDomainNoSlipWalls_BB(fout, fin);
+  computeSquareBounceBack_TEST(fout, fin);
+  for (int x = xmin + 1; x < xmax; x++)
+    for (int y = ymin + 1; y < ymax; y++)
+      for (int k = 0; k < 9; k++) fout[k + 9 * (y + Dy * x)] = w[k];
+}
+
+// One forced step as called at src/mpi_LBM_GK_forcing.cpp:337-351: the
+// spatial-force stream-collide, walls, square, interior reset.
+void ref_step_spatial(double* fin, double* fout, double* rho, double* ux,
+                      double* uy, double beta0, double* map, double tau,
+                      double eps) {
+  streamingAndCollisionComputeMacroBodyForceSpatial(fin, fout, rho, ux, uy,
+                                                    beta0, map, tau, eps);
   computeDomainNoSlipWalls_BB(fout, fin);
   computeSquareBounceBack_TEST(fout, fin);
   for (int x = xmin + 1; x < xmax; x++)

@@ -25,8 +25,17 @@ Fixtures (numpy .npz, FP64 unless noted):
       oracle/_ref/run_bodyforce) for square side 4 from rest, 100 steps:
       the input file, parameters.datout, data_force.datout, ux_t.datout,
       pops.datout and the VTK frames fluid_t<i>.vtk (text).
+  ref_forced_L4.npz
+      a forcing-mode interval (src/mpi_LBM_GK_forcing.cpp:335-395) on the
+      square-side-4 lattice: seeded state S0 and random map (the reference's
+      map[idx(x,y)] as an (Lx, Ly) array), M1 steps of
+      streamingAndCollisionComputeMacroBodyForceSpatial
+      (src/lbm_src/streamCollCompute.cpp:50-93) then M - M1 uniform steps,
+      each with walls, square and interior reset; the post-collision array of
+      the first step, the final state and the raw force after every step.
 
-Run from the repository root: python tests/golden/make_golden.py
+Run from the repository root: python tests/golden/make_golden.py [name ...]
+(names: steps perturb glibc bodyforce forced; default all).
 """
 import os
 import shutil
@@ -121,15 +130,41 @@ def bodyforce_fixture():
         vtk_names=np.array(vtk), vtk=np.array([text(n) for n in vtk]))
 
 
+def forced_fixture(L=4, M1=5, M=9, eps=0.4, seed=21):
+    g = O.ref_box_geom(L, 0.53, 0.05)
+    rng = np.random.default_rng(seed)
+    s0 = seeded_state(g, rng)
+    fmap = rng.standard_normal((g.Lx, g.Ly))
+    a, b = s0.copy(), np.zeros_like(s0)
+    forces, post0 = [], None
+    for n in range(M):
+        F = (O.ref_step_spatial(g, a, b, fmap, eps) if n < M1
+             else O.ref_step(g, a, b))
+        if n == 0:
+            post0 = a.copy()
+        a, b = b, a
+        forces.append(F)
+    np.savez_compressed(
+        os.path.join(OUT, f"ref_forced_L{L}.npz"), L=L, tau=g.tau,
+        geom=np.array([g.Lx, g.Ly, g.xmin, g.xmax, g.ymin, g.ymax]),
+        beta=g.beta, S0=s0, map=fmap, eps=eps, M1=M1, M=M, post0=post0,
+        final=a, forces=np.array(forces))
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libref.so missing: run `make -C oracle ref` "
                  "where /root/reference exists")
-    steps_fixture(4, 12, 4, 0.53, 0.05)
-    steps_fixture(8, 6, 8, 0.515, 0.05)
-    perturb_fixture()
-    glibc_fixture()
-    bodyforce_fixture()
+    jobs = {
+        "steps": lambda: (steps_fixture(4, 12, 4, 0.53, 0.05),
+                          steps_fixture(8, 6, 8, 0.515, 0.05)),
+        "perturb": perturb_fixture,
+        "glibc": glibc_fixture,
+        "bodyforce": bodyforce_fixture,
+        "forced": forced_fixture,
+    }
+    for name in sys.argv[1:] or list(jobs):
+        jobs[name]()
     print("fixtures written to", OUT)
 
 

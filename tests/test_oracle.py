@@ -1,6 +1,7 @@
 """Pins the CPU oracle: against golden fixtures written by the reference's own
 compiled kernels, against the live reference build where it exists, and
 against closed forms and invariants that a plausible mistake would break."""
+import math
 import os
 
 import numpy as np
@@ -359,3 +360,195 @@ def test_scgf_estimator_closed_forms(O):
     # (n-2)/((n-1) dT)
     np.testing.assert_allclose(O.scgf(logR, dT)[-1],
                                logR[1:n - 1].sum() / ((n - 1) * dT), rtol=1e-12)
+
+
+# ---------------------------------------------------------------- forcing mode
+def test_oracle_forced_interval_matches_reference_fixture(O):
+    fx = np.load(os.path.join(GOLDEN, "ref_forced_L4.npz"))
+    g = _geom_from(O, fx)
+    M1, M, eps = int(fx["M1"]), int(fx["M"]), float(fx["eps"])
+    a = fx["S0"].copy()
+    b = np.zeros_like(a)
+    for n in range(M):
+        F = O.step_map(g, a, b, fx["map"], eps) if n < M1 else O.step(g, a, b)
+        if n == 0:
+            assert np.array_equal(a, fx["post0"])
+        assert F == fx["forces"][n], f"force step {n}"
+        a, b = b, a
+    assert np.array_equal(a, fx["final"])
+    s = fx["S0"].copy()
+    acc, A = O.run_steps_forced(g, s, M1, M, fx["map"], eps, 1.0, 0.0, 1.0)
+    assert np.array_equal(s, fx["final"])
+    assert np.array_equal(A, fx["forces"])
+    tot = 0.0
+    for v in fx["forces"]:
+        tot += v
+    assert acc == tot
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(
+    os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+    "_ref", "libref.so")), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("L,eps,seed", [(4, 0.9, 5), (6, 0.05, 6)])
+def test_oracle_spatial_step_matches_live_reference(O, L, eps, seed):
+    g = O.ref_box_geom(L, 0.56, 0.07)
+    rng = np.random.default_rng(seed)
+    f = np.empty((g.Lx, g.Ly, 9))
+    for x in range(g.Lx):
+        for y in range(g.Ly):
+            f[x, y] = O.feq(1 + 1e-2 * rng.standard_normal(),
+                            0.07 + 1e-2 * rng.standard_normal(),
+                            1e-2 * rng.standard_normal())
+    fmap = rng.standard_normal((g.Lx, g.Ly))
+    a, b = f.copy(), np.zeros_like(f)
+    ra, rb = f.copy(), np.zeros_like(f)
+    for n in range(4):
+        F = O.step_map(g, a, b, fmap, eps)
+        Fr = O.ref_step_spatial(g, ra, rb, fmap, eps)
+        assert np.array_equal(a, ra) and np.array_equal(b, rb), f"step {n}"
+        assert F == Fr
+        a, b = b, a
+        ra, rb = rb, ra
+
+
+def test_spatial_force_reduces_to_uniform(O):
+    g = O.ref_box_geom(4, 0.6, 0.06)
+    rng = np.random.default_rng(3)
+    f = np.empty((g.Lx, g.Ly, 9))
+    for x in range(g.Lx):
+        for y in range(g.Ly):
+            f[x, y] = O.feq(1 + 1e-2 * rng.standard_normal(),
+                            0.06 + 1e-2 * rng.standard_normal(),
+                            1e-2 * rng.standard_normal())
+    # a zero map, or eps = 0, is the uniform step bit for bit
+    for fmap, eps in [(np.zeros((g.Lx, g.Ly)), 0.8),
+                      (rng.standard_normal((g.Lx, g.Ly)), 0.0)]:
+        a, b = f.copy(), np.zeros_like(f)
+        c, d = f.copy(), np.zeros_like(f)
+        assert O.step_map(g, a, b, fmap, eps) == O.step(g, c, d)
+        assert np.array_equal(a, c) and np.array_equal(b, d)
+    # a constant map c is the uniform step with beta' = (1 + eps*c)*beta
+    cval, eps = 0.37, 0.5
+    a, b = f.copy(), np.zeros_like(f)
+    O.step_map(g, a, b, np.full((g.Lx, g.Ly), cval), eps)
+    g2 = O.ref_box_geom(4, 0.6, 0.06)
+    g2.beta = (1.0 + eps * cval) * g.beta
+    c, d = f.copy(), np.zeros_like(f)
+    O.step(g2, c, d)
+    assert np.array_equal(a, c) and np.array_equal(b, d)
+    # the map is read as map[y + Ly*x]: a force on one site changes only the
+    # post-collision populations of that site
+    fmap = np.zeros((g.Lx, g.Ly))
+    fmap[3, 11] = 1.0
+    a, b = f.copy(), np.zeros_like(f)
+    c, d = f.copy(), np.zeros_like(f)
+    O.step_map(g, a, b, fmap, 0.5)
+    O.step(g, c, d)
+    diff = np.argwhere((a != c).any(axis=2))
+    assert diff.tolist() == [[3, 11]]
+
+
+def _reference_embedding_loops(n, sigma):
+    """Literal restatement of generate_random_field.cpp:44-111 (Rows/Cols,
+    row/col padding, G = row/N) with covariance_fctn.cpp:3-10, for small n."""
+    M = 2 * n - 1
+    N = M * M
+    f = lambda x, y: math.exp(-float(x * x + y * y) / (2 * sigma * sigma))
+    Rows = [[f(R, y) for y in range(n)] for R in range(n)]
+    Cols = [[f(-R, y) for y in range(n)] for R in range(n)]
+    row = [[0.0] * M for _ in range(M)]
+    col = [[0.0] * M for _ in range(n)]
+    for R in range(n):
+        for j in range(n):
+            row[R][j] = Rows[R][j]
+            col[R][j] = Cols[R][j]
+        ii = 0
+        for j in range(n, M):
+            ii += 1
+            row[R][j] = Cols[R][n - ii]
+            col[R][j] = Rows[R][n - ii]
+    ii = 0
+    for R in range(n, M):
+        ii += 1
+        for j in range(M):
+            row[R][j] = col[n - ii][j]
+    return np.array([[row[R][j] / N for j in range(M)] for R in range(M)])
+
+
+@pytest.mark.parametrize("n,sigma", [(3, 32.0), (5, 32.0), (8, 1.7), (6, 3.0)])
+def test_field_embedding_matches_reference_construction(O, n, sigma):
+    assert np.array_equal(O.field_embedding(n, n, sigma),
+                          _reference_embedding_loops(n, sigma))
+
+
+@pytest.mark.parametrize("Lx,Ly,sigma", [(241, 241, 32.0), (60, 40, 3.0),
+                                         (33, 70, 2.0)])
+def test_field_covariance_is_the_target_without_clipping(O, Lx, Ly, sigma):
+    G = O.field_embedding(Lx, Ly, sigma)
+    raw = np.fft.fft2(G).real
+    assert raw.min() > -1e-15  # nothing clipped beyond rounding
+    lam = O.field_eigenvalues(Lx, Ly, sigma)
+    cov = O.field_covariance(lam, Lx, Ly)
+    dx = np.arange(Lx)[:, None]
+    dy = np.arange(Ly)[None, :]
+    target = np.exp(-(dx * dx + dy * dy) / (2 * sigma * sigma))
+    assert np.abs(cov - target).max() < 1e-12
+
+
+def test_field_eigenvalue_clipping_is_the_int_abs_of_the_reference(O):
+    # generate_random_field.cpp:147 tests abs(a) > 1e-15 on a double with
+    # <cmath>, <cstdlib> and "complex.h" included: that abs is the int one,
+    # so a negative eigenvalue above -1 is zeroed, not flagged
+    import shutil
+    import subprocess
+    import tempfile
+
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = ('#include <cmath>\n#include <cstdlib>\n#include "complex.h"\n'
+           '#include <cstdio>\nint main(){double a=-0.5;'
+           'std::printf("%d %d\\n",(int)(abs(a)>1e-15),(int)(abs(-1.5)>1e-15));}\n')
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "p.cpp"), "w") as fh:
+            fh.write(src)
+        subprocess.run(["g++", "-w", "-o", os.path.join(d, "p"),
+                        os.path.join(d, "p.cpp")], check=True)
+        out = subprocess.run([os.path.join(d, "p")], capture_output=True,
+                             text=True, check=True).stdout.split()
+    assert out == ["0", "1"]
+    # small lattices at sigma 32 have negative eigenvalues; the oracle zeroes
+    # them and the field's variance then exceeds one
+    lam_raw = np.fft.fft2(O.field_embedding(17, 17, 32.0)).real
+    assert lam_raw.min() < -1e-3
+    lam = O.field_eigenvalues(17, 17, 32.0)
+    assert np.array_equal(lam, np.where(lam_raw < 0, 0.0, lam_raw))
+    assert O.field_covariance(lam, 17, 17)[0, 0] > 1.0
+
+
+def test_field_normals_follow_the_philox_layout(O):
+    Z = O.field_normals(0x1234_5678_9ABC, clone=5, interval=3, instance=2,
+                        shape=(3, 5))
+    flat = Z.ravel()
+    for p in range(8):
+        r = O.philox([p, 5, 3, 0xF1E1D000 + 2], [0x56789ABC, 0x1234])
+        v = O.box_muller4(r)
+        assert flat[2 * p] == complex(v[0], v[1])
+        if 2 * p + 1 < flat.size:
+            assert flat[2 * p + 1] == complex(v[2], v[3])
+
+
+def test_field_sample_statistics(O):
+    Lx, Ly, sigma = 24, 16, 2.5
+    lam = O.field_eigenvalues(Lx, Ly, sigma)
+    cov = O.field_covariance(lam, Lx, Ly)
+    rng = np.random.default_rng(11)
+    fields = np.stack([
+        O.field_from_normals(lam, rng.standard_normal(lam.shape)
+                             + 1j * rng.standard_normal(lam.shape), Lx, Ly)
+        for _ in range(600)])
+    assert abs(fields.mean()) < 0.05
+    for dx, dy in [(0, 0), (1, 0), (0, 2), (3, 3), (6, 0)]:
+        a = fields[:, : Lx - dx, : Ly - dy]
+        b = fields[:, dx:, dy:]
+        est = (a * b).mean()
+        assert abs(est - cov[dx, dy]) < 0.06, (dx, dy, est, cov[dx, dy])
