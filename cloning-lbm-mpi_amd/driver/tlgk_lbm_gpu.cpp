@@ -1,11 +1,14 @@
 // tlgk_lbm_gpu -- cloning-sampler driver on one MI355X.
 //
-// Drop-in for the reference's MPI driver TLGK_LBM_POPS_PERTURB
-// (src/mpi_LBM_GK_pops_perturb.cpp): same positional input file, same
-// initial-state files, same outputs (output_TLGK.out, labels.dat,
-// rvalues.dat, clone_<j>/data_force.datout), with every clone on the GPU
-// behind the C ABI of include/lbc.h instead of one MPI rank per block of
-// clones.
+// Drop-in for the reference's MPI drivers TLGK_LBM_POPS_PERTURB
+// (src/mpi_LBM_GK_pops_perturb.cpp) and, with `order = forcing`,
+// TLGK_LBM_FORCING (src/mpi_LBM_GK_forcing.cpp): same positional input
+// file, same initial-state files, same outputs (output_TLGK.out,
+// rvalues.dat and labels.dat, clone_<j>/data_force.datout; in forcing order
+// weights_evolution_<t>, copies_evolution_<t> and
+// clone_<j>/evolution_<t>_clone_<j> instead of data_force.datout), with
+// every clone on the GPU behind the C ABI of include/lbc.h instead of one
+// MPI rank per block of clones.
 //
 //   tlgk_lbm_gpu <inputfile>
 //
@@ -15,7 +18,16 @@
 // optionally followed by `key = value` lines:
 //     order       pops (default) | forcing   positional order; `forcing` reads
 //                 input_TEMPLATE's order Nc T dT dT0 eps L tau U0 alpha path
-//                 master (src/mpi_LBM_GK_forcing.cpp:62-80)
+//                 master (src/mpi_LBM_GK_forcing.cpp:62-80) and runs the
+//                 forcing driver: each interval's first floor(dT0*T0) steps
+//                 with the body force (1 + eps*map)*beta0, map a Gaussian
+//                 random field drawn per clone and interval on the device,
+//                 then floor((dT-dT0)*T0) uniform steps; drag scaled by 1/F0
+//                 with no shift, no snapshot perturbation (:137-144,333-400)
+//     field_sigma forcing: covariance length of the fields (0 = 32)
+//     dump_fields forcing: 1 writes each interval's fields to
+//                 fields_<t>.datout in the master folder (Nc maps of Dx*Dy
+//                 doubles, map[y + Dy*x])
 //     mode        ref (default) | channel
 //     storage     f64 (default) | f32 | f16 | bf16
 //     rng         glibc (default in ref mode: the reference's srand/rand and
@@ -38,7 +50,10 @@
 // In ref mode with rng = glibc and ranks = 1 the run consumes rand() and
 // drand48() exactly as the reference's rank 0 does, and the weights are
 // exp(alpha * s_) evaluated on the host with glibc, so the selections and
-// copy tables are the reference's.
+// copy tables are the reference's.  In forcing order the reference also
+// draws its fields from rand() (randNormal.cpp), which this driver does not
+// (the fields come from Philox on the device), so only drand48's floor-step
+// stream is shared with the reference there.
 #include <sys/stat.h>
 
 #include <array>
@@ -206,6 +221,8 @@ int main(int argc, char** argv) {
   std::cout << "All processes sucessfully read input file " << argv[1] << std::endl;
 
   const bool channel = kv(in, "mode", "ref") == "channel";
+  const bool forcing = !channel && kv(in, "order", "pops") == "forcing";
+  const bool dump_fields = forcing && kvi(in, "dump_fields", 0) != 0;
   const std::string st = kv(in, "storage", "f64");
   const int storage = st == "f32" ? LBC_STORE_F32
                       : st == "f16" ? LBC_STORE_F16
@@ -244,6 +261,19 @@ int main(int argc, char** argv) {
     inst.steps = (int)std::floor(in.dT * T0);
     inst.eps = in.eps;
     inst.n_snapshots = kNN;
+    if (forcing) {
+      // two phases of lbmTimeSteps1 and lbmTimeSteps2 steps, F*oneOvF0
+      // (src/mpi_LBM_GK_forcing.cpp:137-144,358,389)
+      const int M1 = (int)std::floor(in.dT0 * T0);
+      const int M2 = (int)std::floor((in.dT - in.dT0) * T0);
+      inst.steps = M1 + M2;
+      inst.forced_steps = M1;
+      inst.field_eps = in.eps;
+      inst.field_sigma = kvd(in, "field_sigma", 0.0);
+      inst.force_shift = 0.0;
+      inst.eps = 0.0;
+      inst.n_snapshots = 0;
+    }
     log << "READ input file" << std::endl << "---------------" << std::endl;
     log << "  Nc = " << Nc << std::endl;
     log << "  T = " << in.T << " dT = " << in.dT << std::endl;
@@ -364,7 +394,7 @@ int main(int argc, char** argv) {
       }
       check(lbc_load_states(ctx, 0, j, 1, buf.data()), ctx, "lbc_load_states");
     }
-    if (!channel) {
+    if (!channel && !forcing) {
       if ((int)names.size() < kNN) {
         log << "ERROR WITH READING OF POPULATIONS FOR PERTURB IN MASTER" << std::endl;
         return 1;
@@ -399,6 +429,7 @@ int main(int argc, char** argv) {
   std::vector<int32_t> rint(Nc), nb0(Nc), nb1(Nc), pairs(2 * Nc), labels(Nc);
   std::vector<uint8_t> marks(Nc);
   std::vector<double> pref((size_t)Nc * kNN);
+  std::vector<double> fields(dump_fields ? (size_t)Nc * Nsite : 0);
   const bool fast = channel && !glibc_rng && !force_io;
 
   if (fast) {
@@ -415,7 +446,7 @@ int main(int argc, char** argv) {
     // perturbation prefactors: each rank's marked clones in order
     // (generatePerturbedState.cpp:15 via :412-416)
     const double* pf = nullptr;
-    if (!channel && glibc_rng) {
+    if (!channel && !forcing && glibc_rng) {
       check(lbc_get_marks(ctx, 0, marks.data()), ctx, "lbc_get_marks");
       for (int j = 0; j < Nc; j++) {
         if (!marks[j]) continue;
@@ -428,10 +459,22 @@ int main(int argc, char** argv) {
     if (force_io) {
       check(lbc_get_drag(ctx, 0, A.data()), ctx, "lbc_get_drag");
       for (int j = 0; j < Nc; j++) {
-        std::ofstream o(folder[j] + "data_force.datout",
-                        std::ios::binary | std::ios::app);
+        // forcing: one file per clone and interval (src/mpi_LBM_GK_forcing.cpp:325-330)
+        std::ofstream o = forcing
+            ? std::ofstream(folder[j] + "evolution_" + std::to_string(t) +
+                                "_clone_" + std::to_string(j),
+                            std::ios::binary)
+            : std::ofstream(folder[j] + "data_force.datout",
+                            std::ios::binary | std::ios::app);
         o.write(reinterpret_cast<const char*>(&A[(size_t)j * M]), M * sizeof(double));
       }
+    }
+    if (dump_fields) {
+      check(lbc_get_field_maps(ctx, 0, fields.data()), ctx, "lbc_get_field_maps");
+      std::ofstream o(in.master + "fields_" + std::to_string(t) + ".datout",
+                      std::ios::binary);
+      o.write(reinterpret_cast<const char*>(fields.data()),
+              fields.size() * sizeof(double));
     }
     check(lbc_get_logweights(ctx, 0, obs.data(), logw.data()), ctx,
           "lbc_get_logweights");
@@ -450,7 +493,7 @@ int main(int argc, char** argv) {
     }
     for (int i = 1; i < (int)insts.size(); i++)
       check(lbc_select(ctx, i, nullptr, nullptr, nullptr, nullptr), ctx, "lbc_select");
-    if (in.kv.count("order") && in.kv.at("order") == "forcing") {
+    if (forcing) {
       // weights_evolution_t / copies_evolution_t (src/mpi_LBM_GK_forcing.cpp:302-316,448,461,510)
       std::ofstream wf(in.master + "weights_evolution_" + std::to_string(t), std::ios::binary);
       wf.write(reinterpret_cast<const char*>(s.data()), Nc * sizeof(double));

@@ -191,3 +191,79 @@ def test_warmup_snapshot_set_feeds_the_cloning_driver(tmp_path):
     assert r.returncode == 0, r.stderr + r.stdout
     assert np.fromfile(tmp_path / "clones" / "labels.dat",
                        dtype=np.int32).size == 4 * 2
+
+
+def test_forcing_driver_matches_reference_forcing_loop(tmp_path, O):
+    """`order = forcing`: the forcing driver's loop
+    (src/mpi_LBM_GK_forcing.cpp:297-601) restated with the oracle on the
+    fields the driver dumps: per-interval force files, weights and copies
+    files, labels and R values.  The fields themselves match the oracle's
+    generator on the same Philox normals."""
+    L, Nc, T, dT, dT0, eps, tau, U0, alpha = 4, 6, 1.0, 0.25, 0.1, 0.5, 0.53, \
+        0.05, 20.0
+    g, data, names, _ = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
+                                   alpha)
+    inp = tmp_path / "input_forcing"
+    inp.write_text(f"{Nc}\n{T}\n{dT}\n{dT0}\n{eps}\n{L}\n{tau}\n{U0}\n{alpha}\n"
+                   f"{data}/\n{tmp_path}/frun/\n"
+                   "order = forcing\ndump_fields = 1\n")
+    print(f"case: forcing driver L={L} Nc={Nc}", flush=True)
+    r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    run = tmp_path / "frun"
+    T0 = L / U0
+    M1 = math.floor(dT0 * T0)
+    M = M1 + math.floor((dT - dT0) * T0)
+    nT = math.floor(T / dT)
+    F0 = U0 * U0 * L * 0.5
+    fields = [np.fromfile(run / f"fields_{t}.datout").reshape(Nc, g.Lx, g.Ly)
+              for t in range(nT)]
+    lam = O.field_eigenvalues(g.Lx, g.Ly, 32.0)
+    for t in (0, nT - 1):
+        for j in range(Nc):
+            want = O.field_from_normals(
+                lam, O.field_normals(0, j, t, 0, lam.shape), g.Lx, g.Ly)
+            assert np.abs(fields[t][j] - want).max() < 1e-5, (t, j)
+    # the reference loop on the oracle with those fields: rank 0's rand()
+    # after srand(1) and 1000 discards, drand48 from its default state
+    states = [np.fromfile(data / names[j]).reshape(g.Lx, g.Ly, 9)
+              for j in range(Nc)]
+    lib = O.lib()
+    lib.orc_glibc_srand(1)
+    tmpi = np.zeros(1000, np.int32)
+    lib.orc_glibc_rand_n(1000, O._ip(tmpi))
+    lib.orc_glibc_seed48_default()
+    rq = deque()
+    labels_ref, R_ref = [], []
+    for t in range(nT):
+        acc = np.zeros(Nc)
+        for j in range(Nc):
+            acc[j], A = O.run_steps_forced(g, states[j], M1, M, fields[t][j],
+                                           eps, 1.0 / F0, 0.0, 1.0 / T0)
+            got = np.fromfile(run / f"clone_{j}" / f"evolution_{t}_clone_{j}")
+            assert np.array_equal(got, A), (t, j)
+        s = np.array([math.exp(alpha * a) for a in acc])
+        assert np.array_equal(np.fromfile(run / f"weights_evolution_{t}"), s)
+        u = np.zeros(Nc)
+        lib.orc_glibc_drand48_n(Nc, O._dp(u))
+        while len(rq) < Nc:
+            one = np.zeros(1, np.int32)
+            lib.orc_glibc_rand_n(1, O._ip(one))
+            rq.append(int(one[0]))
+        sel = O.select(s, u, np.array(list(rq)[:Nc], np.int32))
+        for _ in range(sel["n_rint"]):
+            rq.popleft()
+        copies = np.fromfile(run / f"copies_evolution_{t}", dtype=np.int32)
+        assert np.array_equal(copies, np.concatenate([sel["nb_before"],
+                                                      sel["nb_after"]]))
+        labels_ref.append(sel["labels"])
+        R_ref.append(sel["total_R"])
+        for src, dst in sel["pairs"]:
+            states[dst] = states[src].copy()
+    labels = np.fromfile(run / "labels.dat", dtype=np.int32).reshape(-1, Nc)
+    assert np.array_equal(labels, np.array(labels_ref))
+    np.testing.assert_allclose(np.fromfile(run / "rvalues.dat"), R_ref,
+                               rtol=1e-12, atol=0)
+    # the run did clone and prune
+    assert any(not np.array_equal(lb, np.arange(Nc)) for lb in labels_ref)
