@@ -29,6 +29,8 @@ constexpr int kMaxIntervals = 1 << 16;
 // forcing mode: largest embedding torus, and the FFT work buffer's budget
 constexpr long long kMaxFieldEntries = 1LL << 26;
 constexpr size_t kFieldWorkBytes = size_t(1) << 30;
+// macroscopic output buffer limit (64 GiB of FP64)
+constexpr long long kMaxMacroDoubles = 1LL << 33;
 
 size_t elem_bytes(int storage) {
   switch (storage) {
@@ -143,6 +145,11 @@ int validate(const lbc_config* cfg, std::string& why) {
       return bad("non-finite parameter");
     if (s.forced_steps < 0 || s.forced_steps > s.steps)
       return bad("forced_steps must be in [0, steps]");
+    if (s.macro_every < 0) return bad("macro_every must be >= 0");
+    if (s.macro_every > 0 &&
+        (long long)s.n_clones * ((s.steps + s.macro_every - 1) / s.macro_every) *
+                3 * s.Lx * s.Ly > kMaxMacroDoubles)
+      return bad("macroscopic output too large: raise macro_every");
     if (s.forced_steps > 0) {
       if (cfg->mode != LBC_MODE_REF)
         return bad("forced steps need LBC_MODE_REF (the body-force mode)");
@@ -328,9 +335,11 @@ int check_inst(lbc_ctx* c, int inst) {
 // pass per sweep).
 int interval_flip(const lbc_ctx* c, int i) {
   const int passes = c->inst[i].steps + 1;
-  return c->passes > 1 ? lbc::fused_sweeps(c->inst[i].steps, c->passes,
-                                          c->inst[i].forced_steps) & 1
-                       : passes & 1;
+  return c->passes > 1
+             ? lbc::fused_sweeps(c->inst[i].steps, c->passes,
+                                 c->inst[i].forced_steps,
+                                 c->inst[i].macro_every) & 1
+             : passes & 1;
 }
 
 int upload_flips(lbc_ctx* c) {
@@ -381,6 +390,7 @@ int autotune(lbc_ctx* c) {
   for (auto& d : tmp) {
     d.steps = std::min(d.steps, 16);
     d.forced_steps = std::min(d.forced_steps, d.steps);
+    d.macro_every = 0;
   }
   HIPC(c, hipMemcpyAsync(c->idd, tmp.data(), sizeof(InstDev) * tmp.size(),
                          hipMemcpyHostToDevice, c->stream));
@@ -593,6 +603,14 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     if (s.forced_steps > 0) {
       c->forced = true;
       if (int rc2 = field_setup(c, i)) return bail(rc2);
+    }
+    if (s.macro_every > 0) {
+      d.macro_every = s.macro_every;
+      d.macro_frames = (s.steps + s.macro_every - 1) / s.macro_every;
+      const size_t mb = sizeof(double) * 3 * (size_t)d.plane * d.macro_frames * n;
+      if (dalloc(c, &d.macro, mb)) return bail(LBC_E_NOMEM);
+      if (hipMemsetAsync(d.macro, 0, mb, c->stream) != hipSuccess)
+        return bail(fail(c, LBC_E_HIP, "hipMemsetAsync failed"));
     }
   }
   if (dalloc(c, &c->idd, sizeof(InstDev) * ni) ||
@@ -995,6 +1013,20 @@ int lbc_get_field_maps(lbc_ctx* c, int inst, double* maps) {
   return LBC_OK;
 }
 
+int lbc_get_macro(lbc_ctx* c, int inst, int clone0, int n, double* out) {
+  if (int rc = check_inst(c, inst)) return rc;
+  const InstDev& d = c->idh[inst];
+  if (d.macro_every <= 0)
+    return fail(c, LBC_E_STATE, "instance has no macroscopic output (macro_every = 0)");
+  if (!out || n < 1 || clone0 < 0 || clone0 + n > d.n_clones)
+    return fail(c, LBC_E_ARG, "clone range out of bounds");
+  const size_t per = 3 * (size_t)d.plane * d.macro_frames;
+  HIPC(c, hipMemcpyAsync(out, d.macro + per * clone0, sizeof(double) * per * n,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LBC_OK;
+}
+
 int lbc_sync(lbc_ctx* c) {
   if (!c) return fail(nullptr, LBC_E_ARG, "null context");
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -1036,6 +1068,7 @@ int lbc_step_host(int mode, const lbc_instance* geom, double* fin, double* fout,
   s.rank_blocks = 1;
   s.n_snapshots = 0;
   s.forced_steps = 0;
+  s.macro_every = 0;
   lbc_config cfg{};
   cfg.mode = mode;
   cfg.storage = LBC_STORE_F64;

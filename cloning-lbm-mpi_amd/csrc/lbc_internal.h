@@ -49,6 +49,9 @@ struct InstDev {
   double* field_map;       // [n_clones][plane] fields, site order y*Lx + x
   double* field_sqrt_ev;   // [fmx*fmy] square roots of the eigenvalues
   double* field_work;      // [chunk][fmx*fmy] complex (re, im), FFT in place
+  // macroscopic output: rho, ux, uy at steps 0, E, 2E, ... (E = 0: off)
+  int macro_every, macro_frames;
+  double* macro;           // [n_clones][macro_frames][3][plane]
   int max_intervals;
   int pad_;
 };
@@ -126,18 +129,21 @@ inline uint32_t wave_word(int Lx, int Ly, int x0, int x1, int y0, int y1,
 
 // Grouping of the M+1 passes of an interval into HBM sweeps with P passes
 // per sweep (P >= 2): pass n starts a group of P when it is a
-// stream-and-collide pass (1 <= n, n+P-1 <= M-1) and the group does not
+// stream-and-collide pass (1 <= n, n+P-1 <= M-1), the group does not
 // straddle the last forced step M1 (all its passes collide steps < M1 or
-// all >= M1); every other pass runs alone.  The step kernel walks the same
+// all >= M1), and no pass of it is a macroscopic-output step (a multiple of
+// E > 0); every other pass runs alone.  The step kernel walks the same
 // rule.
-__host__ __device__ constexpr bool fused_group_at(int n, int M, int P, int M1) {
-  return n >= 1 && n + P - 1 <= M - 1 && (n + P - 1 < M1 || n >= M1);
+__host__ __device__ constexpr bool fused_group_at(int n, int M, int P, int M1,
+                                                  int E) {
+  return n >= 1 && n + P - 1 <= M - 1 && (n + P - 1 < M1 || n >= M1) &&
+         (E <= 0 || (n + E - 1) / E * E > n + P - 1);
 }
 
 // HBM sweeps of that grouping; each sweep swaps the ping-pong buffers.
-constexpr int fused_sweeps(int M, int P, int M1 = 0) {
+constexpr int fused_sweeps(int M, int P, int M1 = 0, int E = 0) {
   int s = 0;
-  for (int n = 0; n <= M; s++) n += fused_group_at(n, M, P, M1) ? P : 1;
+  for (int n = 0; n <= M; s++) n += fused_group_at(n, M, P, M1, E) ? P : 1;
   return s;
 }
 size_t fused_lds_bytes(int storage, int ring_max, int rch, int passes);

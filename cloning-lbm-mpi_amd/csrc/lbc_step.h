@@ -585,6 +585,7 @@ __device__ inline void collide_any(const typename Traits<ST>::C (&f)[9],
 template <bool FRC>
 struct Drive {
   double beta;
+  static constexpr bool kFrame = false;
   __device__ double at(int) const { return beta; }
 };
 
@@ -592,6 +593,7 @@ template <>
 struct Drive<true> {
   double beta, eps;
   const double* __restrict__ map;  // null outside the forced passes
+  static constexpr bool kFrame = false;
   __device__ double at(int site) const {
 #pragma clang fp contract(off)
     return map ? (1.0 + eps * map[site]) * beta : beta;
@@ -607,6 +609,72 @@ __device__ inline Drive<FRC> drive_of(const InstDev* __restrict__ I, int clone,
                               : nullptr};
   else
     return Drive<false>{I->beta};
+}
+
+// Drive of a pass that runs alone (a sweep of one pass): the body force,
+// plus where the step's macroscopic fields go when the pass collides a
+// macroscopic-output step (null otherwise).
+template <class DV>
+struct LonePass {
+  DV dv;
+  double* __restrict__ out;  // [3][plane]: rho, ux, uy
+  static constexpr bool kFrame = true;
+  __device__ double at(int site) const { return dv.at(site); }
+};
+
+// Output planes of pass n of a clone when step n is an output step
+// (n < M, n a multiple of macro_every), else null.
+__device__ inline double* macro_out(const InstDev* __restrict__ I, int clone,
+                                    int n, int M) {
+  const int E = I->macro_every;
+  if (E <= 0 || n >= M || n % E) return nullptr;
+  return I->macro + ((long long)clone * I->macro_frames + n / E) * 3LL * I->plane;
+}
+
+// rho, ux, uy of a site as the collision of its step computes them
+// (src/lbm_src/streamCollCompute.cpp:17-26: FP64 sums in direction order,
+// ux/rho + beta/(2 rho), uy/rho; shifted storages the FP32 moments of
+// collide_shifted).  A solid interior site holds the rest state, as the
+// reference's reset leaves it (src/mpi_LBM_GK_pops_perturb.cpp:430-439).
+template <int MODE, int ST>
+__device__ inline void store_macro(const typename Traits<ST>::C (&f)[9],
+                                   bool solid, double beta,
+                                   double* __restrict__ out, int site,
+                                   int plane) {
+  double rho, ux, uy;
+  if constexpr (!Traits<ST>::shifted) {
+#pragma clang fp contract(off)
+    rho = 0.0;
+    ux = 0.0;
+    uy = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const double ft = solid ? WD[k] : (double)f[k];
+      rho += ft;
+      ux += ft * (double)CX[k];
+      uy += ft * (double)CY[k];
+    }
+    ux = ux / rho + 0.5 * beta / rho;
+    uy /= rho;
+  } else {
+    float d[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) d[k] = solid ? 0.0f : (float)f[k];
+    const float drho = ((d[0] + d[1]) + (d[2] + d[3])) +
+                       ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
+    const float jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
+    const float jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
+    const float r = 1.0f + drho;
+    const float inv = __builtin_amdgcn_rcpf(r);
+    float u = jx * inv;
+    if (MODE == kModeRef) u = fmaf(0.5f * (float)beta, inv, u);
+    rho = r;
+    ux = u;
+    uy = jy * inv;
+  }
+  out[site] = rho;
+  out[plane + site] = ux;
+  out[2 * plane + site] = uy;
 }
 
 // Storage elements of a site: collided (any pass but stream-only) or packed
@@ -907,8 +975,11 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
       channel_x_boundary<ST>(f, x == 0, I->u_in, I->inlet_k);
     }
   }
+  const double beta = dv.at(site);
+  if constexpr (DV::kFrame)
+    if (dv.out) store_macro<MODE, ST>(f, solid, beta, dv.out, site, g.plane);
   typename T::S o[9];
-  finish_site<MODE, ST>(f, o, PH, dv.at(site), tau);
+  finish_site<MODE, ST>(f, o, PH, beta, tau);
   if (w.special && solid) {
     // the reference resets the solid interior to w every step; its
     // post-collision values never reach a fluid node
@@ -1150,8 +1221,9 @@ __global__ void __launch_bounds__(1024)
     const HbmIO<ST>& dst = (n & 1) ? ioA : ioB;
     double* ring = ring_lds + (n & 1) * ring_max;
     const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
-    site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, ring,
-                            drive_of<FRC>(I, wi.clone, n < M1), k.tau, I, ph);
+    const LonePass<Drive<FRC>> lp{drive_of<FRC>(I, wi.clone, n < M1),
+                                  macro_out(I, wi.clone, n, k.M)};
+    site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, ring, lp, k.tau, I, ph);
     __syncthreads();
     if (n >= 1 && tid == 0) take_drag<MODE>(k, ring, n, acc);
   }
@@ -1161,10 +1233,12 @@ __global__ void __launch_bounds__(1024)
 // The same interval with NP passes per HBM sweep where it can: the
 // collide-only pass 0 runs alone, passes 1..M-1 (stream and collide) run NP
 // at a time through the LDS rings (fused_sweep; the remainder alone), and
-// the stream-only pass M runs alone, fused_sweeps(M, NP, M1) sweeps in all,
-// so the interval ends with S_M in buf[parity ^ (fused_sweeps(M, NP, M1) & 1)].
-// With FRC no sweep mixes forced passes (n < M1 = forced_steps) with the
-// rest (fused_group_at), so the force is one choice per sweep.
+// the stream-only pass M runs alone, fused_sweeps(M, NP, M1, E) sweeps in
+// all, so the interval ends with S_M in
+// buf[parity ^ (fused_sweeps(M, NP, M1, E) & 1)].  With FRC no sweep mixes
+// forced passes (n < M1 = forced_steps) with the rest, and macroscopic
+// output steps (multiples of E = macro_every) run alone (fused_group_at),
+// so the force is one choice per sweep and output comes from lone passes.
 // LDS: NP-1 population rings of 9*rch sites, then 2*NP drag slot arrays
 // (NP per sweep, double-buffered over sweeps).
 template <int MODE, int ST, bool NT, int RCH, int NP, bool FRC>
@@ -1181,6 +1255,7 @@ __device__ inline void interval_fused(const InstDev* __restrict__ insts,
   const HbmIO<ST> ioB{clone_rsrc(k.B, k.clone_bytes), k.g.plane};
   double* const slots = lds + (NP - 1) * (LdsIO<ST, RCH>::BYTES / 8);
   const int M1 = FRC ? I->forced_steps : 0;
+  const int E = I->macro_every;
   double acc = 0.0;
   int sweep = 0;
   for (int n = 0; n <= k.M; sweep++) {
@@ -1188,7 +1263,7 @@ __device__ inline void interval_fused(const InstDev* __restrict__ insts,
     const HbmIO<ST>& dst = (sweep & 1) ? ioA : ioB;
     double* drag = slots + (sweep & 1) * NP * ring_max;
     const Drive<FRC> dv = drive_of<FRC>(I, wi.clone, n < M1);
-    if (fused_group_at(n, k.M, NP, M1)) {
+    if (fused_group_at(n, k.M, NP, M1, E)) {
       fused_sweep<MODE, ST, NT, RCH, NP>(k.g, tid, nthr, src, dst, drag,
                                          ring_max, dv, k.tau, I);
       __syncthreads();
@@ -1198,7 +1273,8 @@ __device__ inline void interval_fused(const InstDev* __restrict__ insts,
       n += NP;
     } else {
       const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
-      site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, drag, dv, k.tau, I, ph);
+      const LonePass<Drive<FRC>> lp{dv, macro_out(I, wi.clone, n, k.M)};
+      site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, drag, lp, k.tau, I, ph);
       __syncthreads();
       if (n >= 1 && tid == 0) take_drag<MODE>(k, drag, n, acc);
       n += 1;

@@ -46,6 +46,7 @@
 #include <vector>
 
 #include "lbc.h"
+#include "vtk_writer.h"
 
 namespace {
 
@@ -91,39 +92,12 @@ void fields_of(const std::vector<double>& f, int Dx, int Dy, double beta,
     }
 }
 
-// The reference's VTK writer (write_vtk.cpp:8-53): rectilinear grid, density
-// difference and velocity, default stream formatting.
+// One VTK frame of fields in the reference's idx(x,y) layout.
 bool write_vtk(int t, int Dx, int Dy, const Fields& F, const std::string& folder) {
-  std::ostringstream name;
-  name << folder << "fluid_t" << t << ".vtk";
-  std::ofstream o(name.str());
-  if (!o) return false;
-  const double rho0 = 1.0;
-  o << "# vtk DataFile Version 3.0\n";
-  o << "fluid_state\n";
-  o << "ASCII\n";
-  o << "DATASET RECTILINEAR_GRID\n";
-  o << "DIMENSIONS " << Dx << " " << Dy << " 1" << "\n";
-  o << "X_COORDINATES " << Dx << " float\n";
-  for (int i = 0; i < Dx; ++i) o << i << " ";
-  o << "\n";
-  o << "Y_COORDINATES " << Dy << " float\n";
-  for (int j = 0; j < Dy; ++j) o << j << " ";
-  o << "\n";
-  o << "Z_COORDINATES " << 1 << " float\n";
-  o << 0 << "\n";
-  o << "POINT_DATA " << Dx * Dy << "\n";
-  o << "SCALARS density float 1\n";
-  o << "LOOKUP_TABLE default\n";
-  for (int Y = 0; Y < Dy; ++Y)
-    for (int X = 0; X < Dx; ++X) o << F.rho[(size_t)Y + (size_t)Dy * X] - rho0 << "\n";
-  o << "VECTORS velocity_vector float\n";
-  for (int Y = 0; Y < Dy; ++Y)
-    for (int X = 0; X < Dx; ++X) {
-      const size_t i = (size_t)Y + (size_t)Dy * X;
-      o << F.ux[i] << " " << F.uy[i] << " 0\n";
-    }
-  return (bool)o;
+  auto at = [Dy](const std::vector<double>& v) {
+    return [&v, Dy](int X, int Y) { return v[(size_t)Y + (size_t)Dy * X]; };
+  };
+  return write_fluid_vtk(t, Dx, Dy, folder, at(F.rho), at(F.ux), at(F.uy));
 }
 
 bool make_dir(const std::string& d) {

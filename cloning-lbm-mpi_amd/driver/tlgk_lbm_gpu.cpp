@@ -35,7 +35,10 @@
 //     ranks       p: reproduce a p-rank run's draw streams and total_R order
 //     init        files (default in ref mode) | equilibrium
 //     init_amp    velocity noise amplitude of the equilibrium initial state
-//     force_io    1 (default) | 0: per-clone force files
+//     force_io    1 (default) | 0: per-clone force files and VTK frames
+//     vtk         1 (default) | 0: with force_io, write each clone's VTK
+//                 frames every facquVTK steps of every interval, numbered
+//                 t*floor(M/facquVTK) + k as the reference does (:450-455)
 //     zero_mean   1 (default) | 0: subtract AVG = 1.775 from F/F0
 //     seed        Philox key
 //     threads     step-kernel workgroup size (0 = tuned)
@@ -71,6 +74,7 @@
 #include <vector>
 
 #include "lbc.h"
+#include "vtk_writer.h"
 
 namespace {
 
@@ -223,6 +227,10 @@ int main(int argc, char** argv) {
   const bool channel = kv(in, "mode", "ref") == "channel";
   const bool forcing = !channel && kv(in, "order", "pops") == "forcing";
   const bool dump_fields = forcing && kvi(in, "dump_fields", 0) != 0;
+  // VTK frames of every clone (src/mpi_LBM_GK_pops_perturb.cpp:404-455);
+  // the forcing driver writes none
+  const bool vtk = kvi(in, "force_io", 1) != 0 && !forcing &&
+                   kvi(in, "vtk", 1) != 0 && in.facquVTK > 0;
   const std::string st = kv(in, "storage", "f64");
   const int storage = st == "f32" ? LBC_STORE_F32
                       : st == "f16" ? LBC_STORE_F16
@@ -304,6 +312,7 @@ int main(int argc, char** argv) {
   }
   inst.tau = in.tau;
   inst.n_clones = Nc;
+  if (vtk) inst.macro_every = in.facquVTK;
   inst.bias = channel ? kvd(in, "k", in.alpha) : in.alpha;
   inst.rank_blocks = p;
 
@@ -467,6 +476,38 @@ int main(int argc, char** argv) {
             : std::ofstream(folder[j] + "data_force.datout",
                             std::ios::binary | std::ios::app);
         o.write(reinterpret_cast<const char*>(&A[(size_t)j * M]), M * sizeof(double));
+      }
+    }
+    if (vtk) {
+      // frames of step k*facquVTK of interval t are fluid_t<t*nbrVTK + k>.vtk
+      // (nbrVTK = floor(M/facquVTK): with facquVTK > M every interval
+      // rewrites frame 0, as the reference does)
+      const int E = in.facquVTK;
+      const int frames = (M + E - 1) / E;
+      const int nbrVTK = M / E;
+      const size_t per = (size_t)frames * 3 * Nsite;
+      const int chunk = (int)std::max<size_t>(1, ((size_t)1 << 27) / per);
+      std::vector<double> mac;
+      for (int j0 = 0; j0 < Nc; j0 += chunk) {
+        const int n = std::min(chunk, Nc - j0);
+        mac.resize(per * n);
+        check(lbc_get_macro(ctx, 0, j0, n, mac.data()), ctx, "lbc_get_macro");
+        for (int j = j0; j < j0 + n; j++)
+          for (int k = 0; k < frames; k++) {
+            const double* m = mac.data() + per * (j - j0) + (size_t)k * 3 * Nsite;
+            const int Lx = inst.Lx;
+            auto plane = [m, Lx, Nsite](int q) {
+              return [m, Lx, Nsite, q](int X, int Y) {
+                return m[q * Nsite + (size_t)Y * Lx + X];
+              };
+            };
+            if (!write_fluid_vtk(t * nbrVTK + k, inst.Lx, inst.Ly, folder[j],
+                                 plane(0), plane(1), plane(2))) {
+              log << "ERROR : could not write a VTK frame in " << folder[j]
+                  << std::endl;
+              return 1;
+            }
+          }
       }
     }
     if (dump_fields) {

@@ -38,7 +38,7 @@ EXPORTS = (
     "lbc_set_marks", "lbc_get_marks", "lbc_run_interval", "lbc_get_drag",
     "lbc_get_logweights", "lbc_select", "lbc_gather", "lbc_advance",
     "lbc_get_R_history", "lbc_get_scgf", "lbc_set_field_maps",
-    "lbc_get_field_maps", "lbc_sync", "lbc_step_kernel_ms", "lbc_threads",
+    "lbc_get_field_maps", "lbc_get_macro", "lbc_sync", "lbc_step_kernel_ms", "lbc_threads",
     "lbc_sites_per_thread", "lbc_step_shape", "lbc_step_host",
 )
 
@@ -55,7 +55,7 @@ class Instance(C.Structure):
         ("noise_sigma", C.c_double),
         ("rank_blocks", C.c_int), ("n_snapshots", C.c_int),
         ("forced_steps", C.c_int), ("field_eps", C.c_double),
-        ("field_sigma", C.c_double),
+        ("field_sigma", C.c_double), ("macro_every", C.c_int),
     ]
 
 
@@ -118,6 +118,7 @@ def lib() -> C.CDLL:
                                C.POINTER(C.c_int)]
     L.lbc_set_field_maps.argtypes = [vp, C.c_int, dp]
     L.lbc_get_field_maps.argtypes = [vp, C.c_int, dp]
+    L.lbc_get_macro.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
     L.lbc_sync.argtypes = [vp]
     L.lbc_step_kernel_ms.argtypes = [vp, dp, C.POINTER(C.c_int)]
     L.lbc_threads.argtypes = [vp]
@@ -167,6 +168,7 @@ class InstanceSpec:
     forced_steps: int = 0
     field_eps: float = 0.0
     field_sigma: float = 0.0
+    macro_every: int = 0
 
     def c(self) -> Instance:
         return Instance(self.Lx, self.Ly, self.xmin, self.xmax, self.ymin,
@@ -174,7 +176,8 @@ class InstanceSpec:
                         self.beta, self.u_in, self.force_scale,
                         self.force_shift, self.dt, self.bias, self.eps,
                         self.noise_sigma, self.rank_blocks, self.n_snapshots,
-                        self.forced_steps, self.field_eps, self.field_sigma)
+                        self.forced_steps, self.field_eps, self.field_sigma,
+                        self.macro_every)
 
     @property
     def sites(self) -> int:
@@ -443,6 +446,17 @@ class Sampler:
         out = np.zeros((s.n_clones, s.Lx, s.Ly))
         self._check(lib().lbc_get_field_maps(self._h, inst, _dp(out)),
                     "lbc_get_field_maps")
+        return out
+
+    def macro(self, inst: int, clone0: int = 0, n: int | None = None):
+        """Macroscopic fields of the last interval (macro_every > 0):
+        shape (n, frames, 3, Ly, Lx) -- rho, ux, uy at steps 0, E, 2E, ..."""
+        s = self.specs[inst]
+        n = s.n_clones - clone0 if n is None else n
+        frames = -(-s.steps // s.macro_every) if s.macro_every > 0 else 1
+        out = np.zeros((n, frames, 3, s.Ly, s.Lx))
+        self._check(lib().lbc_get_macro(self._h, inst, clone0, n, _dp(out)),
+                    "lbc_get_macro")
         return out
 
     def sync(self) -> None:

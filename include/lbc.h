@@ -79,6 +79,14 @@ typedef struct lbc_instance {
   int forced_steps;
   double field_eps;
   double field_sigma;
+  /* Macroscopic output (src/mpi_LBM_GK_pops_perturb.cpp:450-455 with
+   * src/lbm_src/write_vtk.cpp): with macro_every = E > 0 every interval
+   * stores rho, ux and uy of every clone at steps 0, E, 2E, ... < steps,
+   * the fields the collision of that step computes
+   * (src/lbm_src/streamCollCompute.cpp:17-26,44-45: ux includes the
+   * beta/(2 rho) shift; the solid interior holds the rest state), for
+   * lbc_get_macro.  0 = off. */
+  int macro_every;
 } lbc_instance;
 
 typedef struct lbc_config {
@@ -208,6 +216,13 @@ int lbc_set_field_maps(lbc_ctx* ctx, int inst, const double* maps);
 /* The random fields the last interval of instance inst ran with, in the same
  * layout. */
 int lbc_get_field_maps(lbc_ctx* ctx, int inst, double* maps);
+
+/* Macroscopic fields of the last interval of clones [clone0, clone0 + n) of
+ * instance inst (macro_every > 0): per clone, frames = ceil(steps /
+ * macro_every) frames (frame k at step k*macro_every), each three planes
+ * rho, ux, uy of Lx*Ly doubles in row order y*Lx + x (the point order of
+ * the reference's VTK writer, src/lbm_src/write_vtk.cpp:38-49). */
+int lbc_get_macro(lbc_ctx* ctx, int inst, int clone0, int n, double* out);
 
 /* Wait for the context's stream. */
 int lbc_sync(lbc_ctx* ctx);

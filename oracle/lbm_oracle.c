@@ -300,6 +300,34 @@ double orc_run_steps(const orc_geom* g, double* state, double* scratch, int M,
   return acc;
 }
 
+/* Fields of a state as the step's collision computes them
+ * (src/lbm_src/streamCollCompute.cpp:17-26,44-45). */
+void orc_macro(const orc_geom* g, const double* f, const double* map,
+               double eps, double* rho, double* ux, double* uy) {
+  const int Dx = g->Lx, Dy = g->Ly;
+  const int periodic = (g->mode == ORC_MODE_REF);
+  for (int x = 0; x < Dx; x++) {
+    for (int y = 0; y < Dy; y++) {
+      const double* p = f + IDX(g, x, y, 0);
+      const double beta =
+          !periodic ? 0.0
+                    : (map ? (1.0 + eps * map[y + Dy * x]) * g->beta : g->beta);
+      double r = 0.0, u = 0.0, v = 0.0;
+      for (int k = 0; k < 9; k++) {
+        double ft = p[k];
+        r += ft;
+        u += ft * orc_c[k][0];
+        v += ft * orc_c[k][1];
+      }
+      u = u / r + 0.5 * beta / r;
+      v /= r;
+      rho[y + Dy * x] = r;
+      ux[y + Dy * x] = u;
+      uy[y + Dy * x] = v;
+    }
+  }
+}
+
 /* Forcing-mode interval (src/mpi_LBM_GK_forcing.cpp:335-400): M1 steps
  * with the per-site force of map, then M - M1 with the uniform one; the
  * drag of every step is scaled, shifted and summed as in orc_run_steps. */

@@ -76,6 +76,13 @@ double orc_run_steps_forced(const orc_geom* g, double* state, double* scratch,
 void orc_field_normals(uint32_t seed_lo, uint32_t seed_hi, uint32_t clone,
                        uint32_t interval, uint32_t instance, int n, double* z);
 
+/* Macroscopic fields of state f as the step's collision computes them
+ * (src/lbm_src/streamCollCompute.cpp:17-26,44-45): rho, ux/rho + beta/(2 rho)
+ * (REF; beta per site with map, NULL = uniform; CHANNEL beta = 0), uy/rho,
+ * each Lx*Ly doubles at idx(x,y) = y + Ly*x. */
+void orc_macro(const orc_geom* g, const double* f, const double* map,
+               double eps, double* rho, double* ux, double* uy);
+
 /* Drag functionals on a post-stream state. */
 double orc_force_stress(const orc_geom* g, const double* f);
 double orc_force_mea(const orc_geom* g, const double* f);

@@ -123,6 +123,8 @@ def lib() -> C.CDLL:
                                            C.c_double, dp]
         L.orc_run_steps_forced.restype = C.c_double
         L.orc_field_normals.argtypes = [C.c_uint32] * 5 + [C.c_int, dp]
+        L.orc_macro.argtypes = [gp, dp, dp, C.c_double, dp, dp, dp]
+        L.orc_macro.restype = None
         L.orc_field_normals.restype = None
         L.orc_glibc_srand.argtypes = [C.c_uint]
         L.orc_glibc_rand_n.argtypes = [C.c_int, ip]
@@ -208,6 +210,17 @@ def step_map(g: Geom, fin: np.ndarray, fout: np.ndarray, fmap: np.ndarray,
     shape (Lx, Ly) (the reference's idx(x,y) = y + Ly*x layout)."""
     m = np.ascontiguousarray(fmap, np.float64)
     return lib().orc_step_map(C.byref(g), _dp(fin), _dp(fout), _dp(m), eps)
+
+
+def macro(g: Geom, f: np.ndarray, fmap: np.ndarray | None = None,
+          eps: float = 0.0) -> np.ndarray:
+    """rho, ux, uy of state f as the step computes them; shape (3, Lx, Ly)."""
+    out = np.zeros((3, g.Lx, g.Ly))
+    m = None if fmap is None else np.ascontiguousarray(fmap, np.float64)
+    lib().orc_macro(C.byref(g), _dp(np.ascontiguousarray(f)),
+                    None if m is None else _dp(m), eps, _dp(out[0]),
+                    _dp(out[1]), _dp(out[2]))
+    return out
 
 
 def run_steps_forced(g: Geom, state: np.ndarray, M1: int, M: int,
@@ -389,6 +402,23 @@ def ref_step(g: Geom, fin: np.ndarray, fout: np.ndarray) -> float:
     uy = np.zeros(S)
     R.ref_step(_dp(fin), _dp(fout), _dp(rho), _dp(ux), _dp(uy), g.beta, g.tau)
     return R.ref_force(_dp(fout), 1.0 / g.tau)
+
+
+def ref_step_fields(g: Geom, fin: np.ndarray, fout: np.ndarray,
+                    fmap: np.ndarray | None = None, eps: float = 0.0):
+    """One reference step (uniform, or spatial with fmap); returns the
+    fields rho, ux, uy the reference kernel writes, shape (3, Lx, Ly)."""
+    R = ref()
+    R.ref_set_geometry(g.Lx, g.Ly, g.xmin, g.xmax, g.ymin, g.ymax)
+    out = np.zeros((3, g.Lx, g.Ly))
+    if fmap is None:
+        R.ref_step(_dp(fin), _dp(fout), _dp(out[0]), _dp(out[1]), _dp(out[2]),
+                   g.beta, g.tau)
+    else:
+        m = np.ascontiguousarray(fmap, np.float64).copy()
+        R.ref_step_spatial(_dp(fin), _dp(fout), _dp(out[0]), _dp(out[1]),
+                           _dp(out[2]), g.beta, _dp(m), g.tau, eps)
+    return out
 
 
 def ref_step_spatial(g: Geom, fin: np.ndarray, fout: np.ndarray,

@@ -34,8 +34,14 @@ Fixtures (numpy .npz, FP64 unless noted):
       each with walls, square and interior reset; the post-collision array of
       the first step, the final state and the raw force after every step.
 
+  ref_macro_L4.npz
+      the macroscopic fields rho, ux, uy (each (Lx, Ly), idx(x,y) layout)
+      that the reference's step kernels write for a seeded state: uniform
+      (streamingAndCollisionComputeMacroBodyForce) and spatial with a map
+      (...Spatial), src/lbm_src/streamCollCompute.cpp:17-26,44-45.
+
 Run from the repository root: python tests/golden/make_golden.py [name ...]
-(names: steps perturb glibc bodyforce forced; default all).
+(names: steps perturb glibc bodyforce forced macro; default all).
 """
 import os
 import shutil
@@ -151,6 +157,19 @@ def forced_fixture(L=4, M1=5, M=9, eps=0.4, seed=21):
         final=a, forces=np.array(forces))
 
 
+def macro_fixture(L=4, eps=0.7, seed=31):
+    g = O.ref_box_geom(L, 0.53, 0.05)
+    rng = np.random.default_rng(seed)
+    s0 = seeded_state(g, rng)
+    fmap = rng.standard_normal((g.Lx, g.Ly))
+    uni = O.ref_step_fields(g, s0.copy(), np.zeros_like(s0))
+    spa = O.ref_step_fields(g, s0.copy(), np.zeros_like(s0), fmap, eps)
+    np.savez_compressed(
+        os.path.join(OUT, f"ref_macro_L{L}.npz"), L=L, tau=g.tau,
+        geom=np.array([g.Lx, g.Ly, g.xmin, g.xmax, g.ymin, g.ymax]),
+        beta=g.beta, S0=s0, map=fmap, eps=eps, uniform=uni, spatial=spa)
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libref.so missing: run `make -C oracle ref` "
@@ -162,6 +181,7 @@ def main():
         "glibc": glibc_fixture,
         "bodyforce": bodyforce_fixture,
         "forced": forced_fixture,
+        "macro": macro_fixture,
     }
     for name in sys.argv[1:] or list(jobs):
         jobs[name]()

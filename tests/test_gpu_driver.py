@@ -267,3 +267,53 @@ def test_forcing_driver_matches_reference_forcing_loop(tmp_path, O):
                                rtol=1e-12, atol=0)
     # the run did clone and prune
     assert any(not np.array_equal(lb, np.arange(Nc)) for lb in labels_ref)
+
+
+def _vtk_text(rho_ux_uy, Dx, Dy):
+    """The reference's VTK frame (src/lbm_src/write_vtk.cpp:8-53) as text;
+    C++ stream defaults print a double like printf's %g."""
+    rho, ux, uy = rho_ux_uy
+    lines = ["# vtk DataFile Version 3.0", "fluid_state", "ASCII",
+             "DATASET RECTILINEAR_GRID", f"DIMENSIONS {Dx} {Dy} 1",
+             f"X_COORDINATES {Dx} float",
+             "".join(f"{i} " for i in range(Dx)),
+             f"Y_COORDINATES {Dy} float",
+             "".join(f"{j} " for j in range(Dy)),
+             "Z_COORDINATES 1 float", "0", f"POINT_DATA {Dx * Dy}",
+             "SCALARS density float 1", "LOOKUP_TABLE default"]
+    lines += [f"{rho[X, Y] - 1.0:g}" for Y in range(Dy) for X in range(Dx)]
+    lines.append("VECTORS velocity_vector float")
+    lines += [f"{ux[X, Y]:g} {uy[X, Y]:g} 0" for Y in range(Dy)
+              for X in range(Dx)]
+    return "\n".join(lines) + "\n"
+
+
+def test_driver_vtk_frames_match_reference_loop(tmp_path, O):
+    """facquVTK = 5 with 20 steps per interval: every clone writes frames
+    fluid_t<4t + k>.vtk of steps 0, 5, 10, 15 (src/mpi_LBM_GK_pops_perturb.cpp:
+    404-455).  The first interval's frames equal the reference's text for the
+    oracle's states byte for byte."""
+    L, Nc, T, dT, eps, tau, U0, alpha = 4, 3, 0.5, 0.25, 0.02, 0.53, 0.05, 5.0
+    g, data, names, _ = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
+                                   alpha)
+    inp = tmp_path / "input_vtk"
+    inp.write_text(f"{Nc}\n{T}\n{dT}\n{eps}\n{L}\n{tau}\n{U0}\n{alpha}\n"
+                   f"{data}/\n{tmp_path}/vrun/\n5\n")
+    print(f"case: driver VTK frames L={L} Nc={Nc}", flush=True)
+    r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    run = tmp_path / "vrun"
+    for j in range(Nc):
+        a = np.fromfile(data / names[j]).reshape(g.Lx, g.Ly, 9)
+        b = np.zeros_like(a)
+        for n in range(20):
+            if n % 5 == 0:
+                text = (run / f"clone_{j}" / f"fluid_t{n // 5}.vtk").read_text()
+                assert text == _vtk_text(O.macro(g, a), g.Lx, g.Ly), (j, n)
+            O.step(g, a, b)
+            a, b = b, a
+        # the second interval's frames follow as 4..7
+        for k in range(4, 8):
+            assert (run / f"clone_{j}" / f"fluid_t{k}.vtk").exists()
+        assert not (run / f"clone_{j}" / "fluid_t8.vtk").exists()

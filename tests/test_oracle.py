@@ -552,3 +552,45 @@ def test_field_sample_statistics(O):
         b = fields[:, dx:, dy:]
         est = (a * b).mean()
         assert abs(est - cov[dx, dy]) < 0.06, (dx, dy, est, cov[dx, dy])
+
+
+# ---------------------------------------------------------------- macroscopic output
+def test_oracle_macro_matches_reference_fixture(O):
+    fx = np.load(os.path.join(GOLDEN, "ref_macro_L4.npz"))
+    g = _geom_from(O, fx)
+    assert np.array_equal(O.macro(g, fx["S0"]), fx["uniform"])
+    assert np.array_equal(O.macro(g, fx["S0"], fx["map"], float(fx["eps"])),
+                          fx["spatial"])
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(
+    os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+    "_ref", "libref.so")), reason="reference build (oracle/_ref) not present")
+def test_oracle_macro_matches_live_reference(O):
+    g = O.ref_box_geom(6, 0.6, 0.08)
+    rng = np.random.default_rng(12)
+    f = np.empty((g.Lx, g.Ly, 9))
+    for x in range(g.Lx):
+        for y in range(g.Ly):
+            f[x, y] = O.feq(1 + 1e-2 * rng.standard_normal(),
+                            0.08 + 1e-2 * rng.standard_normal(),
+                            1e-2 * rng.standard_normal())
+    fmap = rng.standard_normal((g.Lx, g.Ly))
+    for m, eps in [(None, 0.0), (fmap, 0.3)]:
+        want = O.ref_step_fields(g, f.copy(), np.zeros_like(f), m, eps)
+        assert np.array_equal(O.macro(g, f, m, eps), want)
+
+
+def test_macro_of_equilibrium_recovers_its_moments(O):
+    # feq(rho, u) has density rho and momentum rho*u, so the fields are rho,
+    # u_x + beta/(2 rho) and u_y (to rounding)
+    g = O.ref_box_geom(4, 0.6, 0.05)
+    rng = np.random.default_rng(2)
+    f = np.empty((g.Lx, g.Ly, 9))
+    want = np.empty((3, g.Lx, g.Ly))
+    for x in range(g.Lx):
+        for y in range(g.Ly):
+            r, u, v = 1 + 0.1 * rng.random(), 0.05 * rng.random(), 0.03 * rng.random()
+            f[x, y] = O.feq(r, u, v)
+            want[:, x, y] = (r, u + 0.5 * g.beta / r, v)
+    np.testing.assert_allclose(O.macro(g, f), want, rtol=1e-13, atol=1e-15)
