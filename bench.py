@@ -217,11 +217,42 @@ def cpu_baseline(sample_clones_per_core: int = 30, steps: int = 200) -> dict:
                     break
     except OSError:
         pass
-    return dict(value=lups / secs / 1e6, unit="clone-MLUPS", cores=cores,
-                kind="port",
-                sample=(f"{cores} threads x {sample_clones_per_core} clones x "
-                        f"{steps} steps of 400x100 (FP64 AoS push oracle), "
-                        f"{secs:.1f} s on {cpu}"))
+    out = dict(value=lups / secs / 1e6, unit="clone-MLUPS", cores=cores,
+               kind="port",
+               sample=(f"{cores} threads x {sample_clones_per_core} clones x "
+                       f"{steps} steps of 400x100 (FP64 AoS push oracle), "
+                       f"{secs:.1f} s on {cpu}"))
+    cal = calibrate_port(O)
+    if cal:
+        out["calibration"] = cal
+    return out
+
+
+def calibrate_port(O, L: int = 50, steps: int = 60) -> dict | None:
+    """One thread of the oracle's REF step against the reference's own step
+    kernels (oracle/_ref/libref.so: streamCollCompute.cpp, walls, square,
+    interior reset and force, compiled from the reference's sources with
+    its -O3) on the same (4L+1)^2 lattice: the port's speed relative to the
+    reference's.  None when the reference build is absent."""
+    if not O.ref_available():
+        return None
+    g = O.ref_box_geom(L, 0.53, 0.05)
+    f0 = np.broadcast_to(O.W, (g.Lx, g.Ly, 9)).copy()
+    rates = {}
+    for name in ("reference", "port"):
+        a, b = f0.copy(), np.zeros_like(f0)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if name == "port":
+                O.step(g, a, b)
+            else:
+                O.ref_step(g, a, b)
+            a, b = b, a
+        rates[name] = g.Lx * g.Ly * steps / (time.perf_counter() - t0) / 1e6
+    return dict(lattice=f"{g.Lx}x{g.Ly} REF, {steps} steps, 1 thread",
+                reference_kernels_mlups=round(rates["reference"], 2),
+                port_mlups=round(rates["port"], 2),
+                port_over_reference=round(rates["port"] / rates["reference"], 3))
 
 
 def measured_traffic(run: dict) -> int | None:
