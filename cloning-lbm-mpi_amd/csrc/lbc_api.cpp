@@ -384,11 +384,13 @@ int enqueue_perturb(lbc_ctx* c, bool philox) {
 }
 
 int autotune(lbc_ctx* c) {
-  // Time short intervals at each (workgroup size, sites per thread) on the
-  // freshly initialised state; the caller re-initialises the state after.
+  // Time short intervals (at most 48 steps: long enough that the share of
+  // the lone first and last passes is close to a full interval's, so the
+  // ranking of passes per sweep holds) at each shape on the freshly
+  // initialised state; the caller re-initialises the state after.
   std::vector<InstDev> tmp = c->idh;
   for (auto& d : tmp) {
-    d.steps = std::min(d.steps, 16);
+    d.steps = std::min(d.steps, 48);
     d.forced_steps = std::min(d.forced_steps, d.steps);
     d.macro_every = 0;
   }

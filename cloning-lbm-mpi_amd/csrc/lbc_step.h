@@ -783,12 +783,15 @@ __device__ inline void pull9(const LdsIO<ST, RCH, RING>& io,
 }
 
 template <int ST>
+struct PassConst;
+
+template <int ST>
 __device__ inline void own9(const HbmIO<ST>& io, typename Traits<ST>::C (&f)[9],
-                            int site, const int (&sown)[9]) {
+                            int site, const PassConst<ST>& pc) {
   constexpr int ES = HbmIO<ST>::ES;
   typename BufIO<ST>::R v[9];
 #pragma unroll
-  for (int q = 0; q < 9; q++) v[q] = BufIO<ST>::raw(io.r, site * ES, sown[q]);
+  for (int q = 0; q < 9; q++) v[q] = BufIO<ST>::raw(io.r, site * ES, pc.sown(q));
   join9(v);
 #pragma unroll
   for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(v[q]);
@@ -797,48 +800,56 @@ __device__ inline void own9(const HbmIO<ST>& io, typename Traits<ST>::C (&f)[9],
 template <int ST, int RCH, int RING>
 __device__ inline void own9(const LdsIO<ST, RCH, RING>& io,
                             typename Traits<ST>::C (&f)[9], int site,
-                            const int (&)[9]) {
+                            const PassConst<ST>&) {
 #pragma unroll
   for (int q = 0; q < 9; q++) f[q] = io.ld(q, site);
 }
 
 template <int ST, bool NT>
 __device__ inline void put9(const HbmIO<ST>& io, const typename Traits<ST>::S (&o)[9],
-                            int site, const int (&sown)[9]) {
+                            int site, const PassConst<ST>& pc) {
   constexpr int ES = HbmIO<ST>::ES;
 #pragma unroll
   for (int q = 0; q < 9; q++)
-    BufIO<ST>::template st_raw<NT>(io.r, site * ES, sown[q], o[q]);
+    BufIO<ST>::template st_raw<NT>(io.r, site * ES, pc.sown(q), o[q]);
 }
 
 template <int ST, bool NT, int RCH, int RING>
 __device__ inline void put9(const LdsIO<ST, RCH, RING>& io,
                             const typename Traits<ST>::S (&o)[9], int site,
-                            const int (&)[9]) {
+                            const PassConst<ST>&) {
 #pragma unroll
   for (int q = 0; q < 9; q++) io.template st<NT>(q, site, o[q]);
 }
 
-// Per-pass constants of the pull and store addressing.
+// Per-pass constants of the pull and store addressing: two wave-uniform
+// byte counts from which every direction's soffset follows with a scalar
+// multiply-add.  renew() hides their values from the compiler once per
+// chunk, so it recomputes the nine soffsets with scalar instructions where
+// they are used instead of keeping eighteen of them live in SGPRs across
+// the sweep (which, at eight waves per SIMD, would spill them to VGPR
+// lanes and cost a vector instruction per reload).
 template <int ST>
 struct PassConst {
-  int rowb;     // one lattice row in bytes (LDS source)
-  int soff[9];  // pull soffset of direction q, in bytes (HBM source; one
-                // row less for cx = -1, whose voffset is one row ahead)
-  int sown[9];  // own-site soffset of direction q, in bytes
+  static constexpr int ES = (int)sizeof(typename Traits<ST>::S);
+  int planeb;   // one direction plane in bytes
+  int rowb;     // one lattice row in bytes
   int adjL, adjR;
   __device__ PassConst(const Geo& g, int mode) {
-    constexpr int ES = (int)sizeof(typename Traits<ST>::S);
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-      const int off = CY[q] * g.Lx + CX[q];
-      sown[q] = q * g.plane * ES;
-      soff[q] = (q * g.plane - off - (CX[q] < 0 ? g.Lx : 0)) * ES;
-    }
+    planeb = g.plane * ES;
     rowb = g.Lx * ES;
     adjL = (mode == kModeRef) ? g.Lx : 1;
     adjR = (mode == kModeRef) ? -g.Lx : -1;
   }
+  // own-site soffset of direction q
+  __device__ int sown(int q) const { return q * planeb; }
+  // pull soffset of direction q (HBM source): the plane, minus the source's
+  // row (cy) and column (cx) offsets, one row less for cx = -1, whose
+  // voffset is one row ahead
+  __device__ int soff(int q) const {
+    return q * planeb - CY[q] * rowb - CX[q] * ES - (CX[q] < 0 ? rowb : 0);
+  }
+  __device__ void renew() { asm volatile("" : "+s"(planeb), "+s"(rowb)); }
 };
 
 // What one wave's 64 consecutive sites touch (decoded from the instance's
@@ -912,7 +923,7 @@ __device__ inline void pull_issue(const Geo& g, const WaveGeo& w, int lane,
 #pragma unroll
   for (int q = 0; q < 9; q++)
     v[q] = BufIO<ST>::raw(src.r, CX[q] > 0 ? voL : (CX[q] < 0 ? voR : vo),
-                          pc.soff[q]);
+                          pc.soff(q));
 }
 
 // The nine pulled populations of a site.
@@ -943,7 +954,30 @@ __device__ inline void pull_site(const Geo& g, const WaveGeo& w, int lane,
 // lanes load each lane's own populations and take the bounce-back ones
 // from there (ring lanes add their drag term, solid lanes store the rest
 // state); channel x-boundary lanes apply the boundary rule.  Collision
-// (except in the stream-only pass) and nine stores follow.
+// (except in the stream-only pass) and nine stores follow.  Boundary waves
+// and the rest run separate copies of the collide-and-store tail, so the
+// common path carries neither the solid-lane select nor register copies
+// where the two would merge.
+template <int MODE, int ST, bool NT, class SRC, class DST, class DV>
+__device__ inline void collide_store(typename Traits<ST>::C (&f)[9], bool solid,
+                                     const DST& dst, const PassConst<ST>& pc,
+                                     const DV& dv, double tau, int site,
+                                     int plane, int PH) {
+  using T = Traits<ST>;
+  const double beta = dv.at(site);
+  if constexpr (DV::kFrame)
+    if (dv.out) store_macro<MODE, ST>(f, solid, beta, dv.out, site, plane);
+  typename T::S o[9];
+  finish_site<MODE, ST>(f, o, PH, beta, tau);
+  if (solid) {
+    // the reference resets the solid interior to w every step; its
+    // post-collision values never reach a fluid node
+#pragma unroll
+    for (int q = 0; q < 9; q++) o[q] = T::pack(T::rest(q));
+  }
+  put9<ST, NT>(dst, o, site, pc);
+}
+
 template <int MODE, int ST, bool NT, class SRC, class DST, class DV>
 __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
                                    int site, typename Traits<ST>::C (&f)[9],
@@ -960,7 +994,7 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
     lane_xy(g, w, lane, x, y);
     if (w.special) {
       C own[9];
-      own9(src, own, site, pc.sown);
+      own9(src, own, site, pc);
       const int m = wall_mask(y, g.Ly) |
                     square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
 #pragma unroll
@@ -974,19 +1008,12 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
     } else if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1)) {
       channel_x_boundary<ST>(f, x == 0, I->u_in, I->inlet_k);
     }
+    collide_store<MODE, ST, NT, SRC>(f, solid, dst, pc, dv, tau, site, g.plane,
+                                     PH);
+    return;
   }
-  const double beta = dv.at(site);
-  if constexpr (DV::kFrame)
-    if (dv.out) store_macro<MODE, ST>(f, solid, beta, dv.out, site, g.plane);
-  typename T::S o[9];
-  finish_site<MODE, ST>(f, o, PH, beta, tau);
-  if (w.special && solid) {
-    // the reference resets the solid interior to w every step; its
-    // post-collision values never reach a fluid node
-#pragma unroll
-    for (int q = 0; q < 9; q++) o[q] = T::pack(T::rest(q));
-  }
-  put9<ST, NT>(dst, o, site, pc.sown);
+  collide_store<MODE, ST, NT, SRC>(f, false, dst, pc, dv, tau, site, g.plane,
+                                   PH);
 }
 
 // One site of one pass: the collide-only pass loads the site's own
@@ -1001,7 +1028,7 @@ __device__ inline void site_step(const Geo& g, const WaveGeo& w, int lane,
                                  int PH) {
   typename Traits<ST>::C f[9];
   if (PH == kCollide)
-    own9(src, f, site, pc.sown);
+    own9(src, f, site, pc);
   else
     pull_site(g, w, lane, site, src, pc, f);
   site_finish<MODE, ST, NT>(g, w, lane, site, f, src, dst, pc, ring, dv, tau,
@@ -1015,13 +1042,14 @@ __device__ inline void site_pass(const Geo& g, int tid, int nthr,
                                  double* __restrict__ ring, const DV& dv,
                                  double tau, const InstDev* __restrict__ I,
                                  int PH) {
-  const PassConst<ST> pc(g, MODE);
+  PassConst<ST> pc(g, MODE);
   const int lane = tid & 63;
   const int wblk = __builtin_amdgcn_readfirstlane(tid) >> 6;  // wave in chunk
   const int bpc = nthr >> 6;                                  // waves per chunk
   const int nch = (g.plane + nthr - 1) / nthr;
   int site = tid;
   for (int c = 0; c < nch; c++) {
+    pc.renew();
     if (site < g.plane)
       site_step<MODE, ST, NT>(g, wave_of(I, c * bpc + wblk), lane, site, src,
                               dst, pc, ring, dv, tau, I, PH);
@@ -1060,39 +1088,53 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
                                    double* drag, int ring_max, const DV& dv,
                                    double tau, const InstDev* __restrict__ I) {
   static_assert(NP == 2 || NP == 3, "two or three passes per sweep");
-  const PassConst<ST> pc(g, MODE);
+  // HBM prefetch distance in chunks: each wave keeps two chunks' first-pass
+  // loads in flight (measured: 1-2% faster than one at two passes per
+  // sweep, 9% at three, where the rings leave half the waves; three chunks
+  // gain nothing more)
+  constexpr int PF = 2;
+  using R = typename BufIO<ST>::R;
+  PassConst<ST> pc(g, MODE);
   const int CH = nthr;
   const int L = (pull_reach(MODE, g.Lx) + CH - 1) / CH;
   const int nch = (g.plane + CH - 1) / CH;
+  const int total = nch + (NP - 1) * L;  // chunk steps of the sweep
   const int lane = tid & 63;
   const int wblk = __builtin_amdgcn_readfirstlane(tid) >> 6;  // wave in chunk
   const int bpc = CH >> 6;                                    // waves per chunk
   const LdsIO<ST, RCH, 0> r0{};
   const LdsIO<ST, RCH, 1> r1{};
-  // The first pass's HBM pulls are issued one chunk ahead, so they travel
-  // while the later passes of the chunk before compute.  They are issued
-  // for every lane: past the lattice they read padding or, past the clone,
+  // The first pass's HBM pulls are issued PF chunks ahead, so they travel
+  // while the passes of the chunks before compute.  They are issued for
+  // every lane: past the lattice they read padding or, past the clone,
   // zeros (raw buffer range check), and nobody uses them; a conditional
   // load would make the compiler carry the old values along and wait.
-  typename BufIO<ST>::R pre[9];
-  pull_issue(g, wave_of(I, wblk), lane, tid, src, pc, pre);
-  for (int c = 0; c < nch + (NP - 1) * L; c++) {
-    if (c < nch) {
+  // With PF > 1 the steps rotate through PF register sets (the loop body is
+  // unrolled PF times) and the loads are issued on every step, so the wait
+  // before a set's use counts the other sets' loads as still in flight;
+  // their wave-table lookups are clamped to the table.
+  R pre[PF][9];
+#pragma unroll
+  for (int u = 0; u < PF; u++)
+    pull_issue(g, wave_of(I, min(u, nch) * bpc + wblk), lane, tid + u * CH,
+               src, pc, pre[u]);
+  auto step = [&](int c, R (&pre)[9]) {
+    pc.renew();
+    if (PF > 1 || c < nch) {
       const int s = c * CH + tid;
-      const WaveGeo w = wave_of(I, c * bpc + wblk);
       typename Traits<ST>::C f[9];
 #pragma unroll
       for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(pre[q]);
-      // the converted values first, then the next chunk's loads into the
+      // the converted values first, then chunk c + PF's loads into the
       // same registers (loads cannot move above a memory clobber)
       asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]), "+v"(f[3]),
                    "+v"(f[4]), "+v"(f[5]), "+v"(f[6]), "+v"(f[7]), "+v"(f[8])
                    :: "memory");
-      pull_issue(g, wave_of(I, (c + 1) * bpc + wblk), lane, s + CH, src, pc,
-                 pre);
-      if (s < g.plane)
-        site_finish<MODE, ST, false>(g, w, lane, s, f, src, r0, pc, drag, dv,
-                                     tau, I, kBoth);
+      pull_issue(g, wave_of(I, min(c + PF, nch) * bpc + wblk), lane,
+                 s + PF * CH, src, pc, pre);
+      if (c < nch && s < g.plane)
+        site_finish<MODE, ST, false>(g, wave_of(I, c * bpc + wblk), lane, s, f,
+                                     src, r0, pc, drag, dv, tau, I, kBoth);
     }
     lds_barrier();
     if constexpr (NP == 3) {
@@ -1107,7 +1149,7 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
       lds_barrier();
     }
     const int cz = c - (NP - 1) * L;
-    if (cz >= 0) {
+    if (cz >= 0 && cz < nch) {
       const int s = cz * CH + tid;
       if (s < g.plane) {
         double* dz = drag + (NP - 1) * ring_max;
@@ -1119,6 +1161,11 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
                                   dst, pc, dz, dv, tau, I, kBoth);
       }
     }
+  };
+  // a multiple of PF steps; steps past the last do nothing
+  for (int c = 0; c < total; c += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; u++) step(c + u, pre[u]);
   }
 }
 
