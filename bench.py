@@ -26,6 +26,7 @@ import json
 import math
 import os
 import re
+import subprocess
 import sys
 import threading
 import time
@@ -270,6 +271,21 @@ def measured_traffic(run: dict) -> int | None:
     return None
 
 
+def stream_ceilings() -> dict | None:
+    """Measured streaming rates of the box (GB/s, read + write) from the
+    stream_probe summary under profiles/: plain copy, read, write, and the
+    step kernel's FP16 access pattern without arithmetic; None if absent."""
+    pat = re.compile(r"^stream_ceiling (\w+): (\d+)\s*$")
+    out = {}
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
+        with open(path) as f:
+            for line in f:
+                m = pat.match(line.strip())
+                if m:
+                    out[m.group(1)] = int(m.group(2))
+    return out or None
+
+
 def roofline(run: dict) -> dict:
     """HBM roofline of the step kernel.  achieved = algorithmic bytes
     (2*9*b per lattice update) / average kernel time; with P LBM passes
@@ -290,6 +306,11 @@ def roofline(run: dict) -> dict:
         dram = traffic / (run["kernel_ms"] * 1e-3) / 1e9
         out["dram_achieved"] = round(dram, 1)
         out["dram_frac"] = round(dram / HBM_PEAK_GBS, 4)
+        ceil = stream_ceilings()
+        if ceil:
+            out["stream_ceilings"] = ceil
+            if "copy" in ceil:
+                out["dram_frac_of_copy"] = round(dram / ceil["copy"], 4)
     return out
 
 
@@ -316,6 +337,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="run cloning-lbm-mpi_amd/stream_probe (streaming "
+                         "ceilings for the roofline) and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (torch) so that "
                          "HBM counters can be scaled against known bytes")
@@ -332,6 +356,9 @@ def main() -> int:
         return 2
     if args.calibrate_copy:
         return calibrate_copy()
+    if args.stream_ceiling:
+        exe = os.path.join(ROOT, "cloning-lbm-mpi_amd", "stream_probe")
+        return subprocess.run([exe]).returncode
     K, W = args.steps, args.warmup
     want = [x for x in args.only.split(",") if x] or ["f16", "f64", "f32",
                                                        "bf16"]
