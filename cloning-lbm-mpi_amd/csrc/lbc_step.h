@@ -1106,9 +1106,11 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
   const LdsIO<ST, RCH, 1> r1{};
   // The first pass's HBM pulls are issued PF chunks ahead, so they travel
   // while the passes of the chunks before compute.  They are issued for
-  // every lane: past the lattice they read padding or, past the clone,
-  // zeros (raw buffer range check), and nobody uses them; a conditional
-  // load would make the compiler carry the old values along and wait.
+  // every lane and every step; past the last chunk they repeat the last
+  // chunk's loads (served from L2, unused), and lanes past the lattice read
+  // padding or, past the clone, zeros (raw buffer range check).  A
+  // conditional load would make the compiler carry the old values along
+  // and wait.
   // With PF > 1 the steps rotate through PF register sets (the loop body is
   // unrolled PF times) and the loads are issued on every step, so the wait
   // before a set's use counts the other sets' loads as still in flight;
@@ -1116,8 +1118,8 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
   R pre[PF][9];
 #pragma unroll
   for (int u = 0; u < PF; u++)
-    pull_issue(g, wave_of(I, min(u, nch) * bpc + wblk), lane, tid + u * CH,
-               src, pc, pre[u]);
+    pull_issue(g, wave_of(I, min(u, nch - 1) * bpc + wblk), lane,
+               min(u, nch - 1) * CH + tid, src, pc, pre[u]);
   auto step = [&](int c, R (&pre)[9]) {
     pc.renew();
     if (PF > 1 || c < nch) {
@@ -1130,8 +1132,9 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
       asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]), "+v"(f[3]),
                    "+v"(f[4]), "+v"(f[5]), "+v"(f[6]), "+v"(f[7]), "+v"(f[8])
                    :: "memory");
-      pull_issue(g, wave_of(I, min(c + PF, nch) * bpc + wblk), lane,
-                 s + PF * CH, src, pc, pre);
+      const int cp = min(c + PF, nch - 1);
+      pull_issue(g, wave_of(I, cp * bpc + wblk), lane, cp * CH + tid, src, pc,
+                 pre);
       if (c < nch && s < g.plane)
         site_finish<MODE, ST, false>(g, wave_of(I, c * bpc + wblk), lane, s, f,
                                      src, r0, pc, drag, dv, tau, I, kBoth);
