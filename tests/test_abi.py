@@ -58,6 +58,15 @@ def test_create_rejects_invalid_configurations(lbc):
     assert "threads" in _bad(lbc, threads=100)
     assert "sites per thread" in _bad(lbc, threads=256, vec=2)
     assert "non-finite" in _bad(lbc, bias=float("nan"))
+    # forcing mode and macroscopic output
+    assert "forced_steps" in _bad(lbc, forced_steps=11)
+    assert "forced_steps" in _bad(lbc, forced_steps=-1)
+    assert "LBC_MODE_REF" in _bad(lbc, forced_steps=3)  # channel mode
+    assert "field_sigma" in _bad(lbc, mode=lbc.MODE_REF, forced_steps=3,
+                                 field_sigma=-1.0)
+    assert "field_eps" in _bad(lbc, mode=lbc.MODE_REF, forced_steps=3,
+                               field_eps=float("inf"))
+    assert "macro_every" in _bad(lbc, macro_every=-2)
 
 
 def test_null_context_calls_fail_cleanly(lbc):

@@ -186,3 +186,39 @@ def test_forced_advance_graph_matches_stepwise(lbc):
         Ra, Rb = a.R_history(0), b.R_history(0)
         assert len(Ra) == 3 and np.array_equal(Ra, Rb)
         assert np.all(np.isfinite(Ra)) and np.all(Ra > 0)
+
+
+def test_forced_and_uniform_instances_batched(O, lbc):
+    """A forced and an unforced instance in one context (one launch per
+    interval) give the same bits as each in a context of its own."""
+    a = lbc.ref_box(4, 0.55, 0.05, 3, 10, alpha=1.0, zero_mean=False)
+    a.forced_steps = 4
+    a.field_eps = 0.5
+    b = lbc.ref_box(6, 0.6, 0.05, 2, 12, alpha=2.0, zero_mean=False)
+    ga = O.ref_box_geom(4, 0.55, 0.05)
+    gb = O.ref_box_geom(6, 0.6, 0.05)
+    sa = _states(O, ga, 3, 31)
+    sb = _states(O, gb, 2, 32)
+    maps = np.random.default_rng(33).standard_normal((3, ga.Lx, ga.Ly))
+    print("case: forced + uniform instances batched, REF F64", flush=True)
+
+    def run(specs, loads):
+        with lbc.Sampler(specs, mode=lbc.MODE_REF, threads=256, passes=2) as s:
+            for i, (st, fmap) in enumerate(loads):
+                s.load_states(i, 0, st)
+                if fmap is not None:
+                    s.set_field_maps(i, fmap)
+            s.run_interval()
+            return [(s.store_states(i), s.drag(i)) for i in range(len(specs))]
+
+    both = run([a, b], [(sa, maps), (sb, None)])
+    alone_a = run([a], [(sa, maps)])[0]
+    alone_b = run([b], [(sb, None)])[0]
+    for got, want in ((both[0], alone_a), (both[1], alone_b)):
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    # and the forced instance is the oracle's forced interval
+    for j in range(3):
+        ref = sa[j].copy()
+        _, A = O.run_steps_forced(ga, ref, 4, 10, maps[j], 0.5, a.force_scale,
+                                  a.force_shift, a.dt)
+        assert np.array_equal(both[0][0][j], ref) and np.array_equal(both[0][1][j], A)
