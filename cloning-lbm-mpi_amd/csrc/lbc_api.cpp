@@ -82,6 +82,37 @@ struct lbc_ctx {
 
 namespace {
 
+// Makes `dev` the calling thread's current HIP device for one API call and
+// gives the caller's device back when the call returns: a context's buffers,
+// stream, events, graphs and FFT plans all belong to the context's device,
+// and allocations, synchronous copies and event creation follow the current
+// device, not the stream.
+class DeviceScope {
+ public:
+  explicit DeviceScope(int dev) {
+    int cur = -1;
+    if (dev < 0 || hipGetDevice(&cur) != hipSuccess) return;
+    if (cur == dev) {
+      ok_ = true;
+      return;
+    }
+    ok_ = hipSetDevice(dev) == hipSuccess;
+    if (ok_) prev_ = cur;
+  }
+  ~DeviceScope() {
+    if (prev_ >= 0) (void)hipSetDevice(prev_);
+  }
+  DeviceScope(const DeviceScope&) = delete;
+  DeviceScope& operator=(const DeviceScope&) = delete;
+  bool ok() const { return ok_; }
+
+ private:
+  int prev_ = -1;
+  bool ok_ = false;
+};
+
+#define ON_CTX_DEVICE(c) DeviceScope dev_scope_((c) ? (c)->cfg.device : -1)
+
 int fail(lbc_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   g_err = msg;
@@ -472,6 +503,7 @@ int lbc_step_shape(const lbc_ctx* ctx) {
 
 void lbc_destroy(lbc_ctx* c) {
   if (!c) return;
+  ON_CTX_DEVICE(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (int i = 0; i < 2; i++) {
     if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
@@ -504,8 +536,8 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     return bail(fail(nullptr, LBC_E_HIP, "no HIP device available"));
   if (cfg->device < 0 || cfg->device >= ndev)
     return bail(fail(nullptr, LBC_E_ARG, "device ordinal out of range"));
-  if (hipSetDevice(cfg->device) != hipSuccess)
-    return bail(fail(nullptr, LBC_E_HIP, "hipSetDevice failed"));
+  DeviceScope scope(cfg->device);
+  if (!scope.ok()) return bail(fail(nullptr, LBC_E_HIP, "hipSetDevice failed"));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(nullptr, LBC_E_HIP, "hipStreamCreate failed"));
 
@@ -677,6 +709,7 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
 }
 
 int lbc_load_states(lbc_ctx* c, int inst, int clone0, int n, const double* aos) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   const lbc_instance& s = c->inst[inst];
   if (!aos || n < 1 || clone0 < 0 || clone0 + n > s.n_clones)
@@ -692,6 +725,7 @@ int lbc_load_states(lbc_ctx* c, int inst, int clone0, int n, const double* aos) 
 
 static int store_which(lbc_ctx* c, int inst, int which, int clone0, int n,
                        double* aos) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   const lbc_instance& s = c->inst[inst];
   if (!aos || n < 1 || clone0 < 0 || clone0 + n > s.n_clones)
@@ -710,6 +744,7 @@ int lbc_store_states(lbc_ctx* c, int inst, int clone0, int n, double* aos) {
 }
 
 int lbc_init_equilibrium(lbc_ctx* c, int inst, double amp) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   if (!std::isfinite(amp) || amp < 0) return fail(c, LBC_E_ARG, "bad amplitude");
   lbc::Launch L = make_launch(c);
@@ -720,6 +755,7 @@ int lbc_init_equilibrium(lbc_ctx* c, int inst, double amp) {
 }
 
 int lbc_set_snapshots(lbc_ctx* c, int inst, const double* aos) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   const InstDev& d = c->idh[inst];
   if (d.n_snapshots <= 0)
@@ -735,6 +771,7 @@ int lbc_set_snapshots(lbc_ctx* c, int inst, const double* aos) {
 }
 
 int lbc_set_marks(lbc_ctx* c, int inst, const uint8_t* marks) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   if (!marks) return fail(c, LBC_E_ARG, "null marks");
   HIPC(c, hipMemcpyAsync(c->idh[inst].marks, marks, c->inst[inst].n_clones,
@@ -744,6 +781,7 @@ int lbc_set_marks(lbc_ctx* c, int inst, const uint8_t* marks) {
 }
 
 int lbc_get_marks(lbc_ctx* c, int inst, uint8_t* marks) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   if (!marks) return fail(c, LBC_E_ARG, "null marks");
   HIPC(c, hipMemcpyAsync(marks, c->idh[inst].marks, c->inst[inst].n_clones,
@@ -753,6 +791,7 @@ int lbc_get_marks(lbc_ctx* c, int inst, uint8_t* marks) {
 }
 
 int lbc_run_interval(lbc_ctx* c, const double* prefactors) {
+  ON_CTX_DEVICE(c);
   if (!c) return fail(nullptr, LBC_E_ARG, "null context");
   bool philox = true;
   if (prefactors && c->cfg.mode == LBC_MODE_REF) {
@@ -777,6 +816,7 @@ int lbc_run_interval(lbc_ctx* c, const double* prefactors) {
 }
 
 int lbc_get_drag(lbc_ctx* c, int inst, double* A) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   if (!A) return fail(c, LBC_E_ARG, "null output");
   const InstDev& d = c->idh[inst];
@@ -787,6 +827,7 @@ int lbc_get_drag(lbc_ctx* c, int inst, double* A) {
 }
 
 int lbc_get_logweights(lbc_ctx* c, int inst, double* obs, double* logw) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   const InstDev& d = c->idh[inst];
   if (obs)
@@ -801,6 +842,7 @@ int lbc_get_logweights(lbc_ctx* c, int inst, double* obs, double* logw) {
 
 int lbc_select(lbc_ctx* c, int inst, const double* weights, const double* u,
                const int32_t* rint, lbc_selection* out) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   const InstDev& d = c->idh[inst];
   const size_t n = (size_t)d.n_clones;
@@ -856,6 +898,7 @@ int lbc_select(lbc_ctx* c, int inst, const double* weights, const double* u,
 }
 
 int lbc_gather(lbc_ctx* c) {
+  ON_CTX_DEVICE(c);
   if (!c) return fail(nullptr, LBC_E_ARG, "null context");
   for (int i = 0; i < c->cfg.n_instances; i++)
     if (!c->selected[i])
@@ -910,6 +953,7 @@ int capture_step(lbc_ctx* c) {
 }  // namespace
 
 int lbc_advance(lbc_ctx* c, int n_intervals) {
+  ON_CTX_DEVICE(c);
   if (!c) return fail(nullptr, LBC_E_ARG, "null context");
   if (n_intervals < 0) return fail(c, LBC_E_ARG, "negative interval count");
   if (!c->gexec[0] || !c->gexec[1])
@@ -941,6 +985,7 @@ int lbc_advance(lbc_ctx* c, int n_intervals) {
 }
 
 int lbc_get_R_history(lbc_ctx* c, int inst, double* R, int max_n, int* n_out) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   Counters h;
   HIPC(c, hipMemcpyAsync(&h, c->counters, sizeof h, hipMemcpyDeviceToHost, c->stream));
@@ -960,6 +1005,7 @@ int lbc_get_R_history(lbc_ctx* c, int inst, double* R, int max_n, int* n_out) {
 
 int lbc_get_scgf(lbc_ctx* c, int inst, double dT, double* lam, int max_n,
                  int* n_out) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   if (!(dT > 0.0)) return fail(c, LBC_E_ARG, "dT must be positive");
   Counters h;
@@ -977,6 +1023,7 @@ int lbc_get_scgf(lbc_ctx* c, int inst, double dT, double* lam, int max_n,
 }
 
 int lbc_set_field_maps(lbc_ctx* c, int inst, const double* maps) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   const InstDev& d = c->idh[inst];
   if (d.forced_steps <= 0)
@@ -997,6 +1044,7 @@ int lbc_set_field_maps(lbc_ctx* c, int inst, const double* maps) {
 }
 
 int lbc_get_field_maps(lbc_ctx* c, int inst, double* maps) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   const InstDev& d = c->idh[inst];
   if (d.forced_steps <= 0)
@@ -1016,6 +1064,7 @@ int lbc_get_field_maps(lbc_ctx* c, int inst, double* maps) {
 }
 
 int lbc_get_macro(lbc_ctx* c, int inst, int clone0, int n, double* out) {
+  ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
   const InstDev& d = c->idh[inst];
   if (d.macro_every <= 0)
@@ -1030,12 +1079,14 @@ int lbc_get_macro(lbc_ctx* c, int inst, int clone0, int n, double* out) {
 }
 
 int lbc_sync(lbc_ctx* c) {
+  ON_CTX_DEVICE(c);
   if (!c) return fail(nullptr, LBC_E_ARG, "null context");
   HIPC(c, hipStreamSynchronize(c->stream));
   return LBC_OK;
 }
 
 int lbc_step_kernel_ms(lbc_ctx* c, double* avg_ms, int* n_launches) {
+  ON_CTX_DEVICE(c);
   if (!c || !avg_ms) return fail(c, LBC_E_ARG, "null argument");
   HIPC(c, hipStreamSynchronize(c->stream));
   double tot = 0;
