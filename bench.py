@@ -16,6 +16,10 @@ scaled-cumulant errors are measured against FP64 over several cloning
 steps from identical developed states, with both runs making the FP64
 run's selections (shadow mode).
 
+The copy kernel is also timed on its own, with HIP events, on selections
+that copy half the clones ("gather_roofline"): at this workload the weights
+are close enough that a cloning step copies only a few clones.
+
 Prints one JSON line.  Only N=1 is supported (one GPU).
 """
 from __future__ import annotations
@@ -51,10 +55,11 @@ def spec_for(n_clones: int, steps: int | None = None) -> lbc.InstanceSpec:
 
 
 def time_storage(storage: int, K: int, W: int, seed: int = 0,
-                 shape=None) -> dict:
-    """W untimed warm-up intervals then K timed ones (graph replays).
-    shape: (threads, sites per thread, non-temporal, LBM passes per HBM
-    sweep) or None (tuned at context creation)."""
+                 shape=None, G: int = 3) -> dict:
+    """W untimed warm-up intervals then K timed ones (graph replays), then
+    G selections and copies of the last interval's clones with the copy
+    kernel timed alone.  shape: (threads, sites per thread, non-temporal,
+    LBM passes per HBM sweep) or None (tuned at context creation)."""
     spec = spec_for(WORKLOAD["n_clones"])
     kw = {}
     if shape:
@@ -76,6 +81,20 @@ def time_storage(storage: int, K: int, W: int, seed: int = 0,
         vec = s.vec
         nt = s.nt
         passes = s.passes
+        # the copy kernel on its own, through the API (the kernels the graph
+        # replays), each launch timed with events: G selections on the last
+        # interval's weights, then G on weights that copy every other clone
+        # (1 and 1e-9 alternating: nb = 2, 0, 2, 0, ...), which loads the
+        # kernel with N/2 copies -- at this workload the weights are so close
+        # that a cloning step copies only a few clones
+        gather, gather_half = [], []
+        wsyn = np.where(np.arange(spec.n_clones) % 2 == 0, 1.0, 1e-9)
+        for runs, w in ((gather, None), (gather_half, wsyn)):
+            for _ in range(G):
+                sel = s.select(0, weights=w)
+                s.gather()
+                gms, gbytes = s.gather_kernel_ms()
+                runs.append((gms, gbytes, len(sel.pairs)))
     secs = t1 - t0
     lups = spec.n_clones * spec.sites * spec.steps * K
     bpl = lbc.bytes_per_lup(storage)
@@ -97,7 +116,23 @@ def time_storage(storage: int, K: int, W: int, seed: int = 0,
         nontemporal=nt,
         passes_per_sweep=passes,
         logR_last=float(math.log(R[-1])) if len(R) else None,
+        gather_workload=gather_stats(gather),
+        gather=gather_stats(gather_half),
     )
+
+
+def gather_stats(runs: list) -> dict | None:
+    """Copy kernel (k_gather): mean device time and population bytes over
+    the timed gathers; achieved = bytes read + written / time."""
+    if not runs:
+        return None
+    ms = sum(r[0] for r in runs) / len(runs)
+    nbytes = sum(r[1] for r in runs) / len(runs)
+    gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else float("nan")
+    return dict(kernel="k_gather", launches=len(runs),
+                copies=[int(r[2]) for r in runs],
+                avg_ms=ms, bytes_per_launch=nbytes, achieved_gbs=gbs,
+                peak=HBM_PEAK_GBS, frac=gbs / HBM_PEAK_GBS)
 
 
 def log_mean_exp(v: np.ndarray) -> float:
@@ -406,6 +441,7 @@ def main() -> int:
             "lbm_steps_per_interval": WORKLOAD["steps"],
         },
         "roofline": roofline(head),
+        "gather_roofline": head.get("gather"),
         "cpu_baseline": cpu,
         "paths": runs,
         "accuracy_vs_fp64": acc,

@@ -78,6 +78,11 @@ struct lbc_ctx {
   std::vector<hipfftHandle> fft;      // batched in-place 2-D Z2Z plans
   std::vector<int> fchunk;            // clones per FFT batch
   std::vector<char> maps_set;         // next interval uses injected maps
+  // copy kernel of the last lbc_gather: events around it, bytes it moved
+  std::vector<int> last_copies;       // per instance, from the last lbc_select
+  hipEvent_t gev[2] = {nullptr, nullptr};
+  bool gather_timed = false;
+  double gather_bytes = 0.0;
 };
 
 namespace {
@@ -510,6 +515,8 @@ void lbc_destroy(lbc_ctx* c) {
     if (c->graphs[i]) (void)hipGraphDestroy(c->graphs[i]);
   }
   for (auto e : c->ev) (void)hipEventDestroy(e);
+  for (auto e : c->gev)
+    if (e) (void)hipEventDestroy(e);
   for (auto p : c->fft)
     if (p) (void)hipfftDestroy(p);
   for (void* p : c->allocs) (void)hipFree(p);
@@ -549,6 +556,7 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
   c->fft.assign(ni, nullptr);
   c->fchunk.assign(ni, 0);
   c->maps_set.assign(ni, 0);
+  c->last_copies.assign(ni, 0);
   int clone_base = 0;
   for (int i = 0; i < ni; i++) {
     const lbc_instance& s = c->inst[i];
@@ -881,6 +889,7 @@ int lbc_select(lbc_ctx* c, int inst, const double* weights, const double* u,
                              hipMemcpyDeviceToHost, c->stream));
   }
   HIPC(c, hipStreamSynchronize(c->stream));
+  c->last_copies[inst] = info[0];
   if (out) {
     out->n_pairs = info[0];
     out->n_rint_used = info[1];
@@ -903,8 +912,30 @@ int lbc_gather(lbc_ctx* c) {
   for (int i = 0; i < c->cfg.n_instances; i++)
     if (!c->selected[i])
       return fail(c, LBC_E_STATE, "lbc_gather: every instance needs lbc_select first");
+  if (!c->gev[0]) {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    for (int k = 0; k < 2; k++) {
+      const hipError_t he = hipEventCreate(&e[k]);
+      if (he != hipSuccess) {
+        if (e[0]) (void)hipEventDestroy(e[0]);
+        return fail(c, LBC_E_HIP,
+                    std::string("hipEventCreate: ") + hipGetErrorString(he));
+      }
+    }
+    c->gev[0] = e[0];
+    c->gev[1] = e[1];
+  }
   lbc::Launch L = make_launch(c);
+  HIPC(c, hipEventRecord(c->gev[0], c->stream));
   HIPC(c, lbc::launch_gather(L, seed_lo(c), seed_hi(c)));
+  HIPC(c, hipEventRecord(c->gev[1], c->stream));
+  // every copy reads the parent's populations and writes the destination's
+  double bytes = 0.0;
+  for (int i = 0; i < c->cfg.n_instances; i++)
+    bytes += 2.0 * 9.0 * c->idh[i].plane * (double)elem_bytes(c->cfg.storage) *
+             c->last_copies[i];
+  c->gather_bytes = bytes;
+  c->gather_timed = true;
   for (int i = 0; i < c->cfg.n_instances; i++) c->selected[i] = 0;
   HIPC(c, hipStreamSynchronize(c->stream));
   return LBC_OK;
@@ -1106,6 +1137,18 @@ int lbc_step_kernel_ms(lbc_ctx* c, double* avg_ms, int* n_launches) {
   }
   *avg_ms = n ? tot / n : 0.0;
   if (n_launches) *n_launches = n;
+  return LBC_OK;
+}
+
+int lbc_gather_kernel_ms(lbc_ctx* c, double* ms, double* bytes) {
+  ON_CTX_DEVICE(c);
+  if (!c || !ms) return fail(c, LBC_E_ARG, "null argument");
+  if (!c->gather_timed) return fail(c, LBC_E_STATE, "no lbc_gather call to time");
+  HIPC(c, hipEventSynchronize(c->gev[1]));
+  float t = 0.0f;
+  HIPC(c, hipEventElapsedTime(&t, c->gev[0], c->gev[1]));
+  *ms = t;
+  if (bytes) *bytes = c->gather_bytes;
   return LBC_OK;
 }
 

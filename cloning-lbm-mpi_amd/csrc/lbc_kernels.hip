@@ -275,9 +275,10 @@ __global__ void __launch_bounds__(1024)
 }
 
 // ---------------------------------------------------------------- gather
-// Destination slot k <- parent labels[k], site by site, with Gaussian
-// noise on the copy in channel mode.  The interval counter has already been
-// advanced by the selection.
+// Destination slot k <- parent labels[k]: in channel mode with noise site
+// by site, adding Gaussian noise to every population outside the solid
+// interior; otherwise as a plain block copy.  The interval counter has
+// already been advanced by the selection.
 template <int MODE, int ST>
 __global__ void __launch_bounds__(256)
     k_gather(const InstDev* __restrict__ insts,
@@ -297,32 +298,41 @@ __global__ void __launch_bounds__(256)
   const int Lx = I->Lx, plane = I->plane;
   const double sigma = I->sigma;
   const bool noisy = (MODE == kModeChannel) && sigma > 0.0;
+  if (!noisy) {
+    // a plain copy of the whole clone block, padding included: clone_stride
+    // is a multiple of 128 elements, so the block is a whole number of
+    // 16-byte vectors at 256-byte aligned addresses
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const v4u* __restrict__ vs = (const v4u*)src;
+    v4u* __restrict__ vd = (v4u*)dst;
+    const long long nv = I->clone_stride * (long long)sizeof(S) / 16;
+    for (long long i = threadIdx.x; i < nv; i += blockDim.x) vd[i] = vs[i];
+    return;
+  }
   for (int s = threadIdx.x; s < plane; s += blockDim.x) {
     typename T::C f[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) f[q] = T::ld(src + q * plane + s);
-    if (noisy) {
-      const int x = s % Lx, y = s / Lx;
-      const bool interior = x > I->xmin && x < I->xmax && y > I->ymin && y < I->ymax;
-      if (!interior) {
+    const int x = s % Lx, y = s / Lx;
+    const bool interior = x > I->xmin && x < I->xmax && y > I->ymin && y < I->ymax;
+    if (!interior) {
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-          const U4 r = philox(3u * (uint32_t)s + (uint32_t)j, (uint32_t)wi.clone,
-                              (uint32_t)interval, (uint32_t)wi.inst, seed_lo,
-                              seed_hi);
-          if (!T::shifted) {
-            double nrm[4];
-            box_muller_d(r, nrm);
+      for (int j = 0; j < 3; j++) {
+        const U4 r = philox(3u * (uint32_t)s + (uint32_t)j, (uint32_t)wi.clone,
+                            (uint32_t)interval, (uint32_t)wi.inst, seed_lo,
+                            seed_hi);
+        if (!T::shifted) {
+          double nrm[4];
+          box_muller_d(r, nrm);
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-              if (4 * j + i < 9) f[4 * j + i] += sigma * nrm[i];
-          } else {
-            float nrm[4];
-            box_muller_f(r, nrm);
+          for (int i = 0; i < 4; i++)
+            if (4 * j + i < 9) f[4 * j + i] += sigma * nrm[i];
+        } else {
+          float nrm[4];
+          box_muller_f(r, nrm);
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-              if (4 * j + i < 9) f[4 * j + i] += (float)sigma * nrm[i];
-          }
+          for (int i = 0; i < 4; i++)
+            if (4 * j + i < 9) f[4 * j + i] += (float)sigma * nrm[i];
         }
       }
     }

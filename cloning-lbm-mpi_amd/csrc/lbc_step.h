@@ -86,7 +86,9 @@ struct Traits<kBF16> {
 };
 
 // ---------------------------------------------------------------- Philox
-// Philox4x32-10 (Salmon et al., SC'11).
+// Philox4x32-10 (Salmon et al., SC'11).  Each round's two 32x32 -> 64-bit
+// products are one v_mad_u64_u32 each (low and high word together) and
+// its two three-way XORs one v_bitop3_b32 each (truth table 0x96).
 struct U4 {
   uint32_t x, y, z, w;
 };
@@ -99,13 +101,16 @@ __device__ inline U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
       k0 += 0x9E3779B9u;
       k1 += 0xBB67AE85u;
     }
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 =
+        __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+    const uint32_t n2 =
+        __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
     c0 = n0;
-    c1 = lo1;
+    c1 = (uint32_t)p1;
     c2 = n2;
-    c3 = lo0;
+    c3 = (uint32_t)p0;
   }
   return U4{c0, c1, c2, c3};
 }
@@ -131,19 +136,22 @@ __device__ inline void box_muller_d(U4 r, double n[4]) {
   }
 }
 
+// FP32 Box-Muller (noise on shifted storage) on gfx950's transcendental
+// instructions: v_log_f32 (base 2), v_sqrt_f32, and v_sin_f32 / v_cos_f32,
+// which take the angle in revolutions, so cos(2 pi u2) is one instruction.
+// Each is good to about 1e-6 relative, far inside the FP32 (or narrower)
+// populations the noise is added to.
 __device__ inline void box_muller_f(U4 r, float n[4]) {
-  const float two32inv = 2.3283064365386963e-10f;
-  const float twopi = 6.2831853071795865f;
   const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
   for (int h = 0; h < 2; h++) {
-    float u1 = ((float)(rr[2 * h] >> 8) + 1.0f) * (two32inv * 256.0f);
-    float u2 = (float)(rr[2 * h + 1] >> 8) * (two32inv * 256.0f);
-    float rad = sqrtf(-2.0f * logf(u1));
-    float s, c;
-    sincosf(twopi * u2, &s, &c);
-    n[2 * h] = rad * c;
-    n[2 * h + 1] = rad * s;
+    const float u1 = ((float)(rr[2 * h] >> 8) + 1.0f) * 0x1p-24f;  // (0, 1]
+    const float u2 = (float)(rr[2 * h + 1] >> 8) * 0x1p-24f;       // [0, 1)
+    // -2 ln(u1) = -2 ln(2) log2(u1)
+    const float rad =
+        __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+    n[2 * h] = rad * __builtin_amdgcn_cosf(u2);
+    n[2 * h + 1] = rad * __builtin_amdgcn_sinf(u2);
   }
 }
 

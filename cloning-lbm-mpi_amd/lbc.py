@@ -38,7 +38,8 @@ EXPORTS = (
     "lbc_set_marks", "lbc_get_marks", "lbc_run_interval", "lbc_get_drag",
     "lbc_get_logweights", "lbc_select", "lbc_gather", "lbc_advance",
     "lbc_get_R_history", "lbc_get_scgf", "lbc_set_field_maps",
-    "lbc_get_field_maps", "lbc_get_macro", "lbc_sync", "lbc_step_kernel_ms", "lbc_threads",
+    "lbc_get_field_maps", "lbc_get_macro", "lbc_sync", "lbc_step_kernel_ms",
+    "lbc_gather_kernel_ms", "lbc_threads",
     "lbc_sites_per_thread", "lbc_step_shape", "lbc_step_host",
 )
 
@@ -121,6 +122,7 @@ def lib() -> C.CDLL:
     L.lbc_get_macro.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
     L.lbc_sync.argtypes = [vp]
     L.lbc_step_kernel_ms.argtypes = [vp, dp, C.POINTER(C.c_int)]
+    L.lbc_gather_kernel_ms.argtypes = [vp, dp, dp]
     L.lbc_threads.argtypes = [vp]
     L.lbc_sites_per_thread.argtypes = [vp]
     L.lbc_step_shape.argtypes = [vp]
@@ -468,6 +470,15 @@ class Sampler:
         self._check(lib().lbc_step_kernel_ms(self._h, C.byref(ms), C.byref(n)),
                     "lbc_step_kernel_ms")
         return ms.value, n.value
+
+    def gather_kernel_ms(self):
+        """(device ms, population bytes moved) of the last gather's copy
+        kernel."""
+        ms = C.c_double()
+        b = C.c_double()
+        self._check(lib().lbc_gather_kernel_ms(self._h, C.byref(ms), C.byref(b)),
+                    "lbc_gather_kernel_ms")
+        return ms.value, b.value
 
 
 def step_host(mode: int, spec: InstanceSpec, fin: np.ndarray):

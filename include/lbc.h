@@ -232,6 +232,14 @@ int lbc_sync(lbc_ctx* ctx);
  * stream, and the number of launches it averaged. */
 int lbc_step_kernel_ms(lbc_ctx* ctx, double* avg_ms, int* n_launches);
 
+/* Device time (ms) of the copy kernel of the last lbc_gather call, from HIP
+ * events on the context's stream, and the population bytes it moved: each
+ * copy reads the parent's 9*Lx*Ly stored populations and writes the
+ * destination's, with the copies counted by every instance's last
+ * lbc_select.  Instrumentation for the copy's HBM roofline, like
+ * lbc_step_kernel_ms; the reference times nothing. */
+int lbc_gather_kernel_ms(lbc_ctx* ctx, double* ms, double* bytes);
+
 /* Workgroup size and sites per thread the context uses for the step
  * kernel. */
 int lbc_threads(const lbc_ctx* ctx);

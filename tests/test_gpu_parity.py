@@ -181,6 +181,8 @@ def test_gather_copies_parents_exactly(O, lbc):
     print("case: gather REF F64", flush=True)
     with lbc.Sampler(spec, mode=lbc.MODE_REF, threads=256) as s:
         s.load_states(0, 0, states)
+        with pytest.raises(lbc.LbcError):
+            s.gather_kernel_ms()  # nothing gathered yet
         w = np.array([5.0, 0.1, 0.1, 3.0, 0.1, 1.0, 0.1, 0.1])
         u = rng.random(N)
         r = rng.integers(0, 2**31 - 1, N).astype(np.int32)
@@ -189,8 +191,13 @@ def test_gather_copies_parents_exactly(O, lbc):
         assert np.array_equal(sel.labels, ref["labels"])
         assert len(sel.pairs) > 0
         s.gather()
+        gms, gbytes = s.gather_kernel_ms()
         got = s.store_states(0)
         marks = s.get_marks(0)
+    # the copy kernel's bytes: parent read + destination written, FP64
+    copies = int(np.sum(sel.labels != np.arange(N)))
+    assert copies == len(sel.pairs)
+    assert gbytes == 2 * 9 * g.Lx * g.Ly * 8 * copies and gms > 0
     for k in range(N):
         assert np.array_equal(got[k], states[sel.labels[k]])
         assert marks[k] == (sel.labels[k] != k)
@@ -216,6 +223,44 @@ def test_channel_noise_on_copies(O, lbc):
             O.noise(g, exp, 1e-4, 123, k, 1, 0)
             np.testing.assert_allclose(after[k], exp, rtol=0, atol=1e-17)
             assert not np.array_equal(after[k], before[sel.labels[k]])
+        else:
+            assert np.array_equal(after[k], exp)
+
+
+def test_channel_noise_on_copies_f32(O, lbc):
+    """FP32-shifted storage draws the same Philox words as FP64, keeps 24
+    bits of each and turns them into normals with the hardware
+    log2/sqrt/sin/cos.  Each copied population's noise is the FP64 oracle's
+    to within FP32 rounding of the shifted value (< 4e-9 here) plus 1e-3 of
+    the noise (the 24-bit uniforms; ~1e-7 typically, more only where the
+    radius draw is tiny); a wrong or missing draw is off by ~100%."""
+    spec = lbc.channel(48, 24, 6, 12, 12, 0.55, 0.05, 6, 2, k=0.0, sigma=1e-4)
+    g = geom_of(O, spec, O.MODE_CHANNEL)
+    print("case: gather CHANNEL F32 with noise", flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=lbc.STORE_F32,
+                     seed=123, threads=256) as s:
+        s.init_equilibrium(0, 1e-3)
+        before = s.store_states(0)
+        w = np.array([4.0, 0.01, 0.01, 1.0, 0.5, 0.5])
+        u = np.full(6, 0.5)
+        r = np.arange(6, dtype=np.int32)
+        sel = s.select(0, weights=w, u=u, rint=r)
+        s.gather()
+        after = s.store_states(0)
+    assert len(sel.pairs) > 0
+    interior = np.zeros((g.Lx, g.Ly), bool)
+    interior[g.xmin + 1:g.xmax, g.ymin + 1:g.ymax] = True
+    for k in range(6):
+        exp = before[sel.labels[k]].copy()
+        if sel.labels[k] != k:
+            O.noise(g, exp, 1e-4, 123, k, 1, 0)
+            got_n = after[k] - before[sel.labels[k]]
+            ref_n = exp - before[sel.labels[k]]
+            err = np.abs(got_n - ref_n) - (8e-9 + 1e-3 * np.abs(ref_n))
+            assert err.max() <= 0, (k, err.max())
+            moved = np.abs(got_n)
+            assert np.all(moved[~interior].max(axis=-1) > 0)
+            assert np.array_equal(after[k][interior], before[sel.labels[k]][interior])
         else:
             assert np.array_equal(after[k], exp)
 
