@@ -20,6 +20,12 @@ The copy kernel is also timed on its own, with HIP events, on selections
 that copy half the clones ("gather_roofline"): at this workload the weights
 are close enough that a cloning step copies only a few clones.
 
+--workload runs one of the other channel workloads instead (the same
+measurements, with the storages listed for it): channel200x50 (16 clones,
+FP64), channel400x100-n512 (512 clones, FP64), batched8 (8 samplers of four
+lattice sizes and two biases in one launch, FP32/FP64/FP16) and
+channel800x200 (2048 clones, 400 steps, FP16/FP64).
+
 Prints one JSON line.  Only N=1 is supported (one GPU).
 """
 from __future__ import annotations
@@ -43,31 +49,87 @@ import lbc  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-WORKLOAD = dict(Lx=400, Ly=100, side=20, cx=100, cy=50, tau=0.53, u_in=0.05,
-                n_clones=4096, steps=200, k=0.005, sigma=1e-4)
+
+def _inst(Lx, Ly, side, cx, cy, tau, u_in, n_clones, steps, k, sigma):
+    return dict(Lx=Lx, Ly=Ly, side=side, cx=cx, cy=cy, tau=tau, u_in=u_in,
+                n_clones=n_clones, steps=steps, k=k, sigma=sigma)
 
 
-def spec_for(n_clones: int, steps: int | None = None) -> lbc.InstanceSpec:
-    w = WORKLOAD
-    return lbc.channel(w["Lx"], w["Ly"], w["side"], w["cx"], w["cy"], w["tau"],
-                       w["u_in"], n_clones, steps or w["steps"], w["k"],
-                       sigma=w["sigma"])
+# Channel workloads (square centred at (cx, cy)); the first storage listed
+# gives the headline value.  The default is the 4096-clone 400x100 sweep
+# over all four storages.
+WORKLOADS = {
+    "channel400x100": dict(
+        desc="channel 400x100, square side 20 at (100,50), tau 0.53, u_in "
+             "0.05, 4096 clones, 200 LBM steps per cloning interval, k 0.005, "
+             "noise sigma 1e-4",
+        instances=[_inst(400, 100, 20, 100, 50, 0.53, 0.05, 4096, 200, 0.005,
+                         1e-4)],
+        storages=["f16", "f64", "f32", "bf16"]),
+    "channel200x50": dict(
+        desc="channel 200x50, square side 10 at (50,25), tau 0.515, u_in 0.05, "
+             "16 clones, 100 LBM steps per cloning interval, k 0.01, noise "
+             "sigma 1e-4, seed 0",
+        instances=[_inst(200, 50, 10, 50, 25, 0.515, 0.05, 16, 100, 0.01,
+                         1e-4)],
+        storages=["f64"]),
+    "channel400x100-n512": dict(
+        desc="channel 400x100, square side 20 at (100,50), tau 0.53, u_in "
+             "0.05, 512 clones, 200 LBM steps per cloning interval, k 0.005, "
+             "noise sigma 1e-4",
+        instances=[_inst(400, 100, 20, 100, 50, 0.53, 0.05, 512, 200, 0.005,
+                         1e-4)],
+        storages=["f64"]),
+    "batched8": dict(
+        desc="8 samplers in one launch: {200x50, 300x75, 400x100, 600x150} x "
+             "k {0.002, 0.01}, square side Ly/5 at (Lx/4, Ly/2), tau 0.53, "
+             "u_in 0.05, {1024, 512, 384, 128} clones per lattice (4096 in "
+             "all), 200 LBM steps per cloning interval, noise sigma 1e-4",
+        instances=[_inst(Lx, Ly, Ly // 5, Lx // 4, Ly // 2, 0.53, 0.05, n, 200,
+                         k, 1e-4)
+                   for k in (0.002, 0.01)
+                   for (Lx, Ly, n) in ((200, 50, 1024), (300, 75, 512),
+                                       (400, 100, 384), (600, 150, 128))],
+        storages=["f32", "f64", "f16"]),
+    "channel800x200": dict(
+        desc="channel 800x200, square side 40 at (200,100), tau 0.52, u_in "
+             "0.04, 2048 clones, 400 LBM steps per cloning interval, k 0.002, "
+             "noise sigma 5e-5",
+        instances=[_inst(800, 200, 40, 200, 100, 0.52, 0.04, 2048, 400, 0.002,
+                         5e-5)],
+        storages=["f16", "f64"]),
+}
+DEFAULT_WORKLOAD = "channel400x100"
 
 
-def time_storage(storage: int, K: int, W: int, seed: int = 0,
+def specs_of(workload: str, max_clones: int | None = None
+             ) -> list[lbc.InstanceSpec]:
+    """The workload's instances; max_clones caps each instance's clones."""
+    out = []
+    for w in WORKLOADS[workload]["instances"]:
+        n = w["n_clones"] if max_clones is None else min(max_clones,
+                                                         w["n_clones"])
+        out.append(lbc.channel(w["Lx"], w["Ly"], w["side"], w["cx"], w["cy"],
+                               w["tau"], w["u_in"], n, w["steps"], w["k"],
+                               sigma=w["sigma"]))
+    return out
+
+
+def time_storage(storage: int, K: int, W: int, specs: list, seed: int = 0,
                  shape=None, G: int = 3) -> dict:
     """W untimed warm-up intervals then K timed ones (graph replays), then
     G selections and copies of the last interval's clones with the copy
     kernel timed alone.  shape: (threads, sites per thread, non-temporal,
     LBM passes per HBM sweep) or None (tuned at context creation)."""
-    spec = spec_for(WORKLOAD["n_clones"])
     kw = {}
     if shape:
         kw = dict(threads=shape[0], vec=shape[1], nt=bool(shape[2]),
                   passes=shape[3] if len(shape) > 3 else 1)
-    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=storage,
+    ni = len(specs)
+    with lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, storage=storage,
                      seed=seed, **kw) as s:
-        s.init_equilibrium(0, 1e-3)
+        for i in range(ni):
+            s.init_equilibrium(i, 1e-3)
         if W:
             s.advance(W)
         s.sync()
@@ -85,20 +147,24 @@ def time_storage(storage: int, K: int, W: int, seed: int = 0,
         # replays), each launch timed with events: G selections on the last
         # interval's weights, then G on weights that copy every other clone
         # (1 and 1e-9 alternating: nb = 2, 0, 2, 0, ...), which loads the
-        # kernel with N/2 copies -- at this workload the weights are so close
-        # that a cloning step copies only a few clones
+        # kernel with N/2 copies -- at these workloads the weights are so
+        # close that a cloning step copies only a few clones
         gather, gather_half = [], []
-        wsyn = np.where(np.arange(spec.n_clones) % 2 == 0, 1.0, 1e-9)
-        for runs, w in ((gather, None), (gather_half, wsyn)):
+        for runs, half in ((gather, False), (gather_half, True)):
             for _ in range(G):
-                sel = s.select(0, weights=w)
+                copies = 0
+                for i, sp in enumerate(specs):
+                    w = (np.where(np.arange(sp.n_clones) % 2 == 0, 1.0, 1e-9)
+                         if half else None)
+                    copies += len(s.select(i, weights=w).pairs)
                 s.gather()
                 gms, gbytes = s.gather_kernel_ms()
-                runs.append((gms, gbytes, len(sel.pairs)))
+                runs.append((gms, gbytes, copies))
     secs = t1 - t0
-    lups = spec.n_clones * spec.sites * spec.steps * K
+    lup1 = sum(sp.n_clones * sp.sites * sp.steps for sp in specs)
+    lups = lup1 * K
     bpl = lbc.bytes_per_lup(storage)
-    alg_bytes = spec.n_clones * spec.sites * spec.steps * bpl
+    alg_bytes = lup1 * bpl
     achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else float("nan")
     return dict(
         storage=lbc.STORAGE_NAMES[storage],
@@ -106,7 +172,7 @@ def time_storage(storage: int, K: int, W: int, seed: int = 0,
         ms_per_interval=secs / K * 1e3,
         kernel_ms=kms,
         kernel_launches=nk,
-        kernel_clone_mlups=spec.n_clones * spec.sites * spec.steps / (kms * 1e-3) / 1e6,
+        kernel_clone_mlups=lup1 / (kms * 1e-3) / 1e6,
         bytes_per_lup=bpl,
         alg_bytes_per_launch=alg_bytes,
         achieved_gbs=achieved,
@@ -155,52 +221,66 @@ def scgf_last(logR: list, dT: float) -> float:
     return lam
 
 
-def accuracy(storage: int, n: int = 256, warm: int = 10, intervals: int = 8,
-             seed: int = 1) -> dict:
+def accuracy(storage: int, workload: str, n: int = 256, warm: int = 10,
+             intervals: int = 8, seed: int = 1) -> dict:
     """Errors of `storage` against FP64 from identical developed states
-    (n clones after `warm` FP64 intervals): over the first interval (the
-    stated bar), and over `intervals` cloning steps in shadow mode, where
-    both samplers select with the FP64 weights and the same draws, so they
-    copy the same clones (with the same noise) and stay paired while their
-    states drift apart.  Per interval and clone: the interval drag
-    integral's relative error; per interval: the log mean weight error; at
-    the end: the running scaled-cumulant estimate from each sampler's own
-    mean weights."""
-    spec = spec_for(n)
-    dT = spec.steps * spec.dt
+    (at most n clones per instance after `warm` FP64 intervals): over the
+    first interval (the stated bar), and over `intervals` cloning steps in
+    shadow mode, where both samplers select with the FP64 weights and the
+    same draws, so they copy the same clones (with the same noise) and stay
+    paired while their states drift apart.  Per interval and clone: the
+    interval drag integral's relative error; per interval and instance: the
+    log mean weight error; at the end: the running scaled-cumulant estimate
+    from each sampler's own mean weights (the worst instance)."""
+    specs = specs_of(workload, max_clones=n)
+    ni = len(specs)
     rng = np.random.default_rng(seed)
-    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=lbc.STORE_F64,
+    with lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, storage=lbc.STORE_F64,
                      seed=seed) as sw:
-        sw.init_equilibrium(0, 1e-3)
+        for i in range(ni):
+            sw.init_equilibrium(i, 1e-3)
         sw.advance(warm)
-        states = sw.store_states(0)
-    drag, dlr, lr64, lrlp = [], [], [], []
-    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=lbc.STORE_F64,
+        states = [sw.store_states(i) for i in range(ni)]
+    drag, dlr = [], []
+    lr64 = [[] for _ in range(ni)]
+    lrlp = [[] for _ in range(ni)]
+    with lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, storage=lbc.STORE_F64,
                      seed=seed) as s64, \
-            lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=storage,
+            lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, storage=storage,
                         seed=seed) as slp:
-        s64.load_states(0, 0, states)
-        slp.load_states(0, 0, states)
+        for i in range(ni):
+            s64.load_states(i, 0, states[i])
+            slp.load_states(i, 0, states[i])
         for _ in range(intervals):
             s64.run_interval()
             slp.run_interval()
-            o64, w64 = s64.logweights(0)
-            olp, wlp = slp.logweights(0)
-            drag.append(np.abs(olp - o64) / np.abs(o64))
-            lr64.append(log_mean_exp(w64))
-            lrlp.append(log_mean_exp(wlp))
-            dlr.append(abs(lrlp[-1] - lr64[-1]))
-            wsel = np.exp(w64 - w64.max())
-            u = rng.random(n)
-            r = rng.integers(0, 2**31 - 1, n, dtype=np.int64).astype(np.int32)
-            s64.select(0, weights=wsel, u=u, rint=r)
-            slp.select(0, weights=wsel, u=u, rint=r)
+            d_int, l_int = [], []
+            for i, sp in enumerate(specs):
+                o64, w64 = s64.logweights(i)
+                olp, wlp = slp.logweights(i)
+                d_int.append(np.abs(olp - o64) / np.abs(o64))
+                lr64[i].append(log_mean_exp(w64))
+                lrlp[i].append(log_mean_exp(wlp))
+                l_int.append(abs(lrlp[i][-1] - lr64[i][-1]))
+                wsel = np.exp(w64 - w64.max())
+                u = rng.random(sp.n_clones)
+                r = rng.integers(0, 2**31 - 1, sp.n_clones,
+                                 dtype=np.int64).astype(np.int32)
+                s64.select(i, weights=wsel, u=u, rint=r)
+                slp.select(i, weights=wsel, u=u, rint=r)
+            drag.append(np.concatenate(d_int))
+            dlr.append(max(l_int))
             s64.gather()
             slp.gather()
     first = drag[0]
     drag = np.concatenate(drag)
-    psi64, psilp = scgf_last(lr64, dT), scgf_last(lrlp, dT)
-    return dict(storage=lbc.STORAGE_NAMES[storage], clones=n,
+    psi = []
+    for i, sp in enumerate(specs):
+        dT = sp.steps * sp.dt
+        psi.append((scgf_last(lr64[i], dT), scgf_last(lrlp[i], dT)))
+    psi64, psilp = max(psi, key=lambda p: abs(p[1] - p[0]))
+    return dict(storage=lbc.STORAGE_NAMES[storage],
+                clones=sum(sp.n_clones for sp in specs),
                 warmup_intervals=warm,
                 # the first interval, from identical states
                 drag_rel_err_max=float(first.max()),
@@ -218,16 +298,21 @@ def accuracy(storage: int, n: int = 256, warm: int = 10, intervals: int = 8,
                                   if psi64 else None)))
 
 
-def cpu_baseline(sample_clones_per_core: int = 30, steps: int = 200) -> dict:
+def cpu_baseline(workload: str, lups_per_core: float = 2.4e8,
+                 steps: int = 200) -> dict:
     """The CPU oracle (a port of the reference's AoS push step, serial clones
     per worker, one worker thread per core, like one MPI rank per core) on a
-    bounded sample of the same workload."""
+    bounded sample of the workload: clones of its largest lattice, `steps`
+    steps each, about lups_per_core lattice updates per thread (30 clones of
+    400x100)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    w = WORKLOAD
+    w = max(WORKLOADS[workload]["instances"], key=lambda i: i["Lx"] * i["Ly"])
     g = O.channel_geom(w["Lx"], w["Ly"], w["side"], w["cx"], w["cy"], w["tau"],
                        w["u_in"])
+    sample_clones_per_core = max(1, round(lups_per_core /
+                                          (g.Lx * g.Ly * steps)))
     cores = max(1, min(16, os.cpu_count() or 1))
     base = O.init_channel(g, 1e-3, 0, 0)
     states = [np.stack([base] * sample_clones_per_core) for _ in range(cores)]
@@ -256,7 +341,7 @@ def cpu_baseline(sample_clones_per_core: int = 30, steps: int = 200) -> dict:
     out = dict(value=lups / secs / 1e6, unit="clone-MLUPS", cores=cores,
                kind="port",
                sample=(f"{cores} threads x {sample_clones_per_core} clones x "
-                       f"{steps} steps of 400x100 (FP64 AoS push oracle), "
+                       f"{steps} steps of {g.Lx}x{g.Ly} (FP64 AoS push oracle), "
                        f"{secs:.1f} s on {cpu}"))
     cal = calibrate_port(O)
     if cal:
@@ -291,11 +376,13 @@ def calibrate_port(O, L: int = 50, steps: int = 60) -> dict | None:
                 port_over_reference=round(rates["port"] / rates["reference"], 3))
 
 
-def measured_traffic(run: dict) -> int | None:
-    """HBM bytes per step-kernel launch for this storage and number of LBM
-    passes per sweep, from the FETCH_SIZE / WRITE_SIZE summaries under
-    profiles/ (copy-calibrated there), or None when none covers it."""
-    key = f"storage={run['storage']} passes={run['passes_per_sweep']}"
+def measured_traffic(run: dict, workload: str) -> int | None:
+    """HBM bytes per step-kernel launch for this workload, storage and
+    number of LBM passes per sweep, from the FETCH_SIZE / WRITE_SIZE
+    summaries under profiles/ (copy-calibrated there), or None when none
+    covers it."""
+    key = (f"workload={workload} storage={run['storage']} "
+           f"passes={run['passes_per_sweep']}")
     pat = re.compile(r"^traffic_bytes_per_launch (.*): (\d+)\s*$")
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
         with open(path) as f:
@@ -321,13 +408,13 @@ def stream_ceilings() -> dict | None:
     return out or None
 
 
-def roofline(run: dict) -> dict:
+def roofline(run: dict, workload: str) -> dict:
     """HBM roofline of the step kernel.  achieved = algorithmic bytes
     (2*9*b per lattice update) / average kernel time; with P LBM passes
     per HBM sweep the kernel moves about 1/P of that, so the measured
     traffic (copy-calibrated FETCH_SIZE + WRITE_SIZE per launch) and the
     DRAM throughput it implies are given beside it."""
-    traffic = measured_traffic(run)
+    traffic = measured_traffic(run, workload)
     out = {
         "bound": "hbm",
         "achieved": round(run["achieved_gbs"], 1),
@@ -383,7 +470,11 @@ def main() -> int:
                          " for every storage (skips the tuning at context "
                          "creation)")
     ap.add_argument("--only", default="",
-                    help="comma list of storages to time (f64,f32,f16,bf16)")
+                    help="comma list of storages to time (f64,f32,f16,bf16); "
+                         "default: the workload's list")
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD,
+                    choices=sorted(WORKLOADS),
+                    help="channel workload to run (default %(default)s)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (N=1); "
@@ -395,27 +486,43 @@ def main() -> int:
         exe = os.path.join(ROOT, "cloning-lbm-mpi_amd", "stream_probe")
         return subprocess.run([exe]).returncode
     K, W = args.steps, args.warmup
-    want = [x for x in args.only.split(",") if x] or ["f16", "f64", "f32",
-                                                       "bf16"]
+    wl = args.workload
+    specs = specs_of(wl)
+    want = ([x for x in args.only.split(",") if x]
+            or WORKLOADS[wl]["storages"])
     codes = {"f64": lbc.STORE_F64, "f32": lbc.STORE_F32, "f16": lbc.STORE_F16,
              "bf16": lbc.STORE_BF16}
     runs = {}
     for name in want:
-        print(f"bench: timing {name} ({K} intervals after {W} warm-up)",
+        print(f"bench: {wl}: timing {name} ({K} intervals after {W} warm-up)",
               file=sys.stderr, flush=True)
         shape = [int(v) for v in args.shape.split(",")] if args.shape else None
-        runs[name] = time_storage(codes[name], K, W, shape=shape)
+        runs[name] = time_storage(codes[name], K, W, specs, shape=shape)
     acc = {}
     for name in ([] if args.no_accuracy else want):
         if name == "f64":
             continue
-        print(f"bench: accuracy {name} vs f64", file=sys.stderr, flush=True)
-        acc[name] = accuracy(codes[name])
+        print(f"bench: {wl}: accuracy {name} vs f64", file=sys.stderr,
+              flush=True)
+        acc[name] = accuracy(codes[name], wl)
     cpu = None
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline()
-    head = runs.get("f16") or next(iter(runs.values()))
+        cpu = cpu_baseline(wl)
+    head = runs[want[0]]
+    config = {
+        "workload": WORKLOADS[wl]["desc"] + "; one step = one cloning "
+                    "interval of every clone (step kernel + selection + copy)",
+        "name": wl,
+        "clones": sum(sp.n_clones for sp in specs),
+    }
+    if len(specs) == 1:
+        config["lattice"] = [specs[0].Lx, specs[0].Ly]
+        config["lbm_steps_per_interval"] = specs[0].steps
+    else:
+        config["instances"] = [dict(lattice=[sp.Lx, sp.Ly], clones=sp.n_clones,
+                                    lbm_steps_per_interval=sp.steps, k=sp.bias)
+                               for sp in specs]
     out = {
         "metric": "clone-MLUPS + % HBM peak, FP16-shifted/FP32 vs plain FP64, "
                   "at stated drag error",
@@ -428,27 +535,21 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "storage": head["storage"] + "-shifted",
+        "dtype": "f64" if head["storage"] == "f64" else "f32",
+        "storage": (head["storage"] if head["storage"] == "f64"
+                    else head["storage"] + "-shifted"),
         "data": "synthetic",
-        "config": {
-            "workload": "channel 400x100, square side 20 at (100,50), tau 0.53, "
-                        "u_in 0.05, 4096 clones, 200 LBM steps per cloning "
-                        "interval, k 0.005, noise sigma 1e-4; one step = one "
-                        "cloning interval (step kernel + selection + copy)",
-            "clones": WORKLOAD["n_clones"],
-            "lattice": [WORKLOAD["Lx"], WORKLOAD["Ly"]],
-            "lbm_steps_per_interval": WORKLOAD["steps"],
-        },
-        "roofline": roofline(head),
+        "config": config,
+        "roofline": roofline(head, wl),
         "gather_roofline": head.get("gather"),
         "cpu_baseline": cpu,
         "paths": runs,
         "accuracy_vs_fp64": acc,
     }
-    if "f64" in runs and "f16" in runs:
-        out["speedup_f16_vs_f64"] = round(
-            runs["f16"]["clone_mlups"] / runs["f64"]["clone_mlups"], 3)
+    for name in ("f16", "f32", "bf16"):
+        if "f64" in runs and name in runs:
+            out[f"speedup_{name}_vs_f64"] = round(
+                runs[name]["clone_mlups"] / runs["f64"]["clone_mlups"], 3)
     print(json.dumps(out), flush=True)
     return 0
 
