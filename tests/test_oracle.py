@@ -594,3 +594,22 @@ def test_macro_of_equilibrium_recovers_its_moments(O):
             f[x, y] = O.feq(r, u, v)
             want[:, x, y] = (r, u + 0.5 * g.beta / r, v)
     np.testing.assert_allclose(O.macro(g, f), want, rtol=1e-13, atol=1e-15)
+
+
+def test_oracle_is_clean_under_sanitizers():
+    """Every oracle function, driven by oracle/san_check.c, built with
+    AddressSanitizer and UndefinedBehaviorSanitizer: no report, exit 0."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None or shutil.which("make") is None:
+        pytest.skip("gcc or make not available")
+    d = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                     "oracle")
+    b = subprocess.run(["make", "-s", "-C", d, "san"], capture_output=True,
+                       text=True, timeout=300)
+    assert b.returncode == 0, b.stderr
+    r = subprocess.run([os.path.join(d, "_san", "san_check")],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.strip().endswith("san_check: ok")
+    assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr

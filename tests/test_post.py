@@ -10,10 +10,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 RECON = os.path.join(ROOT, "cloning-lbm-mpi_amd", "post", "reconstruct")
 
 
-def _tool():
-    if not os.path.exists(RECON):
-        pytest.skip("post/reconstruct is not built (make -C cloning-lbm-mpi_amd)")
-    return RECON
+@pytest.fixture(params=["reconstruct", "reconstruct_san"])
+def tool(request):
+    """The program as shipped, and the same source built with
+    AddressSanitizer + UndefinedBehaviorSanitizer (any report aborts it)."""
+    path = os.path.join(os.path.dirname(RECON), request.param)
+    if not os.path.exists(path):
+        pytest.skip(f"post/{request.param} is not built "
+                    "(make -C cloning-lbm-mpi_amd)")
+    return path
 
 
 def fake_run(path, Nc, nt, ndt, seed):
@@ -45,11 +50,11 @@ def lineage(parents, j, nt):
 
 
 @pytest.mark.parametrize("Nc,nt,ndt", [(1, 1, 4), (5, 4, 3), (16, 30, 7)])
-def test_reconstruct_follows_the_genealogy(tmp_path, Nc, nt, ndt):
+def test_reconstruct_follows_the_genealogy(tmp_path, tool, Nc, nt, ndt):
     run = str(tmp_path / "run") + "/"
     out = str(tmp_path / "out") + "/"
     parents = fake_run(run, Nc, nt, ndt, seed=Nc * 100 + nt)
-    r = subprocess.run([_tool(), run, str(Nc), str(nt), str(ndt), out, "all"],
+    r = subprocess.run([tool, run, str(Nc), str(nt), str(ndt), out, "all"],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     for j in range(Nc):
@@ -62,7 +67,7 @@ def test_reconstruct_follows_the_genealogy(tmp_path, Nc, nt, ndt):
                 got[t], 1000.0 * line[t] + 10.0 * t + np.arange(ndt))
 
 
-def test_reconstruct_identity_genealogy_returns_own_records(tmp_path):
+def test_reconstruct_identity_genealogy_returns_own_records(tmp_path, tool):
     """With every slot its own parent the trajectory is the slot's file."""
     Nc, nt, ndt = 6, 5, 4
     run = str(tmp_path / "run") + "/"
@@ -70,7 +75,7 @@ def test_reconstruct_identity_genealogy_returns_own_records(tmp_path):
     np.tile(np.arange(Nc, dtype=np.int32), nt).tofile(
         os.path.join(run, "labels.dat"))
     out = str(tmp_path / "out") + "/"
-    r = subprocess.run([_tool(), run, str(Nc), str(nt), str(ndt), out, "2", "5"],
+    r = subprocess.run([tool, run, str(Nc), str(nt), str(ndt), out, "2", "5"],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     for j in (2, 5):
@@ -80,11 +85,11 @@ def test_reconstruct_identity_genealogy_returns_own_records(tmp_path):
     assert not os.path.exists(os.path.join(out, "clone_0"))
 
 
-def test_reconstruct_default_is_slot_zero_and_rejects_bad_input(tmp_path):
+def test_reconstruct_default_is_slot_zero_and_rejects_bad_input(tmp_path, tool):
     Nc, nt, ndt = 3, 3, 2
     run = str(tmp_path / "run") + "/"
     fake_run(run, Nc, nt, ndt, seed=2)
-    r = subprocess.run([_tool(), run, str(Nc), str(nt), str(ndt)],
+    r = subprocess.run([tool, run, str(Nc), str(nt), str(ndt)],
                        capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode == 0, r.stderr
     assert os.path.exists(tmp_path / "outputFolder" / "clone_0" /
@@ -93,10 +98,10 @@ def test_reconstruct_default_is_slot_zero_and_rejects_bad_input(tmp_path):
     # more intervals than the files hold, a slot out of range, a bad label
     for args in ([run, str(Nc), str(nt + 1), str(ndt)],
                  [run, str(Nc), str(nt), str(ndt), "7"]):
-        r = subprocess.run([_tool()] + args, capture_output=True, text=True,
+        r = subprocess.run([tool] + args, capture_output=True, text=True,
                            cwd=tmp_path)
         assert r.returncode == 1
     np.full(Nc * nt, 9, np.int32).tofile(os.path.join(run, "labels.dat"))
-    r = subprocess.run([_tool(), run, str(Nc), str(nt), str(ndt)],
+    r = subprocess.run([tool, run, str(Nc), str(nt), str(ndt)],
                        capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode == 1 and "outside" in r.stderr
