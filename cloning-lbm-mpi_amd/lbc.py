@@ -134,14 +134,23 @@ def lib() -> C.CDLL:
 def _dp(a):
     if a is None:
         return None
-    assert a.dtype == np.float64 and a.flags.c_contiguous
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous float64 array")
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _sized(a: np.ndarray, n: int, what: str) -> np.ndarray:
+    """The library reads exactly n elements from a host array: check it."""
+    if a.size != n:
+        raise ValueError(f"{what}: expected {n} values, got {a.size}")
+    return a
 
 
 def _ip(a):
     if a is None:
         return None
-    assert a.dtype == np.int32 and a.flags.c_contiguous
+    if a.dtype != np.int32 or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous int32 array")
     return a.ctypes.data_as(C.POINTER(C.c_int32))
 
 
@@ -349,12 +358,15 @@ class Sampler:
                     "lbc_init_equilibrium")
 
     def set_snapshots(self, inst: int, snaps: np.ndarray) -> None:
-        a = np.ascontiguousarray(snaps, np.float64)
+        s = self.specs[inst]
+        a = _sized(np.ascontiguousarray(snaps, np.float64),
+                   s.n_snapshots * s.Lx * s.Ly * 9, "snapshots")
         self._check(lib().lbc_set_snapshots(self._h, inst, _dp(a)),
                     "lbc_set_snapshots")
 
     def set_marks(self, inst: int, marks) -> None:
-        m = np.ascontiguousarray(marks, np.uint8)
+        m = _sized(np.ascontiguousarray(marks, np.uint8),
+                   self.specs[inst].n_clones, "marks")
         self._check(lib().lbc_set_marks(
             self._h, inst, m.ctypes.data_as(C.POINTER(C.c_uint8))),
             "lbc_set_marks")
@@ -371,6 +383,9 @@ class Sampler:
         p = None
         if prefactors is not None:
             p = np.ascontiguousarray(prefactors, np.float64)
+            if self.mode == MODE_REF:
+                _sized(p, sum(sp.n_clones * sp.n_snapshots
+                              for sp in self.specs), "prefactors")
         self._check(lib().lbc_run_interval(self._h, _dp(p)),
                     "lbc_run_interval")
 
@@ -392,9 +407,12 @@ class Sampler:
     def select(self, inst: int, weights=None, u=None, rint=None
                ) -> SelectionResult:
         n = self.specs[inst].n_clones
-        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
-        uu = None if u is None else np.ascontiguousarray(u, np.float64)
-        rr = None if rint is None else np.ascontiguousarray(rint, np.int32)
+        w = (None if weights is None else
+             _sized(np.ascontiguousarray(weights, np.float64), n, "weights"))
+        uu = (None if u is None else
+              _sized(np.ascontiguousarray(u, np.float64), n, "u"))
+        rr = (None if rint is None else
+              _sized(np.ascontiguousarray(rint, np.int32), n, "rint"))
         nb0 = np.zeros(n, np.int32)
         nb1 = np.zeros(n, np.int32)
         pairs = np.zeros(2 * n, np.int32)
@@ -439,7 +457,9 @@ class Sampler:
         (n_clones, Lx, Ly) (the reference's map[y + Ly*x] layout)."""
         s = self.specs[inst]
         a = np.ascontiguousarray(maps, np.float64)
-        assert a.shape == (s.n_clones, s.Lx, s.Ly)
+        if a.shape != (s.n_clones, s.Lx, s.Ly):
+            raise ValueError(f"field maps: expected shape "
+                             f"{(s.n_clones, s.Lx, s.Ly)}, got {a.shape}")
         self._check(lib().lbc_set_field_maps(self._h, inst, _dp(a)),
                     "lbc_set_field_maps")
 

@@ -75,3 +75,18 @@ def test_null_context_calls_fail_cleanly(lbc):
     assert L.lbc_gather(None) == -1
     assert L.lbc_sync(None) == -1
     assert b"null" in L.lbc_last_error(None)
+
+
+def test_host_arrays_are_checked_before_the_library_reads_them(lbc):
+    """The C ABI reads a fixed number of elements from caller arrays; the
+    Python mirror checks type, layout and length first."""
+    import numpy as np
+    with pytest.raises(ValueError):
+        lbc._sized(np.zeros(3), 4, "weights")
+    assert lbc._sized(np.zeros(4), 4, "weights").size == 4
+    with pytest.raises(TypeError):
+        lbc._dp(np.zeros(4, np.float32))
+    with pytest.raises(TypeError):
+        lbc._dp(np.zeros((4, 4))[:, 0])
+    with pytest.raises(TypeError):
+        lbc._ip(np.zeros(4, np.int64))
