@@ -24,7 +24,10 @@ are close enough that a cloning step copies only a few clones.
 measurements, with the storages listed for it): channel200x50 (16 clones,
 FP64), channel400x100-n512 (512 clones, FP64), batched8 (8 samplers of four
 lattice sizes and two biases in one launch, FP32/FP64/FP16) and
-channel800x200 (2048 clones, 400 steps, FP16/FP64).
+channel800x200 (2048 clones, 400 steps, FP16/FP64), and ref129, the
+reference's own mode (periodic box with body force, stress-integral drag and
+snapshot perturbation) at its L = 32 size, 4096 clones, FP64/FP16/FP32, with
+the reference's own kernels as the CPU baseline.
 
 Prints one JSON line.  Only N=1 is supported (one GPU).
 """
@@ -98,15 +101,39 @@ WORKLOADS = {
         instances=[_inst(800, 200, 40, 200, 100, 0.52, 0.04, 2048, 400, 0.002,
                          5e-5)],
         storages=["f16", "f64"]),
+    # the reference's own production mode: periodic box with body force,
+    # stress-integral drag and snapshot perturbation of the copies
+    "ref129": dict(
+        desc="reference box: square side L 32 in a periodic-x 129x129 box "
+             "with walls, body force beta0 for U0 0.1, tau 0.6, 4096 clones, "
+             "320 LBM steps per cloning interval (dT 1 at T0 = L/U0), alpha "
+             "50, copies perturbed by mixing 10 snapshots at eps 0.01",
+        mode="ref",
+        ref=dict(L=32, tau=0.6, U0=0.1, n_clones=4096, dT=1.0, alpha=50.0,
+                 eps=0.01, nn=10),
+        storages=["f64", "f16", "f32"]),
 }
 DEFAULT_WORKLOAD = "channel400x100"
+
+
+def mode_of(workload: str) -> int:
+    return (lbc.MODE_REF if WORKLOADS[workload].get("mode") == "ref"
+            else lbc.MODE_CHANNEL)
 
 
 def specs_of(workload: str, max_clones: int | None = None
              ) -> list[lbc.InstanceSpec]:
     """The workload's instances; max_clones caps each instance's clones."""
+    wl = WORKLOADS[workload]
+    if "ref" in wl:
+        r = wl["ref"]
+        n = r["n_clones"] if max_clones is None else min(max_clones,
+                                                         r["n_clones"])
+        steps = math.floor(r["dT"] * r["L"] / r["U0"])
+        return [lbc.ref_box(r["L"], r["tau"], r["U0"], n, steps, r["alpha"],
+                            eps=r["eps"], n_snapshots=r["nn"])]
     out = []
-    for w in WORKLOADS[workload]["instances"]:
+    for w in wl["instances"]:
         n = w["n_clones"] if max_clones is None else min(max_clones,
                                                          w["n_clones"])
         out.append(lbc.channel(w["Lx"], w["Ly"], w["side"], w["cx"], w["cy"],
@@ -115,8 +142,18 @@ def specs_of(workload: str, max_clones: int | None = None
     return out
 
 
-def time_storage(storage: int, K: int, W: int, specs: list, seed: int = 0,
-                 shape=None, G: int = 3) -> dict:
+def prepare(s: "lbc.Sampler", specs: list, amp: float = 1e-3) -> None:
+    """Seeded equilibrium states; in the reference mode the snapshot set of
+    the perturbation is the first clones' initial states, as the reference
+    driver takes its snapshots from the initial pops files."""
+    for i, sp in enumerate(specs):
+        s.init_equilibrium(i, amp)
+        if s.mode == lbc.MODE_REF and sp.n_snapshots > 0:
+            s.set_snapshots(i, s.store_states(i, 0, sp.n_snapshots))
+
+
+def time_storage(storage: int, K: int, W: int, specs: list, mode: int,
+                 seed: int = 0, shape=None, G: int = 3) -> dict:
     """W untimed warm-up intervals then K timed ones (graph replays), then
     G selections and copies of the last interval's clones with the copy
     kernel timed alone.  shape: (threads, sites per thread, non-temporal,
@@ -125,11 +162,9 @@ def time_storage(storage: int, K: int, W: int, specs: list, seed: int = 0,
     if shape:
         kw = dict(threads=shape[0], vec=shape[1], nt=bool(shape[2]),
                   passes=shape[3] if len(shape) > 3 else 1)
-    ni = len(specs)
-    with lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, storage=storage,
-                     seed=seed, **kw) as s:
-        for i in range(ni):
-            s.init_equilibrium(i, 1e-3)
+    with lbc.Sampler(specs, mode=mode, storage=storage, seed=seed,
+                     **kw) as s:
+        prepare(s, specs)
         if W:
             s.advance(W)
         s.sync()
@@ -233,24 +268,27 @@ def accuracy(storage: int, workload: str, n: int = 256, warm: int = 10,
     log mean weight error; at the end: the running scaled-cumulant estimate
     from each sampler's own mean weights (the worst instance)."""
     specs = specs_of(workload, max_clones=n)
+    mode = mode_of(workload)
     ni = len(specs)
     rng = np.random.default_rng(seed)
-    with lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, storage=lbc.STORE_F64,
+    with lbc.Sampler(specs, mode=mode, storage=lbc.STORE_F64,
                      seed=seed) as sw:
-        for i in range(ni):
-            sw.init_equilibrium(i, 1e-3)
+        prepare(sw, specs)
         sw.advance(warm)
         states = [sw.store_states(i) for i in range(ni)]
     drag, dlr = [], []
     lr64 = [[] for _ in range(ni)]
     lrlp = [[] for _ in range(ni)]
-    with lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, storage=lbc.STORE_F64,
+    with lbc.Sampler(specs, mode=mode, storage=lbc.STORE_F64,
                      seed=seed) as s64, \
-            lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, storage=storage,
+            lbc.Sampler(specs, mode=mode, storage=storage,
                         seed=seed) as slp:
-        for i in range(ni):
+        for i, sp in enumerate(specs):
             s64.load_states(i, 0, states[i])
             slp.load_states(i, 0, states[i])
+            if mode == lbc.MODE_REF and sp.n_snapshots > 0:
+                s64.set_snapshots(i, states[i][:sp.n_snapshots])
+                slp.set_snapshots(i, states[i][:sp.n_snapshots])
         for _ in range(intervals):
             s64.run_interval()
             slp.run_interval()
@@ -300,26 +338,42 @@ def accuracy(storage: int, workload: str, n: int = 256, warm: int = 10,
 
 def cpu_baseline(workload: str, lups_per_core: float = 2.4e8,
                  steps: int = 200) -> dict:
-    """The CPU oracle (a port of the reference's AoS push step, serial clones
-    per worker, one worker thread per core, like one MPI rank per core) on a
+    """The CPU path timed on the box's host cores, one worker thread per
+    core with its clones run serially, like one MPI rank per core, on a
     bounded sample of the workload: clones of its largest lattice, `steps`
     steps each, about lups_per_core lattice updates per thread (30 clones of
-    400x100)."""
+    400x100).  In the reference mode the workers run the reference's own
+    kernels (oracle/_ref/libref.so, its sources at its -O3, in its driver's
+    step loop) when that build is present: kind "reference".  Otherwise,
+    and for the channel workloads, which have no reference code, they run
+    the oracle's port of the reference's AoS push step: kind "port"."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    w = max(WORKLOADS[workload]["instances"], key=lambda i: i["Lx"] * i["Ly"])
-    g = O.channel_geom(w["Lx"], w["Ly"], w["side"], w["cx"], w["cy"], w["tau"],
-                       w["u_in"])
+    wl = WORKLOADS[workload]
+    if "ref" in wl:
+        r = wl["ref"]
+        g = O.ref_box_geom(r["L"], r["tau"], r["U0"])
+        base = np.broadcast_to(O.W, (g.Lx, g.Ly, 9)).copy()
+    else:
+        w = max(wl["instances"], key=lambda i: i["Lx"] * i["Ly"])
+        g = O.channel_geom(w["Lx"], w["Ly"], w["side"], w["cx"], w["cy"],
+                           w["tau"], w["u_in"])
+        base = O.init_channel(g, 1e-3, 0, 0)
+    kind = "reference" if "ref" in wl and O.ref_available() else "port"
     sample_clones_per_core = max(1, round(lups_per_core /
                                           (g.Lx * g.Ly * steps)))
     cores = max(1, min(16, os.cpu_count() or 1))
-    base = O.init_channel(g, 1e-3, 0, 0)
     states = [np.stack([base] * sample_clones_per_core) for _ in range(cores)]
     O.lib()
+    if kind == "reference":
+        O.ref_set_geometry(g)  # fixed while the workers run
 
     def work(i):
-        O.run_clones(g, states[i], steps)
+        if kind == "reference":
+            O.ref_run_clones(g, states[i], steps)
+        else:
+            O.run_clones(g, states[i], steps)
 
     threads = [threading.Thread(target=work, args=(i,)) for i in range(cores)]
     t0 = time.perf_counter()
@@ -338,10 +392,12 @@ def cpu_baseline(workload: str, lups_per_core: float = 2.4e8,
                     break
     except OSError:
         pass
+    what = ("the reference's own step kernels in its driver's step loop, "
+            "FP64 AoS" if kind == "reference" else "FP64 AoS push oracle")
     out = dict(value=lups / secs / 1e6, unit="clone-MLUPS", cores=cores,
-               kind="port",
+               kind=kind,
                sample=(f"{cores} threads x {sample_clones_per_core} clones x "
-                       f"{steps} steps of {g.Lx}x{g.Ly} (FP64 AoS push oracle), "
+                       f"{steps} steps of {g.Lx}x{g.Ly} ({what}), "
                        f"{secs:.1f} s on {cpu}"))
     cal = calibrate_port(O)
     if cal:
@@ -497,7 +553,8 @@ def main() -> int:
         print(f"bench: {wl}: timing {name} ({K} intervals after {W} warm-up)",
               file=sys.stderr, flush=True)
         shape = [int(v) for v in args.shape.split(",")] if args.shape else None
-        runs[name] = time_storage(codes[name], K, W, specs, shape=shape)
+        runs[name] = time_storage(codes[name], K, W, specs, mode_of(wl),
+                                  shape=shape)
     acc = {}
     for name in ([] if args.no_accuracy else want):
         if name == "f64":

@@ -149,6 +149,9 @@ def ref() -> C.CDLL:
                                        C.c_double, C.c_double]
         R.ref_force.argtypes = [dp, C.c_double]
         R.ref_force.restype = C.c_double
+        R.ref_run_clones.argtypes = [dp, dp, C.c_int, C.c_int, C.c_double,
+                                     C.c_double, dp]
+        R.ref_run_clones.restype = None
         R.ref_perturb.argtypes = [dp, dp, C.c_double, C.c_int, C.c_int]
         R.ref_seed48.argtypes = [C.c_ushort] * 3
         R.ref_drand48.restype = C.c_double
@@ -402,6 +405,24 @@ def ref_step(g: Geom, fin: np.ndarray, fout: np.ndarray) -> float:
     uy = np.zeros(S)
     R.ref_step(_dp(fin), _dp(fout), _dp(rho), _dp(ux), _dp(uy), g.beta, g.tau)
     return R.ref_force(_dp(fout), 1.0 / g.tau)
+
+
+def ref_set_geometry(g: Geom) -> None:
+    """Point the reference kernels' geometry globals at g (before threads
+    start running ref_run_clones on it)."""
+    ref().ref_set_geometry(g.Lx, g.Ly, g.xmin, g.xmax, g.ymin, g.ymax)
+
+
+def ref_run_clones(g: Geom, states: np.ndarray, M: int) -> np.ndarray:
+    """The reference driver's step loop on n clones (states (n, Lx, Ly, 9),
+    advanced in place by M production steps of the reference's own kernels);
+    returns each clone's sum of raw forces.  ref_set_geometry(g) first."""
+    n = states.shape[0]
+    scratch = np.empty_like(states[0])
+    acc = np.zeros(n)
+    ref().ref_run_clones(_dp(states), _dp(scratch), n, M, g.beta, g.tau,
+                         _dp(acc))
+    return acc
 
 
 def ref_step_fields(g: Geom, fin: np.ndarray, fout: np.ndarray,

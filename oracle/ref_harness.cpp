@@ -70,6 +70,37 @@ double ref_force(double* f, double omega) {
   return computeForceOnSquare(f, omega);
 }
 
+// The driver's per-clone interval loop (src/mpi_LBM_GK_pops_perturb.cpp:
+// 399-449) over n clones held back to back in `states` (each Dx*Dy*9
+// doubles, post-stream form in and out): M production steps and the force
+// on every new state, summed into acc[j] (raw F; the driver's 1/F0 and
+// AVG are two more flops per step).  scratch holds one state.  Used to time
+// the reference's own kernels; the geometry globals must be set first and
+// stay fixed while several threads run this on their own clones.
+void ref_run_clones(double* states, double* scratch, int n, int M,
+                    double beta, double tau, double* acc) {
+  const long N = 9L * Dx * Dy;
+  const long S = (long)Dx * Dy;
+  double* fields = (double*)std::malloc(3 * S * sizeof(double));
+  if (!fields) return;
+  for (int j = 0; j < n; j++) {
+    double* fin = states + j * N;
+    double* fout = scratch;
+    double sum = 0.0;
+    for (int m = 0; m < M; m++) {
+      ref_step(fin, fout, fields, fields + S, fields + 2 * S, beta, tau);
+      sum += computeForceOnSquare(fout, 1.0 / tau);
+      double* t = fin;
+      fin = fout;
+      fout = t;
+    }
+    if (fin != states + j * N)
+      for (long i = 0; i < N; i++) states[j * N + i] = fin[i];
+    acc[j] = sum;
+  }
+  std::free(fields);
+}
+
 // generatePerturbedState draws its NN prefactors from drand48(); the caller
 // controls that stream with ref_seed48 / ref_drand48.
 void ref_perturb(double* fin, double* snaps, double eps, int NN, int N) {

@@ -16,17 +16,22 @@ def bench():
     return b
 
 
-def test_workloads_are_valid_channel_samplers(bench):
+def test_workloads_are_valid_samplers(bench):
     for name, w in bench.WORKLOADS.items():
         specs = bench.specs_of(name)
-        assert len(specs) == len(w["instances"]) >= 1, name
+        ref = bench.mode_of(name) == bench.lbc.MODE_REF
+        assert len(specs) == (1 if ref else len(w["instances"])) >= 1, name
         assert w["storages"] and set(w["storages"]) <= {"f64", "f32", "f16",
                                                         "bf16"}
         for sp in specs:
             # the square keeps two fluid nodes from every boundary
             assert 2 <= sp.xmin < sp.xmax <= sp.Lx - 3, name
             assert 2 <= sp.ymin < sp.ymax <= sp.Ly - 3, name
-            assert sp.tau > 0.5 and sp.noise_sigma > 0 and sp.n_clones >= 1
+            assert sp.tau > 0.5 and sp.n_clones >= 1
+            if ref:  # body force, snapshot perturbation, no noise
+                assert sp.beta > 0 and sp.n_snapshots > 0 and sp.eps > 0
+            else:
+                assert sp.noise_sigma > 0 and sp.beta == 0
 
 
 def test_workload_sizes(bench):
@@ -37,6 +42,10 @@ def test_workload_sizes(bench):
     assert sizes["channel200x50"] == [(200, 50, 16, 100)]
     assert sizes["channel400x100-n512"] == [(400, 100, 512, 200)]
     assert sizes["channel800x200"] == [(800, 200, 2048, 400)]
+    # the reference driver's box for L = 32: Dx = Dy = 129, dT*T0 = 320
+    assert sizes["ref129"] == [(129, 129, 4096, 320)]
+    (r,) = bench.specs_of("ref129")
+    assert (r.xmin, r.xmax, r.ymin, r.ymax) == (64, 96, 48, 80)
     b8 = bench.specs_of("batched8")
     assert sum(sp.n_clones for sp in b8) == 4096
     assert sorted({sp.bias for sp in b8}) == [0.002, 0.01]

@@ -613,3 +613,29 @@ def test_oracle_is_clean_under_sanitizers():
     assert r.returncode == 0, r.stderr[-2000:]
     assert r.stdout.strip().endswith("san_check: ok")
     assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr
+
+
+@pytest.mark.skipif(not __import__("oracle").ref_available(),
+                    reason="reference kernels not built")
+def test_reference_clone_loop_equals_its_steps(O):
+    """ref_run_clones (the reference's kernels in the driver's loop, used to
+    time the reference on the CPU) is the same sequence of ref_step calls."""
+    g = O.ref_box_geom(4, 0.53, 0.05)
+    rng = np.random.default_rng(8)
+    states = np.stack([np.broadcast_to(O.W, (g.Lx, g.Ly, 9)) *
+                       (1 + 1e-3 * rng.standard_normal((g.Lx, g.Ly, 1)))
+                       for _ in range(3)])
+    states[:, g.xmin + 1:g.xmax, g.ymin + 1:g.ymax] = O.W
+    for M in (4, 5):
+        want = states.copy()
+        acc_want = np.zeros(3)
+        for j in range(3):
+            a, b = want[j].copy(), np.zeros_like(want[j])
+            for _ in range(M):
+                acc_want[j] += O.ref_step(g, a, b)
+                a, b = b, a
+            want[j] = a
+        got = states.copy()
+        O.ref_set_geometry(g)
+        acc = O.ref_run_clones(g, got, M)
+        assert np.array_equal(got, want) and np.array_equal(acc, acc_want)
