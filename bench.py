@@ -289,7 +289,10 @@ def accuracy(storage: int, workload: str, n: int = 256, warm: int = 10,
             if mode == lbc.MODE_REF and sp.n_snapshots > 0:
                 s64.set_snapshots(i, states[i][:sp.n_snapshots])
                 slp.set_snapshots(i, states[i][:sp.n_snapshots])
-        for _ in range(intervals):
+        for it in range(intervals):
+            if intervals > 8 and it % 10 == 0:
+                print(f"bench: accuracy {lbc.STORAGE_NAMES[storage]}: shadow "
+                      f"interval {it}/{intervals}", file=sys.stderr, flush=True)
             s64.run_interval()
             slp.run_interval()
             d_int, l_int = [], []
@@ -515,6 +518,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--acc-clones", type=int, default=256,
+                    help="clones per instance in the accuracy comparison")
+    ap.add_argument("--acc-intervals", type=int, default=8,
+                    help="cloning steps of the accuracy comparison's shadow "
+                         "run (the workloads' full runs are 40-50)")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="run cloning-lbm-mpi_amd/stream_probe (streaming "
                          "ceilings for the roofline) and exit")
@@ -561,7 +569,8 @@ def main() -> int:
             continue
         print(f"bench: {wl}: accuracy {name} vs f64", file=sys.stderr,
               flush=True)
-        acc[name] = accuracy(codes[name], wl)
+        acc[name] = accuracy(codes[name], wl, n=args.acc_clones,
+                             intervals=args.acc_intervals)
     cpu = None
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
