@@ -42,14 +42,24 @@ def write_case(tmp, O, L, Nc, T, dT, eps, tau, U0, alpha, extra=""):
     return g, data, names, inp
 
 
-def reference_loop(O, g, data, names, Nc, T, dT, eps, U0, alpha, L):
-    """The reference driver's interval loop for one rank, on the oracle."""
+def reference_loop(O, g, data, names, Nc, T, dT, eps, U0, alpha, L, p=1):
+    """The reference driver's interval loop with p MPI ranks of Nc/p clones,
+    on the oracle (src/mpi_LBM_GK_pops_perturb.cpp:251-319,387-714): the
+    master hands the first Nc - Nc/p file names to ranks 1.. and keeps the
+    next Nc/p; every rank reads the first 10 names as snapshots and draws
+    its marked clones' prefactors from its own drand48 (never seeded, so
+    each rank's stream starts from the default state); the master draws the
+    floor-step uniforms from its drand48 after its own prefactors, the
+    prune/add ints from its rand() (srand(1), 1000 discarded), and sums
+    total_R in rank blocks."""
     T0 = L / U0
     F0 = U0 * U0 * L * 0.5
     dt = 1.0 / T0
     M = int(math.floor(dT * T0))
     nT = int(math.floor(T / dT))
-    states = [np.fromfile(data / names[j]).reshape(g.Lx, g.Ly, 9)
+    local = Nc // p
+    first = [j - local if j >= local else (Nc - local) + j for j in range(Nc)]
+    states = [np.fromfile(data / names[first[j]]).reshape(g.Lx, g.Ly, 9)
               for j in range(Nc)]
     snaps = np.stack([np.fromfile(data / names[n]).reshape(g.Lx, g.Ly, 9)
                       for n in range(10)])
@@ -58,10 +68,13 @@ def reference_loop(O, g, data, names, Nc, T, dT, eps, U0, alpha, L):
     tmpi = np.zeros(1000, np.int32)
     lib.orc_glibc_rand_n(1000, O._ip(tmpi))
     lib.orc_glibc_seed48_default()
+    default48 = np.zeros(100000)  # drand48's sequence from its default state
+    lib.orc_glibc_drand48_n(default48.size, O._dp(default48))
+    pos = [0] * p
 
-    def drand(n):
-        out = np.zeros(n)
-        lib.orc_glibc_drand48_n(n, O._dp(out))
+    def drand(n, rank=0):
+        out = default48[pos[rank]:pos[rank] + n]
+        pos[rank] += n
         return out
 
     rq = deque()
@@ -71,7 +84,7 @@ def reference_loop(O, g, data, names, Nc, T, dT, eps, U0, alpha, L):
     for t in range(nT):
         for j in range(Nc):
             if marks[j]:
-                O.perturb(states[j], snaps, eps, drand(10))
+                O.perturb(states[j], snaps, eps, drand(10, j // local))
         acc = np.zeros(Nc)
         for j in range(Nc):
             acc[j], A = O.run_steps(g, states[j], M, 1.0 / F0, 1.775, dt)
@@ -82,7 +95,7 @@ def reference_loop(O, g, data, names, Nc, T, dT, eps, U0, alpha, L):
             tmp1 = np.zeros(1, np.int32)
             lib.orc_glibc_rand_n(1, O._ip(tmp1))
             rq.append(int(tmp1[0]))
-        sel = O.select(s, u, np.array(list(rq)[:Nc], np.int32))
+        sel = O.select(s, u, np.array(list(rq)[:Nc], np.int32), p_blocks=p)
         for _ in range(sel["n_rint"]):
             rq.popleft()
         labels_all.append(sel["labels"])
@@ -139,6 +152,38 @@ def test_driver_matches_reference_loop(tmp_path, O, Nc, alpha):
         got = np.fromfile(tmp_path / "traj" / f"clone_{j}" /
                           "data_force.datout").reshape(nt, 20)
         np.testing.assert_allclose(got, np.array(want), rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("Nc,p", [(8, 2), (9, 3)])
+def test_driver_reproduces_a_multi_rank_run(tmp_path, O, Nc, p):
+    """ranks = p: the initial-state assignment, the per-rank prefactor
+    streams, the master's draws and the rank-blocked total_R of a p-rank
+    run of the reference driver."""
+    L, T, dT, eps, tau, U0, alpha = 4, 1.0, 0.25, 0.02, 0.53, 0.05, 40.0
+    g, data, names, inp = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
+                                     alpha, extra=f"ranks = {p}\n"
+                                                  "dump_states = 1\n")
+    print(f"case: driver REF L={L} Nc={Nc} ranks={p}", flush=True)
+    r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    labels_ref, R_ref, forces_ref, states_ref = reference_loop(
+        O, g, data, names, Nc, T, dT, eps, U0, alpha, L, p=p)
+    run = tmp_path / "run"
+    labels = np.fromfile(run / "labels.dat", dtype=np.int32).reshape(-1, Nc)
+    assert np.array_equal(labels, np.array(labels_ref))
+    # a slot of rank >= 1 was overwritten before the last interval, so its
+    # perturbation drew from that rank's own stream
+    assert any(lab[j] != j for lab in labels_ref[:-1]
+               for j in range(Nc // p, Nc))
+    np.testing.assert_allclose(np.fromfile(run / "rvalues.dat"), R_ref,
+                               rtol=1e-12, atol=0)
+    for j in range(Nc):
+        Fj = np.fromfile(run / f"clone_{j}" / "data_force.datout").reshape(-1, 20)
+        assert np.array_equal(Fj[0], forces_ref[j][0])
+        fin = np.fromfile(run / f"clone_{j}" / "pops_final.datout").reshape(
+            g.Lx, g.Ly, 9)
+        np.testing.assert_allclose(fin, states_ref[j], rtol=1e-11, atol=0)
 
 
 WARMUP = os.path.join(PKG, "driver", "lbm_warmup")
