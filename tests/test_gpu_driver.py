@@ -186,6 +186,89 @@ def test_driver_reproduces_a_multi_rank_run(tmp_path, O, Nc, p):
         np.testing.assert_allclose(fin, states_ref[j], rtol=1e-11, atol=0)
 
 
+def _channel_input(tmp, Nc, extra):
+    inp = tmp / "input"
+    # positional fields: Nc T dT eps L tau U0 alpha path master facquVTK
+    inp.write_text(f"{Nc}\n1\n1\n0\n4\n0.55\n0.05\n1.0\n{tmp}/\n"
+                   f"{tmp}/run/\n5\nmode = channel\nLx = 80\nLy = 40\n"
+                   "side = 6\ncx = 20\ncy = 20\nu_in = 0.05\nk = 1.0\n"
+                   "sigma = 1e-4\nsteps = 12\nintervals = 3\nseed = 3\n"
+                   + extra)
+    return inp
+
+
+@pytest.mark.parametrize("force_io", [1, 0])
+def test_driver_channel_mode_matches_the_library(tmp_path, lbc, force_io):
+    """mode = channel: the driver's interval loop (force_io = 1) and its
+    all-device path (force_io = 0: one lbc_advance of every interval) give
+    the same labels, mean weights and per-step drag as the same calls made
+    through the library directly, bit for bit."""
+    Nc = 12
+    inp = _channel_input(tmp_path, Nc, f"force_io = {force_io}\nvtk = 0\n"
+                                       "dump_states = 1\n")
+    print(f"case: channel driver force_io={force_io}", flush=True)
+    r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    run = tmp_path / "run"
+    spec = lbc.channel(80, 40, 6, 20, 20, 0.55, 0.05, Nc, 12, k=1.0,
+                       sigma=1e-4)
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, seed=3) as s:
+        s.init_equilibrium(0, 1e-3)
+        if force_io:
+            labels, drag = [], []
+            for _ in range(3):
+                s.run_interval()
+                drag.append(s.drag(0))
+                labels.append(s.select(0).labels)
+                s.gather()
+        else:
+            s.advance(3)
+        R = s.R_history(0)
+        final = s.store_states(0)
+    assert np.array_equal(np.fromfile(run / "rvalues.dat"), R)
+    if force_io:
+        got = np.fromfile(run / "labels.dat", dtype=np.int32).reshape(3, Nc)
+        assert np.array_equal(got, np.array(labels))
+        for j in range(Nc):
+            Fj = np.fromfile(run / f"clone_{j}" / "data_force.datout")
+            assert np.array_equal(Fj.reshape(3, 12),
+                                  np.array([d[j] for d in drag]))
+    for j in range(Nc):
+        fin = np.fromfile(run / f"clone_{j}" / "pops_final.datout").reshape(
+            80, 40, 9)
+        assert np.array_equal(fin, final[j])
+
+
+def test_driver_batches_extra_channel_instances(tmp_path, lbc):
+    """`instance = ...` lines add samplers to the driver's launches; the
+    first sampler's results are those of the same batched context built
+    through the library."""
+    Nc = 8
+    inp = _channel_input(tmp_path, Nc, "force_io = 0\ndump_states = 1\n"
+                                       "instance = 60 30 6 15 15 0.55 0.05 "
+                                       "16 12 0.5 1e-4\n")
+    print("case: channel driver with an extra instance", flush=True)
+    r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    specs = [lbc.channel(80, 40, 6, 20, 20, 0.55, 0.05, Nc, 12, k=1.0,
+                         sigma=1e-4),
+             lbc.channel(60, 30, 6, 15, 15, 0.55, 0.05, 16, 12, k=0.5,
+                         sigma=1e-4)]
+    with lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, seed=3) as s:
+        s.init_equilibrium(0, 1e-3)
+        s.init_equilibrium(1, 1e-3)
+        s.advance(3)
+        R = s.R_history(0)
+        final = s.store_states(0)
+    assert np.array_equal(np.fromfile(tmp_path / "run" / "rvalues.dat"), R)
+    for j in range(Nc):
+        fin = np.fromfile(tmp_path / "run" / f"clone_{j}" /
+                          "pops_final.datout").reshape(80, 40, 9)
+        assert np.array_equal(fin, final[j])
+
+
 WARMUP = os.path.join(PKG, "driver", "lbm_warmup")
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
