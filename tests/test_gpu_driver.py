@@ -197,23 +197,34 @@ def _channel_input(tmp, Nc, extra):
     return inp
 
 
-@pytest.mark.parametrize("force_io", [1, 0])
-def test_driver_channel_mode_matches_the_library(tmp_path, lbc, force_io):
+@pytest.mark.parametrize("force_io,storage,extra",
+                         [(1, "f64", ""), (0, "f64", ""),
+                          (1, "f16", "passes = 3\nthreads = 256\n"),
+                          (0, "bf16", "")],
+                         ids=["loop-f64", "device-f64", "loop-f16-pinned",
+                              "device-bf16"])
+def test_driver_channel_mode_matches_the_library(tmp_path, lbc, force_io,
+                                                 storage, extra):
     """mode = channel: the driver's interval loop (force_io = 1) and its
-    all-device path (force_io = 0: one lbc_advance of every interval) give
+    all-device path (force_io = 0: one lbc_advance of every interval), in
+    FP64 and in shifted storage, with a tuned or a pinned kernel shape, give
     the same labels, mean weights and per-step drag as the same calls made
     through the library directly, bit for bit."""
     Nc = 12
     inp = _channel_input(tmp_path, Nc, f"force_io = {force_io}\nvtk = 0\n"
-                                       "dump_states = 1\n")
-    print(f"case: channel driver force_io={force_io}", flush=True)
+                                       f"dump_states = 1\nstorage = {storage}\n"
+                                       + extra)
+    code = {"f64": lbc.STORE_F64, "f16": lbc.STORE_F16,
+            "bf16": lbc.STORE_BF16}[storage]
+    print(f"case: channel driver force_io={force_io} storage={storage} "
+          f"{extra!r}", flush=True)
     r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr + r.stdout
     run = tmp_path / "run"
     spec = lbc.channel(80, 40, 6, 20, 20, 0.55, 0.05, Nc, 12, k=1.0,
                        sigma=1e-4)
-    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, seed=3) as s:
+    with lbc.Sampler(spec, mode=lbc.MODE_CHANNEL, storage=code, seed=3) as s:
         s.init_equilibrium(0, 1e-3)
         if force_io:
             labels, drag = [], []
