@@ -23,6 +23,8 @@
  *    one lattice after another.
  *  - A context is single-threaded.  All device work is ordered on the
  *    context's own stream; functions that return host data synchronise it.
+ *  - Every call makes the context's device current and restores the
+ *    caller's current device before it returns.
  */
 #ifndef LBC_H
 #define LBC_H
@@ -133,7 +135,9 @@ const char* lbc_last_error(const lbc_ctx* ctx);
  * Replaces the pops-file reads of src/mpi_LBM_GK_pops_perturb.cpp:291,305. */
 int lbc_load_states(lbc_ctx* ctx, int inst, int clone0, int n,
                     const double* aos);
-/* Read n lattices back (AoS FP64, unshifted). */
+/* Read n lattices back (AoS FP64, unshifted): the layout of the pops files
+ * the reference's warm-up program writes (src/lbm_src/main_bodyforce.cpp:
+ * 179-182) and its cloning driver reads. */
 int lbc_store_states(lbc_ctx* ctx, int inst, int clone0, int n, double* aos);
 
 /* Equilibrium initial state feq(rho=1, u=(u_in + a(2U-1), a(2U-1))) per
@@ -224,7 +228,8 @@ int lbc_get_field_maps(lbc_ctx* ctx, int inst, double* maps);
  * the reference's VTK writer, src/lbm_src/write_vtk.cpp:38-49). */
 int lbc_get_macro(lbc_ctx* ctx, int inst, int clone0, int n, double* out);
 
-/* Wait for the context's stream. */
+/* Wait for the context's stream (the reference's MPI_Barrier points, e.g.
+ * src/mpi_LBM_GK_pops_perturb.cpp:471,507, have no other counterpart here). */
 int lbc_sync(lbc_ctx* ctx);
 
 /* Average device time (ms) of the step kernel over the last
