@@ -295,6 +295,43 @@ def test_ref_perturbation(O, lbc):
             assert np.array_equal(got[j], ref)
 
 
+@pytest.mark.parametrize("storage,tol", [(1, 2e-6), (2, 6e-5)])
+def test_ref_perturbation_shifted_storage(O, lbc, storage, tol):
+    """The snapshot perturbation on FP32- and FP16-shifted states: the mix
+    is computed in FP64 from the stored values plus the rest weights, then
+    stored shifted again.  After the perturbation and one step, every clone
+    is within the storage's rounding of the FP64 oracle's, and the marked
+    clones moved by far more than that."""
+    L, N, NN = 4, 4, 3
+    spec = lbc.ref_box(L, 0.53, 0.05, N, 1, alpha=1.0, eps=0.3, n_snapshots=NN)
+    g = O.ref_box_geom(L, 0.53, 0.05)
+    rng = np.random.default_rng(29)
+    states = np.stack([rand_state(O, g, rng) for _ in range(N)])
+    snaps = np.stack([rand_state(O, g, rng, amp=1e-2) for _ in range(NN)])
+    pref = rng.random((N, NN))
+    marks = np.array([0, 1, 1, 0], np.uint8)
+    print(f"case: REF snapshot perturbation storage={storage}", flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_REF, storage=storage,
+                     threads=256) as s:
+        s.load_states(0, 0, states)
+        start = s.store_states(0)  # the states as stored
+        s.set_snapshots(0, snaps)
+        s.set_marks(0, marks)
+        s.run_interval(prefactors=pref)
+        got = s.store_states(0)
+    for j in range(N):
+        ref = start[j].copy()
+        if marks[j]:
+            O.perturb(ref, snaps, 0.3, pref[j])
+        plain = start[j].copy()
+        O.run_steps(g, ref, 1, spec.force_scale, spec.force_shift, spec.dt)
+        O.run_steps(g, plain, 1, spec.force_scale, spec.force_shift, spec.dt)
+        err = np.abs(got[j] - ref).max()
+        assert err < tol, (j, err)
+        if marks[j]:
+            assert np.abs(ref - plain).max() > 20 * tol
+
+
 def test_advance_graph_matches_stepwise(lbc):
     """lbc_advance (captured graph, device draws) equals the same sequence
     issued call by call."""
