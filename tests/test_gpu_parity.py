@@ -102,6 +102,53 @@ def test_channel_interval_f64_bitexact(O, lbc, M, passes):
         assert lw[j] == 0.01 * acc
 
 
+@pytest.mark.parametrize("Lx,Ly,side,cx,cy",
+                         [(8, 8, 2, 4, 4), (63, 12, 4, 20, 6),
+                          (64, 12, 4, 20, 6), (65, 13, 4, 20, 6),
+                          (130, 9, 2, 60, 4)])
+@pytest.mark.parametrize("mode", ["channel", "ref"])
+def test_lattice_shapes_f64_bitexact(O, lbc, Lx, Ly, side, cx, cy, mode):
+    """Lattice widths where a 64-site wave covers several rows (8, 63),
+    exactly one (64), or straddles a row end (65, 130), the smallest lattice
+    accepted (8x8), a 9-row lattice whose walls are close to the square,
+    both x-boundary rules (inlet/outlet, periodic), one, two and three
+    passes per sweep: FP64 fields, drag and observables bit-exact against
+    the oracle over 7 steps."""
+    M = 7
+    spec = lbc.channel(Lx, Ly, side, cx, cy, 0.56, 0.05, 2, M, k=0.01)
+    if mode == "ref":
+        spec.u_in = 0.0
+        spec.beta = 1e-5
+    m = O.MODE_CHANNEL if mode == "channel" else O.MODE_REF
+    g = geom_of(O, spec, m)
+    rng = np.random.default_rng(Lx * 100 + Ly)
+    states = np.stack([rand_state(O, g, rng, amp=1e-2) for _ in range(2)])
+    for passes in (1, 2, 3):
+        print(f"case: {mode} {Lx}x{Ly} passes={passes}", flush=True)
+        try:
+            s = lbc.Sampler(spec, mode=lbc.MODE_CHANNEL if mode == "channel"
+                            else lbc.MODE_REF, threads=256, passes=passes)
+        except lbc.LbcError as e:
+            # the periodic wrap reaches 2*Lx sites: at 130 wide the two FP64
+            # rings of three passes need 2048 sites each and exceed LDS
+            assert "do not fit" in str(e) and mode == "ref" and Lx == 130
+            assert passes == 3
+            continue
+        with s:
+            s.load_states(0, 0, states)
+            s.run_interval()
+            got = s.store_states(0)
+            A = s.drag(0)
+            obs, _ = s.logweights(0)
+        for j in range(2):
+            ref = states[j].copy()
+            acc, Aref = O.run_steps(g, ref, M, spec.force_scale,
+                                    spec.force_shift, spec.dt)
+            assert np.array_equal(got[j], ref), (passes, j)
+            assert np.array_equal(A[j], Aref), (passes, j)
+            assert obs[j] == acc
+
+
 @pytest.mark.parametrize("passes", [1, 2, 3])
 def test_batched_uneven_instances_f64_bitexact(O, lbc, passes):
     """Eight samplers of four lattice sizes and two biases in one launch per
