@@ -149,6 +149,43 @@ def test_lattice_shapes_f64_bitexact(O, lbc, Lx, Ly, side, cx, cy, mode):
             assert obs[j] == acc
 
 
+def test_sixty_four_instances_in_one_context(O, lbc):
+    """The most instances a context takes (64), lattices of 16..31 x 12,
+    one to three clones each, biases and interval lengths varying: every
+    instance's clones bit-exact against the oracle after one interval, then
+    per-instance selection and copies, then two more cloning steps from the
+    captured graphs with finite mean weights everywhere."""
+    specs = []
+    for i in range(64):
+        Lx = 16 + (i % 16)
+        specs.append(lbc.channel(Lx, 12, 2, Lx // 3, 6, 0.55 + 0.001 * i, 0.05,
+                                 1 + i % 3, 2 + i % 4, k=0.001 * (i + 1),
+                                 sigma=1e-4))
+    print("case: 64 instances in one context", flush=True)
+    with lbc.Sampler(specs, mode=lbc.MODE_CHANNEL, seed=11, threads=256,
+                     passes=2) as s:
+        for i in range(64):
+            s.init_equilibrium(i, 1e-3)
+        init = [s.store_states(i) for i in range(64)]
+        s.run_interval()
+        for i, sp in enumerate(specs):
+            g = geom_of(O, sp, O.MODE_CHANNEL)
+            got = s.store_states(i)
+            obs, _ = s.logweights(i)
+            for j in range(sp.n_clones):
+                ref = init[i][j].copy()
+                acc, _ = O.run_steps(g, ref, sp.steps)
+                assert np.array_equal(got[j], ref), (i, j)
+                assert obs[j] == acc, (i, j)
+        for i in range(64):
+            s.select(i)
+        s.gather()
+        s.advance(2)
+        for i in range(64):
+            R = s.R_history(i)
+            assert len(R) == 3 and np.all(np.isfinite(R)) and np.all(R > 0), i
+
+
 @pytest.mark.parametrize("passes", [1, 2, 3])
 def test_batched_uneven_instances_f64_bitexact(O, lbc, passes):
     """Eight samplers of four lattice sizes and two biases in one launch per
