@@ -464,6 +464,36 @@ def test_low_precision_channel_interval(O, lbc, storage, tol):
         assert np.max(np.abs(got[j] - ref)) < 50 * tol
 
 
+@pytest.mark.parametrize("storage,tol", [(1, 2e-6), (2, 2e-4), (3, 5e-3)])
+def test_low_precision_ref_interval(O, lbc, storage, tol):
+    """The reference mode (periodic x, Guo body force, stress-integral drag)
+    on shifted storage vs the FP64 oracle over one 80-step interval of the
+    reference driver's L = 8 box, from developed-flow states: interval drag
+    within `tol` relative, fields within 10 * tol."""
+    spec = lbc.ref_box(8, 0.6, 0.1, 3, 80, alpha=1.0, zero_mean=False)
+    g = O.ref_box_geom(8, 0.6, 0.1)
+    rng = np.random.default_rng(41)
+    init = np.stack([rand_state(O, g, rng, amp=1e-2, u0=0.1)
+                     for _ in range(3)])
+    print(f"case: REF low precision storage={storage} 33x33 M=80", flush=True)
+    with lbc.Sampler(spec, mode=lbc.MODE_REF, storage=storage,
+                     threads=256) as s:
+        s.load_states(0, 0, init)
+        start = s.store_states(0)
+        s.run_interval()
+        obs, _ = s.logweights(0)
+        got = s.store_states(0)
+    for j in range(3):
+        ref = start[j].copy()
+        acc, _ = O.run_steps(g, ref, 80, spec.force_scale, spec.force_shift,
+                             spec.dt)
+        rel = abs(obs[j] - acc) / abs(acc)
+        print(f"storage {storage} clone {j}: drag {obs[j]:.9g} vs {acc:.9g} "
+              f"rel {rel:.2e}")
+        assert rel < tol
+        assert np.max(np.abs(got[j] - ref)) < 10 * tol
+
+
 @pytest.mark.parametrize("storage", [1, 2, 3])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_multi_pass_sweeps_equal_single_passes(O, lbc, storage, mode):
