@@ -453,9 +453,10 @@ def measured_traffic(run: dict, workload: str) -> int | None:
 
 
 def stream_ceilings() -> dict | None:
-    """Measured streaming rates of the box (GB/s, read + write) from the
-    stream_probe summary under profiles/: plain copy, read, write, and the
-    step kernel's FP16 access pattern without arithmetic; None if absent."""
+    """Measured streaming rates of the profiled box (GB/s, read + write) from
+    the stream_probe summary under profiles/: plain copy, read, write, and
+    the step kernel's FP16 access pattern without arithmetic; None if
+    absent."""
     pat = re.compile(r"^stream_ceiling (\w+): (\d+)\s*$")
     out = {}
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
@@ -464,10 +465,31 @@ def stream_ceilings() -> dict | None:
                 m = pat.match(line.strip())
                 if m:
                     out[m.group(1)] = int(m.group(2))
+    if out:
+        out["source"] = "profiles/stream_ceilings.txt"
     return out or None
 
 
-def roofline(run: dict, workload: str) -> dict:
+def live_stream_ceilings() -> dict | None:
+    """The same rates measured now, on this box, by running stream_probe as
+    a child process (a few seconds); None if it is not built or fails.
+    Boxes differ: the copy rate has ranged over 5.4-5.7 TB/s."""
+    exe = os.path.join(ROOT, "cloning-lbm-mpi_amd", "stream_probe")
+    if not os.path.exists(exe):
+        return None
+    try:
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return None
+    out = {k: v for k, v in json.loads(lines[-1]).items() if k != "unit"}
+    out["source"] = "measured in this run"
+    return out
+
+
+def roofline(run: dict, workload: str, live: dict | None = None) -> dict:
     """HBM roofline of the step kernel.  achieved = algorithmic bytes
     (2*9*b per lattice update) / average kernel time; with P LBM passes
     per HBM sweep the kernel moves about 1/P of that, so the measured
@@ -487,11 +509,15 @@ def roofline(run: dict, workload: str) -> dict:
         dram = traffic / (run["kernel_ms"] * 1e-3) / 1e9
         out["dram_achieved"] = round(dram, 1)
         out["dram_frac"] = round(dram / HBM_PEAK_GBS, 4)
-        ceil = stream_ceilings()
+        ceil = live or stream_ceilings()
         if ceil:
             out["stream_ceilings"] = ceil
             if "copy" in ceil:
                 out["dram_frac_of_copy"] = round(dram / ceil["copy"], 4)
+            if "pattern_f16" in ceil and run["storage"] in ("f16", "bf16"):
+                # the 2-byte pull pattern with no arithmetic
+                out["dram_frac_of_pattern"] = round(dram / ceil["pattern_f16"],
+                                                    4)
     return out
 
 
@@ -536,6 +562,11 @@ def main() -> int:
     ap.add_argument("--only", default="",
                     help="comma list of storages to time (f64,f32,f16,bf16); "
                          "default: the workload's list")
+    ap.add_argument("--no-stream-probe", action="store_true",
+                    help="skip measuring this box's streaming ceilings (the "
+                         "roofline then quotes the profiled box's); runs "
+                         "restricted with --only or --shape, such as the "
+                         "profiling runs, skip it too")
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD,
                     choices=sorted(WORKLOADS),
                     help="channel workload to run (default %(default)s)")
@@ -556,6 +587,11 @@ def main() -> int:
             or WORKLOADS[wl]["storages"])
     codes = {"f64": lbc.STORE_F64, "f32": lbc.STORE_F32, "f16": lbc.STORE_F16,
              "bf16": lbc.STORE_BF16}
+    live = None
+    if not (args.no_stream_probe or args.only or args.shape):
+        print("bench: streaming ceilings of this box", file=sys.stderr,
+              flush=True)
+        live = live_stream_ceilings()
     runs = {}
     for name in want:
         print(f"bench: {wl}: timing {name} ({K} intervals after {W} warm-up)",
@@ -606,7 +642,7 @@ def main() -> int:
                     else head["storage"] + "-shifted"),
         "data": "synthetic",
         "config": config,
-        "roofline": roofline(head, wl),
+        "roofline": roofline(head, wl, live),
         "gather_roofline": head.get("gather"),
         "cpu_baseline": cpu,
         "paths": runs,
