@@ -459,7 +459,12 @@ int autotune(lbc_ctx* c) {
       L.nt = nt;
       L.passes = passes;
       L.rch = rch;
-      HIPC(c, lbc::launch_interval(L));  // warm-up
+      // a shape the device refuses to launch (a configuration error,
+      // reported at launch) is skipped; the others are still timed
+      if (lbc::launch_interval(L) != hipSuccess) {  // warm-up
+        (void)hipGetLastError();
+        continue;
+      }
       HIPC(c, hipEventRecord(e0, c->stream));
       for (int r = 0; r < 2; r++) HIPC(c, lbc::launch_interval(L));
       HIPC(c, hipEventRecord(e1, c->stream));
@@ -477,6 +482,8 @@ int autotune(lbc_ctx* c) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (best >= 1e30f)
+    return fail(c, LBC_E_HIP, "autotune: no step-kernel shape could be launched");
   c->threads = best_t;
   c->vec = best_v;
   c->nt = best_nt;
