@@ -163,6 +163,14 @@ int validate(const lbc_config* cfg, std::string& why) {
     if ((long long)s.Lx * s.Ly * 9 > (1LL << 30) || s.Lx > 32767 ||
         s.Ly > 32767)
       return bad("lattice too large");
+    {
+      // the step kernels address one clone through a buffer resource with
+      // 32-bit byte offsets
+      const long long cstride = ((long long)9 * s.Lx * s.Ly + 127) / 128 * 128;
+      if (cstride * (long long)elem_bytes(cfg->storage) >= (1LL << 31))
+        return bad("lattice too large for this storage: one clone must stay "
+                   "below 2 GiB");
+    }
     if (s.xmin < 2 || s.xmax > s.Lx - 3 || s.ymin < 2 || s.ymax > s.Ly - 3)
       return bad("square must keep two fluid nodes from every boundary");
     if (s.xmax - s.xmin < 2 || s.ymax - s.ymin < 2)

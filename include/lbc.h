@@ -55,7 +55,10 @@ extern "C" {
 #define LBC_STORE_BF16 3
 
 typedef struct lbc_instance {
-  int Lx, Ly;                 /* lattice (Dx, Dy in the reference) */
+  int Lx, Ly;                 /* lattice (Dx, Dy in the reference); one
+                                 clone's populations must stay below 2 GiB
+                                 in the chosen storage (FP64: about 29.8M
+                                 sites) */
   int xmin, xmax, ymin, ymax; /* square ring nodes; solid = open interior */
   int n_clones;               /* Nc */
   int steps;                  /* LBM steps per cloning interval (M) */

@@ -53,6 +53,9 @@ def test_create_rejects_invalid_configurations(lbc):
     assert "steps" in _bad(lbc, steps=0)
     assert "rank_blocks" in _bad(lbc, rank_blocks=3)
     assert "lattice" in _bad(lbc, Lx=6)
+    # one FP64 clone of 6000x6000 is 2.6 GB: beyond 32-bit byte offsets;
+    # the same lattice fits in FP16 storage, so only the bytes are refused
+    assert "2 GiB" in _bad(lbc, Lx=6000, Ly=6000)
     assert "mode" in _bad(lbc, mode=7)
     assert "storage" in _bad(lbc, storage=9)
     assert "threads" in _bad(lbc, threads=100)
