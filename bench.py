@@ -24,10 +24,12 @@ are close enough that a cloning step copies only a few clones.
 measurements, with the storages listed for it): channel200x50 (16 clones,
 FP64), channel400x100-n512 (512 clones, FP64), batched8 (8 samplers of four
 lattice sizes and two biases in one launch, FP32/FP64/FP16) and
-channel800x200 (2048 clones, 400 steps, FP16/FP64), and ref129, the
+channel800x200 (2048 clones, 400 steps, FP16/FP64), ref129, the
 reference's own mode (periodic box with body force, stress-integral drag and
 snapshot perturbation) at its L = 32 size, 4096 clones, FP64/FP16/FP32, with
-the reference's own kernels as the CPU baseline.
+the reference's own kernels as the CPU baseline, and forcing129, its forcing
+driver at the same size (random-field body force in the first half of every
+interval, fields drawn on the device).
 
 Prints one JSON line.  Only N=1 is supported (one GPU).
 """
@@ -112,6 +114,20 @@ WORKLOADS = {
         ref=dict(L=32, tau=0.6, U0=0.1, n_clones=4096, dT=1.0, alpha=50.0,
                  eps=0.01, nn=10),
         storages=["f64", "f16", "f32"]),
+    # the reference's forcing driver at the same size: the first half of
+    # every interval under a spatial body force (1 + eps*map)*beta0, map a
+    # Gaussian random field drawn per clone and interval on the device
+    "forcing129": dict(
+        desc="reference forcing mode: square side L 32 in a periodic-x "
+             "129x129 box with walls, U0 0.1, tau 0.6, 4096 clones, 320 LBM "
+             "steps per cloning interval of which the first 160 (dT0 0.5 at "
+             "T0 = L/U0) run under (1 + 0.1*map)*beta0 with a fresh random "
+             "field per clone and interval (covariance length 32, hipFFT), "
+             "alpha 50",
+        mode="ref",
+        forcing=dict(L=32, tau=0.6, U0=0.1, n_clones=4096, dT=1.0, dT0=0.5,
+                     eps=0.1, alpha=50.0),
+        storages=["f64", "f16", "f32"]),
 }
 DEFAULT_WORKLOAD = "channel400x100"
 
@@ -132,6 +148,12 @@ def specs_of(workload: str, max_clones: int | None = None
         steps = math.floor(r["dT"] * r["L"] / r["U0"])
         return [lbc.ref_box(r["L"], r["tau"], r["U0"], n, steps, r["alpha"],
                             eps=r["eps"], n_snapshots=r["nn"])]
+    if "forcing" in wl:
+        r = wl["forcing"]
+        n = r["n_clones"] if max_clones is None else min(max_clones,
+                                                         r["n_clones"])
+        return [lbc.forcing_box(r["L"], r["tau"], r["U0"], n, r["dT"],
+                                r["dT0"], r["eps"], r["alpha"])]
     out = []
     for w in wl["instances"]:
         n = w["n_clones"] if max_clones is None else min(max_clones,
@@ -199,7 +221,9 @@ def time_storage(storage: int, K: int, W: int, specs: list, mode: int,
     lup1 = sum(sp.n_clones * sp.sites * sp.steps for sp in specs)
     lups = lup1 * K
     bpl = lbc.bytes_per_lup(storage)
-    alg_bytes = lup1 * bpl
+    # plus, in forced steps, the clone's random-field map (8 B per site)
+    alg_bytes = lup1 * bpl + sum(8 * sp.n_clones * sp.sites * sp.forced_steps
+                                 for sp in specs)
     achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else float("nan")
     return dict(
         storage=lbc.STORAGE_NAMES[storage],
@@ -345,17 +369,20 @@ def cpu_baseline(workload: str, lups_per_core: float = 2.4e8,
     core with its clones run serially, like one MPI rank per core, on a
     bounded sample of the workload: clones of its largest lattice, `steps`
     steps each, about lups_per_core lattice updates per thread (30 clones of
-    400x100).  In the reference mode the workers run the reference's own
-    kernels (oracle/_ref/libref.so, its sources at its -O3, in its driver's
-    step loop) when that build is present: kind "reference".  Otherwise,
-    and for the channel workloads, which have no reference code, they run
-    the oracle's port of the reference's AoS push step: kind "port"."""
+    400x100).  In the reference and forcing modes the workers run the
+    reference's own kernels (oracle/_ref/libref.so, its sources at its -O3,
+    in its driver's step loop, uniform force: a forced step adds one
+    multiply-add per site) when that build is present: kind "reference".
+    Otherwise, and for the channel workloads, which have no reference code,
+    they run the oracle's port of the reference's AoS push step: kind
+    "port"."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     wl = WORKLOADS[workload]
-    if "ref" in wl:
-        r = wl["ref"]
+    ref_mode = "ref" in wl or "forcing" in wl
+    if ref_mode:
+        r = wl.get("ref") or wl["forcing"]
         g = O.ref_box_geom(r["L"], r["tau"], r["U0"])
         base = np.broadcast_to(O.W, (g.Lx, g.Ly, 9)).copy()
     else:
@@ -363,7 +390,7 @@ def cpu_baseline(workload: str, lups_per_core: float = 2.4e8,
         g = O.channel_geom(w["Lx"], w["Ly"], w["side"], w["cx"], w["cy"],
                            w["tau"], w["u_in"])
         base = O.init_channel(g, 1e-3, 0, 0)
-    kind = "reference" if "ref" in wl and O.ref_available() else "port"
+    kind = "reference" if ref_mode and O.ref_available() else "port"
     sample_clones_per_core = max(1, round(lups_per_core /
                                           (g.Lx * g.Ly * steps)))
     cores = max(1, min(16, os.cpu_count() or 1))

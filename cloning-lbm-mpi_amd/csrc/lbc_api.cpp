@@ -976,7 +976,9 @@ int capture_step(lbc_ctx* c) {
                                 c->n_max, 0, 0, seed_lo(c), seed_hi(c),
                                 c->stream) == hipSuccess;
       if (ok) ok = lbc::launch_gather(L, seed_lo(c), seed_hi(c)) == hipSuccess;
-      if (ok && c->cfg.mode == LBC_MODE_REF)
+      bool snapshots = false;
+      for (const auto& s : c->inst) snapshots |= s.n_snapshots > 0;
+      if (ok && c->cfg.mode == LBC_MODE_REF && snapshots)
         ok = lbc::launch_perturb(L, 1, seed_lo(c), seed_hi(c)) == hipSuccess;
     }
     hipGraph_t g = nullptr;

@@ -355,8 +355,11 @@ __global__ void __launch_bounds__(256)
   using S = typename T::S;
   const WorkItem wi = items[blockIdx.x];
   const InstDev* __restrict__ I = insts + wi.inst;
-  if (!I->marks[wi.clone]) return;
   const int NN = I->n_snapshots;
+  // an instance without snapshots (the forcing mode) is not perturbed: with
+  // no snapshot the mix f' would be zero and r = |f|/|f'| infinite; its
+  // marks only record the last selection's copies
+  if (NN <= 0 || !I->marks[wi.clone]) return;
   const int plane = I->plane;
   const int par = counters->parity[wi.inst];
   S* f = (S*)I->buf[par] + (long long)wi.clone * I->clone_stride;

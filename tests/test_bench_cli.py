@@ -20,6 +20,7 @@ def test_workloads_are_valid_samplers(bench):
     for name, w in bench.WORKLOADS.items():
         specs = bench.specs_of(name)
         ref = bench.mode_of(name) == bench.lbc.MODE_REF
+        forcing = "forcing" in w
         assert len(specs) == (1 if ref else len(w["instances"])) >= 1, name
         assert w["storages"] and set(w["storages"]) <= {"f64", "f32", "f16",
                                                         "bf16"}
@@ -28,7 +29,10 @@ def test_workloads_are_valid_samplers(bench):
             assert 2 <= sp.xmin < sp.xmax <= sp.Lx - 3, name
             assert 2 <= sp.ymin < sp.ymax <= sp.Ly - 3, name
             assert sp.tau > 0.5 and sp.n_clones >= 1
-            if ref:  # body force, snapshot perturbation, no noise
+            if forcing:  # random-field force in the first forced_steps
+                assert sp.beta > 0 and 0 < sp.forced_steps < sp.steps
+                assert sp.field_eps > 0 and sp.n_snapshots == 0
+            elif ref:  # body force, snapshot perturbation, no noise
                 assert sp.beta > 0 and sp.n_snapshots > 0 and sp.eps > 0
             else:
                 assert sp.noise_sigma > 0 and sp.beta == 0
@@ -44,6 +48,8 @@ def test_workload_sizes(bench):
     assert sizes["channel800x200"] == [(800, 200, 2048, 400)]
     # the reference driver's box for L = 32: Dx = Dy = 129, dT*T0 = 320
     assert sizes["ref129"] == [(129, 129, 4096, 320)]
+    assert sizes["forcing129"] == [(129, 129, 4096, 320)]
+    assert bench.specs_of("forcing129")[0].forced_steps == 160
     (r,) = bench.specs_of("ref129")
     assert (r.xmin, r.xmax, r.ymin, r.ymax) == (64, 96, 48, 80)
     b8 = bench.specs_of("batched8")

@@ -168,7 +168,7 @@ def test_forced_advance_graph_matches_stepwise(lbc):
     """lbc_advance draws each interval's fields outside the captured graph;
     the result equals the same sequence issued call by call."""
     spec = lbc.forcing_box(8, 0.55, 0.05, 8, dT=0.1, dT0=0.04, eps=0.5,
-                           alpha=1.0, field_sigma=3.0)
+                           alpha=400.0, field_sigma=3.0)
     # T0 = 160: floor(0.04*160) = 6 forced, floor(0.06*160) = 9 uniform
     assert spec.forced_steps == 6 and spec.steps == 15
     print("case: forced advance graph vs stepwise, REF F64", flush=True)
@@ -177,15 +177,51 @@ def test_forced_advance_graph_matches_stepwise(lbc):
         a.init_equilibrium(0, 1e-3)
         b.init_equilibrium(0, 1e-3)
         a.advance(3)
+        copies = 0
         for _ in range(3):
             b.run_interval()
-            b.select(0)
+            copies += len(b.select(0).pairs)
             b.gather()
+        # the bias is large enough that clones are copied: the forcing mode
+        # has no snapshots, so a copy is not perturbed in either path
+        assert copies > 0
         assert np.array_equal(a.field_maps(0), b.field_maps(0))
-        assert np.array_equal(a.store_states(0), b.store_states(0))
+        sa = a.store_states(0)
+        assert np.all(np.isfinite(sa)) and np.array_equal(sa, b.store_states(0))
         Ra, Rb = a.R_history(0), b.R_history(0)
         assert len(Ra) == 3 and np.array_equal(Ra, Rb)
         assert np.all(np.isfinite(Ra)) and np.all(Ra > 0)
+
+
+def test_forced_instance_beside_a_perturbed_one(lbc):
+    """A forced instance (no snapshots) and a snapshot-perturbed instance in
+    one context, with copies in both: the perturbation kernel runs for the
+    second instance's copies and leaves the first's alone, in the
+    call-by-call sequence and in the captured graphs (a copy of a clone with
+    no snapshots to mix was once turned into NaN)."""
+    a = lbc.forcing_box(8, 0.55, 0.05, 8, dT=0.1, dT0=0.04, eps=0.5,
+                        alpha=400.0, field_sigma=3.0)
+    b = lbc.ref_box(6, 0.6, 0.05, 8, 12, alpha=400.0, eps=0.1, n_snapshots=3)
+    print("case: forced + perturbed instances with copies", flush=True)
+    for graph in (False, True):
+        with lbc.Sampler([a, b], mode=lbc.MODE_REF, seed=9) as s:
+            s.init_equilibrium(0, 1e-3)
+            s.init_equilibrium(1, 1e-3)
+            s.set_snapshots(1, s.store_states(1, 0, 3))
+            copies = [0, 0]
+            if graph:
+                s.advance(3)
+            else:
+                for _ in range(3):
+                    s.run_interval()
+                    for i in range(2):
+                        copies[i] += len(s.select(i).pairs)
+                    s.gather()
+                assert copies[0] > 0 and copies[1] > 0
+            for i in range(2):
+                assert np.all(np.isfinite(s.store_states(i))), (graph, i)
+                R = s.R_history(i)
+                assert len(R) == 3 and np.all(np.isfinite(R)), (graph, i)
 
 
 def test_forced_and_uniform_instances_batched(O, lbc):
