@@ -76,7 +76,7 @@ struct lbc_ctx {
   // forcing mode, per instance
   bool forced = false;                // some instance has forced steps
   std::vector<hipfftHandle> fft;      // batched in-place 2-D Z2Z plans
-  std::vector<int> fchunk;            // clones per FFT batch
+  std::vector<int> fchunk;            // transforms per FFT batch (two clones each)
   std::vector<char> maps_set;         // next interval uses injected maps
   // copy kernel of the last lbc_gather: events around it, bytes it moved
   std::vector<int> last_copies;       // per instance, from the last lbc_select
@@ -287,8 +287,13 @@ int field_setup(lbc_ctx* c, int i) {
   d.fmx = fmx;
   d.fmy = fmy;
   const size_t per = 2 * sizeof(double) * fmn;
-  const int chunk = (int)std::max<size_t>(
-      1, std::min<size_t>((size_t)s.n_clones, kFieldWorkBytes / per));
+  // One transform serves two clones (the real and imaginary parts).  The
+  // plan runs its whole batch every time, so the transforms are spread
+  // evenly over the fewest batches that fit the work buffer.
+  const size_t transforms = ((size_t)s.n_clones + 1) / 2;
+  const size_t fit = std::max<size_t>(1, kFieldWorkBytes / per);
+  const size_t batches = (transforms + fit - 1) / fit;
+  const int chunk = (int)((transforms + batches - 1) / batches);
   c->fchunk[i] = chunk;
   const size_t map_bytes = sizeof(double) * (size_t)s.n_clones * d.plane;
   if (dalloc(c, &d.field_map, map_bytes) ||
@@ -296,6 +301,12 @@ int field_setup(lbc_ctx* c, int i) {
       dalloc(c, &d.field_work, per * chunk))
     return LBC_E_NOMEM;
   HIPC(c, hipMemsetAsync(d.field_map, 0, map_bytes, c->stream));
+  // the shifted storages collide in FP32 and read an FP32 copy of the maps:
+  // half the bytes of each forced step's map read
+  if (c->cfg.storage != LBC_STORE_F64) {
+    if (dalloc(c, &d.field_map32, map_bytes / 2)) return LBC_E_NOMEM;
+    HIPC(c, hipMemsetAsync(d.field_map32, 0, map_bytes / 2, c->stream));
+  }
   HIPC(c, hipMemsetAsync(d.field_work, 0, per * chunk, c->stream));
 
   std::vector<double> G(2 * fmn, 0.0);
@@ -345,8 +356,9 @@ int field_setup(lbc_ctx* c, int i) {
 }
 
 // Random fields of the coming interval for every forced instance: Philox
-// normals, the batched FFT and the lattice block, chunk by chunk; an
-// instance whose maps the caller set (lbc_set_field_maps) keeps them once.
+// normals, the batched FFT and the lattice block, chunk by chunk, each
+// transform giving two clones' fields; an instance whose maps the caller set
+// (lbc_set_field_maps) keeps them once.
 int enqueue_fields(lbc_ctx* c) {
   for (int i = 0; i < c->cfg.n_instances; i++) {
     const InstDev& d = c->idh[i];
@@ -356,8 +368,10 @@ int enqueue_fields(lbc_ctx* c) {
       continue;
     }
     auto* work = reinterpret_cast<hipfftDoubleComplex*>(d.field_work);
-    for (int c0 = 0; c0 < d.n_clones; c0 += c->fchunk[i]) {
-      const int n = std::min(c->fchunk[i], d.n_clones - c0);
+    // fchunk transforms per batch, two clones per transform
+    const int per_batch = 2 * c->fchunk[i];
+    for (int c0 = 0; c0 < d.n_clones; c0 += per_batch) {
+      const int n = std::min(per_batch, d.n_clones - c0);
       HIPC(c, lbc::launch_field_draw(c->idd, c->counters, i, c0, n,
                                      d.fmx * d.fmy, seed_lo(c), seed_hi(c),
                                      c->stream));
@@ -1086,6 +1100,8 @@ int lbc_set_field_maps(lbc_ctx* c, int inst, const double* maps) {
             maps[j * plane + y + (size_t)d.Ly * x];
   HIPC(c, hipMemcpyAsync(d.field_map, tmp.data(), sizeof(double) * tmp.size(),
                          hipMemcpyHostToDevice, c->stream));
+  HIPC(c, lbc::launch_field_narrow(c->idd, inst, d.n_clones, d.plane,
+                                   d.field_map32 != nullptr, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   c->maps_set[inst] = 1;
   return LBC_OK;

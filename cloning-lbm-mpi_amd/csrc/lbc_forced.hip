@@ -8,9 +8,13 @@
 // torus.  With the embedding's eigenvalues lambda (the forward 2-D DFT of the
 // wrapped covariance divided by fmx*fmy, computed once by the host), a field
 // is Re F on the lattice block, F the forward 2-D DFT of sqrt(lambda) * Z and
-// Z complex with independent standard normal parts (:127-178).  The host runs
-// the DFT (hipFFT, batched over clones) between k_field_draw and
-// k_field_extract.
+// Z complex with independent standard normal parts (:127-178).  Im F is a
+// second field of the same law, independent of Re F (the real and imaginary
+// parts of a circulant embedding draw; Dietrich & Newsam 1997, Wood & Chan
+// 1994), which the reference discards: here clones 2k and 2k + 1 of an
+// instance take Re F and Im F of one transform, so an interval runs half the
+// transforms.  The host runs the DFT (hipFFT, batched over clone pairs)
+// between k_field_draw and k_field_extract.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -20,17 +24,18 @@
 namespace lbc {
 namespace {
 
-// One Philox block per pair of embedding entries m = 2p, 2p + 1 of a clone:
-// Box-Muller gives (re, im) of Z_2p from its first two words and of Z_2p+1
-// from its last two.  The pair index p runs over blockIdx.x * blockDim.x +
-// threadIdx.x, the clone over blockIdx.y.
+// One Philox block per pair of embedding entries m = 2p, 2p + 1 of a clone
+// pair: Box-Muller gives (re, im) of Z_2p from its first two words and of
+// Z_2p+1 from its last two; the counter is the even clone's.  The pair index
+// p runs over blockIdx.x * blockDim.x + threadIdx.x, the transform slot
+// (clones clone0 + 2*slot and clone0 + 2*slot + 1) over blockIdx.y.
 __global__ void __launch_bounds__(256)
     k_field_draw(const InstDev* __restrict__ insts,
                  const Counters* __restrict__ counters, int inst, int clone0,
                  int fmn, uint32_t seed_lo, uint32_t seed_hi) {
   const InstDev* __restrict__ I = insts + inst;
   const int slot = blockIdx.y;
-  const int clone = clone0 + slot;
+  const int clone = clone0 + 2 * slot;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int m = 2 * p;
   if (m >= fmn) return;
@@ -50,22 +55,46 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// map[clone][y*Lx + x] = Re F[y + fmy*x] (generate_random_field.cpp:170-178
-// with the lattice's x as the embedding's row index).
+// map[clone][y*Lx + x] = Re F[y + fmy*x] for the even clone of a slot and
+// Im F[y + fmy*x] for the odd one (generate_random_field.cpp:170-178 with the
+// lattice's x as the embedding's row index); clone0 + n ends the chunk.  The
+// FP32 copy of the maps, where the instance has one, is written alongside.
 __global__ void __launch_bounds__(256)
-    k_field_extract(const InstDev* __restrict__ insts, int inst, int clone0) {
+    k_field_extract(const InstDev* __restrict__ insts, int inst, int clone0,
+                    int n) {
   const InstDev* __restrict__ I = insts + inst;
   const int slot = blockIdx.y;
   const int Lx = I->Lx, plane = I->plane, fmy = I->fmy;
   const long long fmn = (long long)I->fmx * fmy;
   const double* __restrict__ F = I->field_work + 2 * (long long)slot * fmn;
-  double* __restrict__ map =
-      I->field_map + (long long)(clone0 + slot) * plane;
+  const int c = clone0 + 2 * slot;
+  const bool odd = 2 * slot + 1 < n;
+  double* __restrict__ re = I->field_map + (long long)c * plane;
+  double* __restrict__ im = re + plane;
+  float* __restrict__ re32 =
+      I->field_map32 ? I->field_map32 + (long long)c * plane : nullptr;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < plane;
        s += gridDim.x * blockDim.x) {
     const int x = s % Lx, y = s / Lx;
-    map[s] = F[2 * (y + (long long)fmy * x)];
+    const long long k = 2 * (y + (long long)fmy * x);
+    const double a = F[k], b = F[k + 1];
+    re[s] = a;
+    if (odd) im[s] = b;
+    if (re32) {
+      re32[s] = (float)a;
+      if (odd) re32[s + plane] = (float)b;
+    }
   }
+}
+
+// field_map32 = (float)field_map over all n*plane entries of an instance
+// (maps the caller supplied).
+__global__ void __launch_bounds__(256)
+    k_field_narrow(const InstDev* __restrict__ insts, int inst, long long n) {
+  const InstDev* __restrict__ I = insts + inst;
+  for (long long s = blockIdx.x * (long long)blockDim.x + threadIdx.x; s < n;
+       s += (long long)gridDim.x * blockDim.x)
+    I->field_map32[s] = (float)I->field_map[s];
 }
 
 }  // namespace
@@ -82,7 +111,7 @@ hipError_t launch_field_draw(const InstDev* insts, const Counters* counters,
                              hipStream_t stream) {
   if (n <= 0) return hipSuccess;
   const int pairs = (fmn + 1) / 2;
-  const dim3 grid((pairs + 255) / 256, n);
+  const dim3 grid((pairs + 255) / 256, (n + 1) / 2);
   hipLaunchKernelGGL(k_field_draw, grid, dim3(256), 0, stream, insts, counters,
                      inst, clone0, fmn, seed_lo, seed_hi);
   return hipGetLastError();
@@ -92,8 +121,19 @@ hipError_t launch_field_extract(const InstDev* insts, int inst, int clone0,
                                 int n, int plane, hipStream_t stream) {
   if (n <= 0) return hipSuccess;
   const int bx = (plane + 255) / 256 < 64 ? (plane + 255) / 256 : 64;
-  hipLaunchKernelGGL(k_field_extract, dim3(bx, n), dim3(256), 0, stream, insts,
-                     inst, clone0);
+  hipLaunchKernelGGL(k_field_extract, dim3(bx, (n + 1) / 2), dim3(256), 0,
+                     stream, insts, inst, clone0, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_field_narrow(const InstDev* insts, int inst, int n_clones,
+                               int plane, bool has_f32, hipStream_t stream) {
+  if (!has_f32) return hipSuccess;
+  const long long n = (long long)n_clones * plane;
+  const long long want = (n + 255) / 256;
+  const int blocks = (int)(want < 4096 ? want : 4096);
+  hipLaunchKernelGGL(k_field_narrow, dim3(blocks), dim3(256), 0, stream, insts,
+                     inst, n);
   return hipGetLastError();
 }
 

@@ -47,6 +47,7 @@ struct InstDev {
   int fmx, fmy;            // circulant embedding (2Lx-1) x (2Ly-1)
   double field_eps;        // beta(x,y) = (1 + field_eps*map(x,y))*beta
   double* field_map;       // [n_clones][plane] fields, site order y*Lx + x
+  float* field_map32;      // the same rounded to FP32 (shifted storage only)
   double* field_sqrt_ev;   // [fmx*fmy] square roots of the eigenvalues
   double* field_work;      // [chunk][fmx*fmy] complex (re, im), FFT in place
   // macroscopic output: rho, ux, uy at steps 0, E, 2E, ... (E = 0: off)
@@ -151,11 +152,13 @@ hipError_t launch_flip(const Launch& L, int n_inst, const int* flip_mask_dev);
 hipError_t launch_scgf(const InstDev* insts, int inst, int n, double dT,
                        double* lam, hipStream_t stream);
 // Random fields of the forcing mode (lbc_forced.hip), for clones
-// [clone0, clone0 + n) of one instance through its FFT work slots 0..n-1:
+// [clone0, clone0 + n) of one instance (clone0 even) through its FFT work
+// slots 0..(n+1)/2-1, slot k serving clones clone0 + 2k and clone0 + 2k + 1:
 // draw fills each slot with complex standard normals (Philox4x32-10 counter
-// (pair, clone, interval, kFieldStream + inst)) times the square-rooted
+// (pair, even clone, interval, kFieldStream + inst)) times the square-rooted
 // eigenvalues; after the caller's in-place 2-D forward FFT, extract writes
-// the real part of the lattice block to the clones' maps.  fmn = fmx*fmy.
+// the real part of the lattice block to the even clone's map and the
+// imaginary part to the odd one's.  fmn = fmx*fmy.
 constexpr uint32_t kFieldStream = 0xF1E1D000u;
 hipError_t launch_field_draw(const InstDev* insts, const Counters* counters,
                              int inst, int clone0, int n, int fmn,
@@ -163,6 +166,10 @@ hipError_t launch_field_draw(const InstDev* insts, const Counters* counters,
                              hipStream_t stream);
 hipError_t launch_field_extract(const InstDev* insts, int inst, int clone0,
                                 int n, int plane, hipStream_t stream);
+// field_map32 from field_map for all clones of an instance (maps the caller
+// set); nothing when the instance has no FP32 copy (has_f32 false).
+hipError_t launch_field_narrow(const InstDev* insts, int inst, int n_clones,
+                               int plane, bool has_f32, hipStream_t stream);
 hipError_t launch_perturb(const Launch& L, int philox_prefactors,
                           uint32_t seed_lo, uint32_t seed_hi);
 hipError_t launch_select(const InstDev* insts, Counters* counters, int inst0,

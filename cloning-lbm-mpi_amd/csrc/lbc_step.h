@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "lbc_internal.h"
 
 namespace lbc {
@@ -585,37 +587,52 @@ __device__ inline void collide_any(const typename Traits<ST>::C (&f)[9],
 }
 
 // Body force of a pass.  Drive<false>: the instance's uniform beta.
-// Drive<true>: in a forced pass (map set) the per-site force
+// Drive<true, MT>: in a forced pass (map set) the per-site force
 // beta(x, y) = (1 + eps*map[site])*beta of
 // src/lbm_src/streamCollCompute.cpp:63, in FP64 with the reference's
-// operation order; map holds the clone's random field in site order.
-template <bool FRC>
+// operation order; map holds the clone's random field in site order, FP64
+// for FP64 storage, its FP32 rounding for the shifted storages (which
+// collide in FP32).
+template <bool FRC, class MT = double>
 struct Drive {
   double beta;
   static constexpr bool kFrame = false;
   __device__ double at(int) const { return beta; }
 };
 
-template <>
-struct Drive<true> {
+template <class MT>
+struct Drive<true, MT> {
   double beta, eps;
-  const double* __restrict__ map;  // null outside the forced passes
+  const MT* __restrict__ map;  // null outside the forced passes
   static constexpr bool kFrame = false;
   __device__ double at(int site) const {
 #pragma clang fp contract(off)
-    return map ? (1.0 + eps * map[site]) * beta : beta;
+    return map ? (1.0 + eps * (double)map[site]) * beta : beta;
   }
 };
 
-template <bool FRC>
-__device__ inline Drive<FRC> drive_of(const InstDev* __restrict__ I, int clone,
-                                      bool forced) {
-  if constexpr (FRC)
-    return Drive<true>{I->beta, I->field_eps,
-                       forced ? I->field_map + (long long)clone * I->plane
-                              : nullptr};
-  else
-    return Drive<false>{I->beta};
+// Element type of the field maps the step kernel reads in storage ST.
+template <int ST>
+using map_t = typename std::conditional<ST == kF64, double, float>::type;
+
+template <bool FRC, int ST>
+using DriveOf = Drive<FRC, map_t<ST>>;
+
+template <bool FRC, int ST>
+__device__ inline DriveOf<FRC, ST> drive_of(const InstDev* __restrict__ I,
+                                            int clone, bool forced) {
+  if constexpr (FRC) {
+    const map_t<ST>* base;
+    if constexpr (ST == kF64)
+      base = I->field_map;
+    else
+      base = I->field_map32;
+    return DriveOf<FRC, ST>{
+        I->beta, I->field_eps,
+        forced ? base + (long long)clone * I->plane : nullptr};
+  } else {
+    return DriveOf<FRC, ST>{I->beta};
+  }
 }
 
 // Drive of a pass that runs alone (a sweep of one pass): the body force,
@@ -1278,7 +1295,7 @@ __global__ void __launch_bounds__(1024)
     const HbmIO<ST>& dst = (n & 1) ? ioA : ioB;
     double* ring = ring_lds + (n & 1) * ring_max;
     const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
-    const LonePass<Drive<FRC>> lp{drive_of<FRC>(I, wi.clone, n < M1),
+    const LonePass<DriveOf<FRC, ST>> lp{drive_of<FRC, ST>(I, wi.clone, n < M1),
                                   macro_out(I, wi.clone, n, k.M)};
     site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, ring, lp, k.tau, I, ph);
     __syncthreads();
@@ -1319,7 +1336,7 @@ __device__ inline void interval_fused(const InstDev* __restrict__ insts,
     const HbmIO<ST>& src = (sweep & 1) ? ioB : ioA;
     const HbmIO<ST>& dst = (sweep & 1) ? ioA : ioB;
     double* drag = slots + (sweep & 1) * NP * ring_max;
-    const Drive<FRC> dv = drive_of<FRC>(I, wi.clone, n < M1);
+    const DriveOf<FRC, ST> dv = drive_of<FRC, ST>(I, wi.clone, n < M1);
     if (fused_group_at(n, k.M, NP, M1, E)) {
       fused_sweep<MODE, ST, NT, RCH, NP>(k.g, tid, nthr, src, dst, drag,
                                          ring_max, dv, k.tau, I);
@@ -1330,7 +1347,7 @@ __device__ inline void interval_fused(const InstDev* __restrict__ insts,
       n += NP;
     } else {
       const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
-      const LonePass<Drive<FRC>> lp{dv, macro_out(I, wi.clone, n, k.M)};
+      const LonePass<DriveOf<FRC, ST>> lp{dv, macro_out(I, wi.clone, n, k.M)};
       site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, drag, lp, k.tau, I, ph);
       __syncthreads();
       if (n >= 1 && tid == 0) take_drag<MODE>(k, drag, n, acc);

@@ -288,6 +288,17 @@ def field_from_normals(lam: np.ndarray, Z: np.ndarray, Lx: int,
     return np.fft.fft2(np.sqrt(lam) * Z).real[:Lx, :Ly].copy()
 
 
+def field_of_clone(lam: np.ndarray, seed: int, clone: int, interval: int,
+                   instance: int, Lx: int, Ly: int) -> np.ndarray:
+    """The field the device gives `clone`: clones 2k and 2k + 1 share the
+    transform of clone 2k's normals and take its real and imaginary parts
+    (two independent fields of the reference's law; the reference keeps
+    only the real part of a fresh draw per clone)."""
+    Z = field_normals(seed, clone - clone % 2, interval, instance, lam.shape)
+    F = np.fft.fft2(np.sqrt(lam) * Z)[:Lx, :Ly]
+    return (F.real if clone % 2 == 0 else F.imag).copy()
+
+
 def field_covariance(lam: np.ndarray, Lx: int, Ly: int) -> np.ndarray:
     """Covariance of the field at lag (dx, dy), dx < Lx, dy < Ly, implied by
     the (clipped) eigenvalues: sum_k lam_k cos(2 pi k.d / M)."""

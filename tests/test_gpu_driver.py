@@ -361,8 +361,7 @@ def test_forcing_driver_matches_reference_forcing_loop(tmp_path, O):
     lam = O.field_eigenvalues(g.Lx, g.Ly, 32.0)
     for t in (0, nT - 1):
         for j in range(Nc):
-            want = O.field_from_normals(
-                lam, O.field_normals(0, j, t, 0, lam.shape), g.Lx, g.Ly)
+            want = O.field_of_clone(lam, 0, j, t, 0, g.Lx, g.Ly)
             assert np.abs(fields[t][j] - want).max() < 1e-5, (t, j)
     # the reference loop on the oracle with those fields: rank 0's rand()
     # after srand(1) and 1000 discards, drand48 from its default state

@@ -67,7 +67,7 @@ def test_forced_interval_f64_bitexact(O, lbc, threads, passes, M1):
 
 @pytest.mark.parametrize("sigma,atol", [(1.5, 1e-12), (0.0, 1e-5)])
 def test_device_fields_match_oracle(O, lbc, sigma, atol):
-    L, N, seed = 4, 6, 0x5EED_0001_2345
+    L, N, seed = 4, 7, 0x5EED_0001_2345  # odd: the last transform has one clone
     spec = lbc.ref_box(L, 0.55, 0.05, N, 5, alpha=1.0, zero_mean=False)
     spec.forced_steps = 3
     spec.field_eps = 0.4
@@ -85,8 +85,11 @@ def test_device_fields_match_oracle(O, lbc, sigma, atol):
             maps = s.field_maps(0)
             got = s.store_states(0)
             for j in range(N):
-                Z = O.field_normals(seed, j, t, 0, lam.shape)
-                want = O.field_from_normals(lam, Z, g.Lx, g.Ly)
+                want = O.field_of_clone(lam, seed, j, t, 0, g.Lx, g.Ly)
+                if j % 2 == 0:  # the even clone's is the reference's form
+                    Z = O.field_normals(seed, j, t, 0, lam.shape)
+                    assert np.array_equal(
+                        want, O.field_from_normals(lam, Z, g.Lx, g.Ly))
                 err = np.abs(maps[j] - want).max()
                 assert err < atol, f"interval {t} clone {j}: {err:.3g}"
                 # the interval ran with the maps it reports
@@ -101,10 +104,12 @@ def test_device_fields_match_oracle(O, lbc, sigma, atol):
 
 def test_device_fields_large_lattice_batches_and_statistics(O, lbc):
     """Side-241 lattice (the smallest whose sigma-32 embedding needs no
-    clipping): 300 clones take two FFT batches; clones on both sides of the
-    batch boundary match the oracle, and the sample covariance over all
-    fields matches the target exp(-r^2 / 2048)."""
-    L, N, seed = 60, 300, 77
+    clipping): 600 clones need 300 transforms, two clones each; at most 290
+    fit the work buffer, so they run as two batches of 150 (clones 0-299 and
+    300-599); clones on both sides of the batch boundary match the
+    oracle, and the sample covariance over all fields, real and imaginary
+    parts alike, matches the target exp(-r^2 / 2048)."""
+    L, N, seed = 60, 600, 77
     spec = lbc.ref_box(L, 0.55, 0.05, N, 1, alpha=1.0, zero_mean=False)
     spec.forced_steps = 1
     spec.field_eps = 0.1
@@ -117,18 +122,21 @@ def test_device_fields_large_lattice_batches_and_statistics(O, lbc):
         s.run_interval()
         maps = s.field_maps(0)
     assert np.all(np.isfinite(maps))
-    for j in (0, 1, 289, 290, 299):
-        Z = O.field_normals(seed, j, 0, 0, lam.shape)
-        want = O.field_from_normals(lam, Z, g.Lx, g.Ly)
+    for j in (0, 1, 298, 299, 300, 301, 599):
+        want = O.field_of_clone(lam, seed, j, 0, 0, g.Lx, g.Ly)
         err = np.abs(maps[j] - want).max()
         assert err < 1e-5, f"clone {j}: {err:.3g}"
     assert abs(maps.mean()) < 0.1
-    for dx, dy in [(0, 0), (16, 0), (0, 32), (24, 24), (64, 0)]:
-        a = maps[:, : g.Lx - dx, : g.Ly - dy]
-        b = maps[:, dx:, dy:]
-        est = float((a * b).mean())
-        target = np.exp(-(dx * dx + dy * dy) / 2048.0)
-        assert abs(est - target) < 0.12, (dx, dy, est, target)
+    for part in (maps[0::2], maps[1::2], maps):  # Re, Im, both
+        for dx, dy in [(0, 0), (16, 0), (0, 32), (24, 24), (64, 0)]:
+            a = part[:, : g.Lx - dx, : g.Ly - dy]
+            b = part[:, dx:, dy:]
+            est = float((a * b).mean())
+            target = np.exp(-(dx * dx + dy * dy) / 2048.0)
+            assert abs(est - target) < 0.12, (dx, dy, est, target)
+    # the two parts of one transform are independent: no cross-correlation
+    cross = float((maps[0::2] * maps[1::2]).mean())
+    assert abs(cross) < 0.1, cross
 
 
 @pytest.mark.parametrize("storage", [2, 1])
@@ -162,6 +170,41 @@ def test_forced_multi_pass_sweeps_equal_single_passes(O, lbc, storage, M1):
         rel = np.abs(out[0][1][j] - A).max() / np.abs(A).max()
         print(f"storage {storage} clone {j}: drag rel err {rel:.2e}")
         assert rel < (1e-2 if storage == 2 else 1e-4), (j, rel)
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_shifted_storage_reads_fp32_maps(O, lbc, storage):
+    """Shifted storages read the FP32 rounding of the field maps: injected
+    maps and their FP32 roundings give the same bits, and an interval run
+    with device-drawn fields equals the same interval re-run with those
+    fields injected (the draw's FP32 copy is the rounding of its FP64 maps)."""
+    spec = lbc.ref_box(12, 0.55, 0.05, 5, 9, alpha=2.0, zero_mean=False)
+    spec.forced_steps = 5
+    spec.field_eps = 0.5
+    spec.field_sigma = 4.0
+    g = O.ref_box_geom(12, 0.55, 0.05)
+    st = _states(O, g, 5, 6)
+    maps = np.random.default_rng(9).standard_normal((5, g.Lx, g.Ly))
+    print(f"case: fp32 maps storage={storage}", flush=True)
+
+    def run(m, seed=3):
+        with lbc.Sampler(spec, mode=lbc.MODE_REF, storage=storage, seed=seed,
+                         threads=256, passes=2) as s:
+            s.load_states(0, 0, st)
+            if m is not None:
+                s.set_field_maps(0, m)
+            s.run_interval()
+            return s.store_states(0), s.drag(0), s.field_maps(0)
+
+    a = run(maps)
+    b = run(maps.astype(np.float32).astype(np.float64))
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    c = run(maps + 1e-3)  # the maps are used
+    assert not np.array_equal(a[1], c[1])
+    drawn = run(None)
+    again = run(drawn[2])
+    assert np.array_equal(drawn[0], again[0])
+    assert np.array_equal(drawn[1], again[1])
 
 
 def test_forced_advance_graph_matches_stepwise(lbc):
