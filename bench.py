@@ -221,9 +221,11 @@ def time_storage(storage: int, K: int, W: int, specs: list, mode: int,
     lup1 = sum(sp.n_clones * sp.sites * sp.steps for sp in specs)
     lups = lup1 * K
     bpl = lbc.bytes_per_lup(storage)
-    # plus, in forced steps, the clone's random-field map (8 B per site)
-    alg_bytes = lup1 * bpl + sum(8 * sp.n_clones * sp.sites * sp.forced_steps
-                                 for sp in specs)
+    # plus, in forced steps, the clone's random-field map: 8 B per site in
+    # FP64 storage, 4 B (its FP32 copy) in the shifted storages
+    map_b = 8 if storage == lbc.STORE_F64 else 4
+    alg_bytes = lup1 * bpl + sum(map_b * sp.n_clones * sp.sites *
+                                 sp.forced_steps for sp in specs)
     achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else float("nan")
     return dict(
         storage=lbc.STORAGE_NAMES[storage],
