@@ -80,7 +80,10 @@ typedef struct lbc_instance {
    * circulant embedding of the covariance exp(-r^2/(2 field_sigma^2)) in a
    * (2Lx-1) x (2Ly-1) torus, negative eigenvalues set to zero
    * (src/perturbation_routines/generate_random_field.cpp:9-181;
-   * field_sigma 0 = the reference's 32).  forced_steps = 0: off. */
+   * field_sigma 0 = the reference's 32).  Clones 2k and 2k+1 take the real
+   * and imaginary parts of one transform: two independent fields of the
+   * same law.  The shifted storages collide with the maps rounded to FP32.
+   * forced_steps = 0: off. */
   int forced_steps;
   double field_eps;
   double field_sigma;
@@ -221,7 +224,7 @@ int lbc_get_scgf(lbc_ctx* ctx, int inst, double dT, double* lam, int max_n,
 int lbc_set_field_maps(lbc_ctx* ctx, int inst, const double* maps);
 
 /* The random fields the last interval of instance inst ran with, in the same
- * layout. */
+ * layout, in FP64 (the shifted storages used their FP32 rounding). */
 int lbc_get_field_maps(lbc_ctx* ctx, int inst, double* maps);
 
 /* Macroscopic fields of the last interval of clones [clone0, clone0 + n) of
