@@ -286,7 +286,10 @@ int field_setup(lbc_ctx* c, int i) {
   d.field_eps = s.field_eps;
   d.fmx = fmx;
   d.fmy = fmy;
-  const size_t per = 2 * sizeof(double) * fmn;
+  // The shifted storages collide with FP32 maps, so their transforms run in
+  // single precision (C2C) at half the bytes; FP64 storage runs Z2Z.
+  const bool f32 = c->cfg.storage != LBC_STORE_F64;
+  const size_t per = 2 * (f32 ? sizeof(float) : sizeof(double)) * fmn;
   // One transform serves two clones (the real and imaginary parts).  The
   // plan runs its whole batch every time, so the transforms are spread
   // evenly over the fewest batches that fit the work buffer.
@@ -294,11 +297,14 @@ int field_setup(lbc_ctx* c, int i) {
   const size_t fit = std::max<size_t>(1, kFieldWorkBytes / per);
   const size_t batches = (transforms + fit - 1) / fit;
   const int chunk = (int)((transforms + batches - 1) / batches);
+  // the embedding's spectrum below is one FP64 transform in the same buffer
+  const size_t work_bytes =
+      std::max(per * chunk, 2 * sizeof(double) * fmn);
   c->fchunk[i] = chunk;
   const size_t map_bytes = sizeof(double) * (size_t)s.n_clones * d.plane;
   if (dalloc(c, &d.field_map, map_bytes) ||
       dalloc(c, &d.field_sqrt_ev, sizeof(double) * fmn) ||
-      dalloc(c, &d.field_work, per * chunk))
+      dalloc(c, &d.field_work, work_bytes))
     return LBC_E_NOMEM;
   HIPC(c, hipMemsetAsync(d.field_map, 0, map_bytes, c->stream));
   // the shifted storages collide in FP32 and read an FP32 copy of the maps:
@@ -307,7 +313,7 @@ int field_setup(lbc_ctx* c, int i) {
     if (dalloc(c, &d.field_map32, map_bytes / 2)) return LBC_E_NOMEM;
     HIPC(c, hipMemsetAsync(d.field_map32, 0, map_bytes / 2, c->stream));
   }
-  HIPC(c, hipMemsetAsync(d.field_work, 0, per * chunk, c->stream));
+  HIPC(c, hipMemsetAsync(d.field_work, 0, work_bytes, c->stream));
 
   std::vector<double> G(2 * fmn, 0.0);
   const double N = (double)fmn;
@@ -349,7 +355,7 @@ int field_setup(lbc_ctx* c, int i) {
                          hipMemcpyHostToDevice, c->stream));
   int dims[2] = {fmx, fmy};
   FFTC(c, hipfftPlanMany(&c->fft[i], 2, dims, nullptr, 1, (int)fmn, nullptr, 1,
-                         (int)fmn, HIPFFT_Z2Z, chunk));
+                         (int)fmn, f32 ? HIPFFT_C2C : HIPFFT_Z2Z, chunk));
   FFTC(c, hipfftSetStream(c->fft[i], c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   return LBC_OK;
@@ -367,16 +373,23 @@ int enqueue_fields(lbc_ctx* c) {
       c->maps_set[i] = 0;
       continue;
     }
-    auto* work = reinterpret_cast<hipfftDoubleComplex*>(d.field_work);
+    const bool f32 = c->cfg.storage != LBC_STORE_F64;
     // fchunk transforms per batch, two clones per transform
     const int per_batch = 2 * c->fchunk[i];
     for (int c0 = 0; c0 < d.n_clones; c0 += per_batch) {
       const int n = std::min(per_batch, d.n_clones - c0);
       HIPC(c, lbc::launch_field_draw(c->idd, c->counters, i, c0, n,
                                      d.fmx * d.fmy, seed_lo(c), seed_hi(c),
-                                     c->stream));
-      FFTC(c, hipfftExecZ2Z(c->fft[i], work, work, HIPFFT_FORWARD));
-      HIPC(c, lbc::launch_field_extract(c->idd, i, c0, n, d.plane, c->stream));
+                                     f32, c->stream));
+      if (f32) {
+        auto* w = reinterpret_cast<hipfftComplex*>(d.field_work);
+        FFTC(c, hipfftExecC2C(c->fft[i], w, w, HIPFFT_FORWARD));
+      } else {
+        auto* w = reinterpret_cast<hipfftDoubleComplex*>(d.field_work);
+        FFTC(c, hipfftExecZ2Z(c->fft[i], w, w, HIPFFT_FORWARD));
+      }
+      HIPC(c, lbc::launch_field_extract(c->idd, i, c0, n, d.plane, f32,
+                                        c->stream));
     }
   }
   return LBC_OK;

@@ -13,8 +13,9 @@
 // parts of a circulant embedding draw; Dietrich & Newsam 1997, Wood & Chan
 // 1994), which the reference discards: here clones 2k and 2k + 1 of an
 // instance take Re F and Im F of one transform, so an interval runs half the
-// transforms.  The host runs the DFT (hipFFT, batched over clone pairs)
-// between k_field_draw and k_field_extract.
+// transforms.  The host runs the DFT (hipFFT, batched over clone pairs;
+// Z2Z for FP64 storage, C2C for the shifted storages) between k_field_draw
+// and k_field_extract.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -29,6 +30,9 @@ namespace {
 // Z_2p+1 from its last two; the counter is the even clone's.  The pair index
 // p runs over blockIdx.x * blockDim.x + threadIdx.x, the transform slot
 // (clones clone0 + 2*slot and clone0 + 2*slot + 1) over blockIdx.y.
+// W: the transform's precision (double for FP64 storage, float for the
+// shifted storages, whose collision reads the maps in FP32).
+template <typename W>
 __global__ void __launch_bounds__(256)
     k_field_draw(const InstDev* __restrict__ insts,
                  const Counters* __restrict__ counters, int inst, int clone0,
@@ -44,14 +48,15 @@ __global__ void __launch_bounds__(256)
                       kFieldStream + (uint32_t)inst, seed_lo, seed_hi);
   double z[4];
   box_muller_d(r, z);
-  double* __restrict__ w = I->field_work + 2 * ((long long)slot * fmn + m);
+  W* __restrict__ w =
+      reinterpret_cast<W*>(I->field_work) + 2 * ((long long)slot * fmn + m);
   const double a0 = I->field_sqrt_ev[m];
-  w[0] = a0 * z[0];
-  w[1] = a0 * z[1];
+  w[0] = (W)(a0 * z[0]);
+  w[1] = (W)(a0 * z[1]);
   if (m + 1 < fmn) {
     const double a1 = I->field_sqrt_ev[m + 1];
-    w[2] = a1 * z[2];
-    w[3] = a1 * z[3];
+    w[2] = (W)(a1 * z[2]);
+    w[3] = (W)(a1 * z[3]);
   }
 }
 
@@ -59,6 +64,7 @@ __global__ void __launch_bounds__(256)
 // Im F[y + fmy*x] for the odd one (generate_random_field.cpp:170-178 with the
 // lattice's x as the embedding's row index); clone0 + n ends the chunk.  The
 // FP32 copy of the maps, where the instance has one, is written alongside.
+template <typename W>
 __global__ void __launch_bounds__(256)
     k_field_extract(const InstDev* __restrict__ insts, int inst, int clone0,
                     int n) {
@@ -66,7 +72,8 @@ __global__ void __launch_bounds__(256)
   const int slot = blockIdx.y;
   const int Lx = I->Lx, plane = I->plane, fmy = I->fmy;
   const long long fmn = (long long)I->fmx * fmy;
-  const double* __restrict__ F = I->field_work + 2 * (long long)slot * fmn;
+  const W* __restrict__ F =
+      reinterpret_cast<const W*>(I->field_work) + 2 * (long long)slot * fmn;
   const int c = clone0 + 2 * slot;
   const bool odd = 2 * slot + 1 < n;
   double* __restrict__ re = I->field_map + (long long)c * plane;
@@ -77,7 +84,7 @@ __global__ void __launch_bounds__(256)
        s += gridDim.x * blockDim.x) {
     const int x = s % Lx, y = s / Lx;
     const long long k = 2 * (y + (long long)fmy * x);
-    const double a = F[k], b = F[k + 1];
+    const double a = (double)F[k], b = (double)F[k + 1];
     re[s] = a;
     if (odd) im[s] = b;
     if (re32) {
@@ -107,22 +114,32 @@ hipError_t launch_interval_forced(const Launch& L) {
 
 hipError_t launch_field_draw(const InstDev* insts, const Counters* counters,
                              int inst, int clone0, int n, int fmn,
-                             uint32_t seed_lo, uint32_t seed_hi,
+                             uint32_t seed_lo, uint32_t seed_hi, bool f32,
                              hipStream_t stream) {
   if (n <= 0) return hipSuccess;
   const int pairs = (fmn + 1) / 2;
   const dim3 grid((pairs + 255) / 256, (n + 1) / 2);
-  hipLaunchKernelGGL(k_field_draw, grid, dim3(256), 0, stream, insts, counters,
-                     inst, clone0, fmn, seed_lo, seed_hi);
+  if (f32)
+    hipLaunchKernelGGL(k_field_draw<float>, grid, dim3(256), 0, stream, insts,
+                       counters, inst, clone0, fmn, seed_lo, seed_hi);
+  else
+    hipLaunchKernelGGL(k_field_draw<double>, grid, dim3(256), 0, stream, insts,
+                       counters, inst, clone0, fmn, seed_lo, seed_hi);
   return hipGetLastError();
 }
 
 hipError_t launch_field_extract(const InstDev* insts, int inst, int clone0,
-                                int n, int plane, hipStream_t stream) {
+                                int n, int plane, bool f32,
+                                hipStream_t stream) {
   if (n <= 0) return hipSuccess;
   const int bx = (plane + 255) / 256 < 64 ? (plane + 255) / 256 : 64;
-  hipLaunchKernelGGL(k_field_extract, dim3(bx, (n + 1) / 2), dim3(256), 0,
-                     stream, insts, inst, clone0, n);
+  const dim3 grid(bx, (n + 1) / 2);
+  if (f32)
+    hipLaunchKernelGGL(k_field_extract<float>, grid, dim3(256), 0, stream,
+                       insts, inst, clone0, n);
+  else
+    hipLaunchKernelGGL(k_field_extract<double>, grid, dim3(256), 0, stream,
+                       insts, inst, clone0, n);
   return hipGetLastError();
 }
 

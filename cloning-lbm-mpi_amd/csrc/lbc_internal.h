@@ -49,7 +49,8 @@ struct InstDev {
   double* field_map;       // [n_clones][plane] fields, site order y*Lx + x
   float* field_map32;      // the same rounded to FP32 (shifted storage only)
   double* field_sqrt_ev;   // [fmx*fmy] square roots of the eigenvalues
-  double* field_work;      // [chunk][fmx*fmy] complex (re, im), FFT in place
+  double* field_work;      // [chunk][fmx*fmy] complex (re, im), FFT in place;
+                           // FP32 pairs for the shifted storages
   // macroscopic output: rho, ux, uy at steps 0, E, 2E, ... (E = 0: off)
   int macro_every, macro_frames;
   double* macro;           // [n_clones][macro_frames][3][plane]
@@ -158,14 +159,17 @@ hipError_t launch_scgf(const InstDev* insts, int inst, int n, double dT,
 // (pair, even clone, interval, kFieldStream + inst)) times the square-rooted
 // eigenvalues; after the caller's in-place 2-D forward FFT, extract writes
 // the real part of the lattice block to the even clone's map and the
-// imaginary part to the odd one's.  fmn = fmx*fmy.
+// imaginary part to the odd one's.  fmn = fmx*fmy.  f32: the work slots
+// hold single-precision complex values (a C2C transform; the shifted
+// storages), else double (Z2Z; FP64 storage).
 constexpr uint32_t kFieldStream = 0xF1E1D000u;
 hipError_t launch_field_draw(const InstDev* insts, const Counters* counters,
                              int inst, int clone0, int n, int fmn,
-                             uint32_t seed_lo, uint32_t seed_hi,
+                             uint32_t seed_lo, uint32_t seed_hi, bool f32,
                              hipStream_t stream);
 hipError_t launch_field_extract(const InstDev* insts, int inst, int clone0,
-                                int n, int plane, hipStream_t stream);
+                                int n, int plane, bool f32,
+                                hipStream_t stream);
 // field_map32 from field_map for all clones of an instance (maps the caller
 // set); nothing when the instance has no FP32 copy (has_f32 false).
 hipError_t launch_field_narrow(const InstDev* insts, int inst, int n_clones,

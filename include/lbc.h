@@ -82,7 +82,8 @@ typedef struct lbc_instance {
    * (src/perturbation_routines/generate_random_field.cpp:9-181;
    * field_sigma 0 = the reference's 32).  Clones 2k and 2k+1 take the real
    * and imaginary parts of one transform: two independent fields of the
-   * same law.  The shifted storages collide with the maps rounded to FP32.
+   * same law.  The shifted storages transform in single precision and
+   * collide with FP32 maps.
    * forced_steps = 0: off. */
   int forced_steps;
   double field_eps;
@@ -224,7 +225,7 @@ int lbc_get_scgf(lbc_ctx* ctx, int inst, double dT, double* lam, int max_n,
 int lbc_set_field_maps(lbc_ctx* ctx, int inst, const double* maps);
 
 /* The random fields the last interval of instance inst ran with, in the same
- * layout, in FP64 (the shifted storages used their FP32 rounding). */
+ * layout, in FP64 (FP32 values widened for the shifted storages). */
 int lbc_get_field_maps(lbc_ctx* ctx, int inst, double* maps);
 
 /* Macroscopic fields of the last interval of clones [clone0, clone0 + n) of
