@@ -306,7 +306,18 @@ __global__ void __launch_bounds__(256)
     const v4u* __restrict__ vs = (const v4u*)src;
     v4u* __restrict__ vd = (v4u*)dst;
     const long long nv = I->clone_stride * (long long)sizeof(S) / 16;
-    for (long long i = threadIdx.x; i < nv; i += blockDim.x) vd[i] = vs[i];
+    const int bd = blockDim.x;
+    long long i = threadIdx.x;
+    // four loads in flight per thread before their stores
+    for (; i + 3LL * bd < nv; i += 4LL * bd) {
+      const v4u a = vs[i], b = vs[i + bd], c = vs[i + 2 * bd],
+                d = vs[i + 3 * bd];
+      vd[i] = a;
+      vd[i + bd] = b;
+      vd[i + 2 * bd] = c;
+      vd[i + 3 * bd] = d;
+    }
+    for (; i < nv; i += bd) vd[i] = vs[i];
     return;
   }
   for (int s = threadIdx.x; s < plane; s += blockDim.x) {
