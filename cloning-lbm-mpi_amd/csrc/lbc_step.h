@@ -597,6 +597,7 @@ template <bool FRC, class MT = double>
 struct Drive {
   double beta;
   static constexpr bool kFrame = false;
+  static constexpr bool kMap = false;  // the force is one number
   __device__ double at(int) const { return beta; }
 };
 
@@ -605,6 +606,7 @@ struct Drive<true, MT> {
   double beta, eps;
   const MT* __restrict__ map;  // null outside the forced passes
   static constexpr bool kFrame = false;
+  static constexpr bool kMap = true;
   __device__ double at(int site) const {
 #pragma clang fp contract(off)
     return map ? (1.0 + eps * (double)map[site]) * beta : beta;
@@ -985,10 +987,9 @@ __device__ inline void pull_site(const Geo& g, const WaveGeo& w, int lane,
 template <int MODE, int ST, bool NT, class SRC, class DST, class DV>
 __device__ inline void collide_store(typename Traits<ST>::C (&f)[9], bool solid,
                                      const DST& dst, const PassConst<ST>& pc,
-                                     const DV& dv, double tau, int site,
-                                     int plane, int PH) {
+                                     const DV& dv, double beta, double tau,
+                                     int site, int plane, int PH) {
   using T = Traits<ST>;
-  const double beta = dv.at(site);
   if constexpr (DV::kFrame)
     if (dv.out) store_macro<MODE, ST>(f, solid, beta, dv.out, site, plane);
   typename T::S o[9];
@@ -1008,8 +1009,8 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
                                    const SRC& src, const DST& dst,
                                    const PassConst<ST>& pc,
                                    double* __restrict__ ring, const DV& dv,
-                                   double tau, const InstDev* __restrict__ I,
-                                   int PH) {
+                                   double beta, double tau,
+                                   const InstDev* __restrict__ I, int PH) {
   using T = Traits<ST>;
   using C = typename T::C;
   bool solid = false;
@@ -1032,11 +1033,13 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
     } else if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1)) {
       channel_x_boundary<ST>(f, x == 0, I->u_in, I->inlet_k);
     }
-    collide_store<MODE, ST, NT, SRC>(f, solid, dst, pc, dv, tau, site, g.plane,
+    collide_store<MODE, ST, NT, SRC>(f, solid, dst, pc, dv, beta, tau, site,
+                                     g.plane,
                                      PH);
     return;
   }
-  collide_store<MODE, ST, NT, SRC>(f, false, dst, pc, dv, tau, site, g.plane,
+  collide_store<MODE, ST, NT, SRC>(f, false, dst, pc, dv, beta, tau, site,
+                                   g.plane,
                                    PH);
 }
 
@@ -1048,15 +1051,15 @@ __device__ inline void site_step(const Geo& g, const WaveGeo& w, int lane,
                                  int site, const SRC& src, const DST& dst,
                                  const PassConst<ST>& pc,
                                  double* __restrict__ ring, const DV& dv,
-                                 double tau, const InstDev* __restrict__ I,
-                                 int PH) {
+                                 double beta, double tau,
+                                 const InstDev* __restrict__ I, int PH) {
   typename Traits<ST>::C f[9];
   if (PH == kCollide)
     own9(src, f, site, pc);
   else
     pull_site(g, w, lane, site, src, pc, f);
-  site_finish<MODE, ST, NT>(g, w, lane, site, f, src, dst, pc, ring, dv, tau,
-                            I, PH);
+  site_finish<MODE, ST, NT>(g, w, lane, site, f, src, dst, pc, ring, dv, beta,
+                            tau, I, PH);
 }
 
 // One HBM-to-HBM pass of one thread over its sites tid, tid + T, ...
@@ -1076,7 +1079,7 @@ __device__ inline void site_pass(const Geo& g, int tid, int nthr,
     pc.renew();
     if (site < g.plane)
       site_step<MODE, ST, NT>(g, wave_of(I, c * bpc + wblk), lane, site, src,
-                              dst, pc, ring, dv, tau, I, PH);
+                              dst, pc, ring, dv, dv.at(site), tau, I, PH);
     site += nthr;
   }
 }
@@ -1144,8 +1147,32 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
   for (int u = 0; u < PF; u++)
     pull_issue(g, wave_of(I, min(u, nch - 1) * bpc + wblk), lane,
                min(u, nch - 1) * CH + tid, src, pc, pre[u]);
+  // the body force of the site a pass handles in chunk cc.  With field maps
+  // in FP64 or FP32 storage the map load is issued at the top of a step,
+  // ahead of the pulls and LDS barriers its collision waits behind; the
+  // 2-byte storages (bound by VALU issue, where the bounds guard costs more
+  // than the early load saves) load it at the collision, and a uniform
+  // force needs no load at all.
+  constexpr bool kEarly = DV::kMap && (ST == kF64 || ST == kF32);
+  auto beta_at = [&](int cc) {
+    if constexpr (!kEarly) {
+      return 0.0;
+    } else {
+      const int s = cc * CH + tid;
+      return (cc >= 0 && cc < nch && s < g.plane) ? dv.at(s) : 0.0;
+    }
+  };
+  auto beta_of = [&](double early, int s) {
+    if constexpr (kEarly)
+      return early;
+    else
+      return dv.at(s);
+  };
   auto step = [&](int c, R (&pre)[9]) {
     pc.renew();
+    const double b0 = beta_at(c);
+    const double b1 = NP == 3 ? beta_at(c - L) : 0.0;
+    const double b2 = beta_at(c - (NP - 1) * L);
     if (PF > 1 || c < nch) {
       const int s = c * CH + tid;
       typename Traits<ST>::C f[9];
@@ -1161,7 +1188,8 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
                  pre);
       if (c < nch && s < g.plane)
         site_finish<MODE, ST, false>(g, wave_of(I, c * bpc + wblk), lane, s, f,
-                                     src, r0, pc, drag, dv, tau, I, kBoth);
+                                     src, r0, pc, drag, dv, beta_of(b0, s), tau, I,
+                                     kBoth);
     }
     lds_barrier();
     if constexpr (NP == 3) {
@@ -1170,7 +1198,8 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
         const int s = cm * CH + tid;
         if (s < g.plane)
           site_step<MODE, ST, false>(g, wave_of(I, cm * bpc + wblk), lane, s, r0,
-                                     r1, pc, drag + ring_max, dv, tau, I,
+                                     r1, pc, drag + ring_max, dv, beta_of(b1, s),
+                                     tau, I,
                                      kBoth);
       }
       lds_barrier();
@@ -1182,10 +1211,12 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
         double* dz = drag + (NP - 1) * ring_max;
         if constexpr (NP == 3)
           site_step<MODE, ST, NT>(g, wave_of(I, cz * bpc + wblk), lane, s, r1,
-                                  dst, pc, dz, dv, tau, I, kBoth);
+                                  dst, pc, dz, dv, beta_of(b2, s), tau, I,
+                                  kBoth);
         else
           site_step<MODE, ST, NT>(g, wave_of(I, cz * bpc + wblk), lane, s, r0,
-                                  dst, pc, dz, dv, tau, I, kBoth);
+                                  dst, pc, dz, dv, beta_of(b2, s), tau, I,
+                                  kBoth);
       }
     }
   };
