@@ -1150,9 +1150,9 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
   // the body force of the site a pass handles in chunk cc.  With field maps
   // in FP64 or FP32 storage the map load is issued at the top of a step,
   // ahead of the pulls and LDS barriers its collision waits behind; the
-  // 2-byte storages (bound by VALU issue, where the bounds guard costs more
-  // than the early load saves) load it at the collision, and a uniform
-  // force needs no load at all.
+  // 2-byte storages (bound by VALU issue, and measured slower with the
+  // guarded early load) load it at the collision, and a uniform force needs
+  // no load at all.
   constexpr bool kEarly = DV::kMap && (ST == kF64 || ST == kF32);
   auto beta_at = [&](int cc) {
     if constexpr (!kEarly) {
