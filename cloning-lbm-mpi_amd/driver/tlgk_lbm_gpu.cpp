@@ -425,11 +425,14 @@ int main(int argc, char** argv) {
 
   // Draw streams of the reference: every rank calls srand(rank+1) and
   // discards 1000 rand() (:176-178); drand48 is never seeded, so every rank
-  // starts from drand48's default state.  Rank 0's streams drive selection.
+  // starts from glibc's never-seeded state: X = 0 (the library's static
+  // drand48 state is zero-filled and its first use only installs the
+  // multiplier 0x5DEECE66D and addend 0xB), whose first draw is 11 / 2^48.
+  // Rank 0's streams drive selection.
   srand(1);
   for (int i = 0; i < 1000; i++) (void)rand();
   std::vector<std::array<unsigned short, 3>> d48(p);
-  for (auto& s : d48) s = {0x330E, 0xABCD, 0x1234};
+  for (auto& s : d48) s = {0, 0, 0};
   std::deque<int32_t> rand_queue;
 
   const int M = inst.steps;

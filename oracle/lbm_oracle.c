@@ -598,8 +598,13 @@ void orc_glibc_srand(unsigned seed) { srand(seed); }
 void orc_glibc_rand_n(int n, int32_t* out) {
   for (int i = 0; i < n; i++) out[i] = rand();
 }
+/* drand48's state in a process that never seeds it, as the reference's ranks
+ * do (src/mpi_LBM_GK_pops_perturb.cpp:517, generatePerturbedState.cpp:15):
+ * glibc zero-fills the static state and its first use installs only the
+ * multiplier and addend, so X starts at 0 (the first draw is 11 / 2^48), not
+ * at the 0x1234ABCD330E that srand48-style seeding would give. */
 void orc_glibc_seed48_default(void) {
-  unsigned short s[3] = {0x330E, 0xABCD, 0x1234};
+  unsigned short s[3] = {0, 0, 0};
   seed48(s);
 }
 void orc_glibc_drand48_n(int n, double* out) {

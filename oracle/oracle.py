@@ -474,12 +474,12 @@ def ref_perturb_default_stream(fin: np.ndarray, snaps: np.ndarray, eps: float,
     """Run the reference generatePerturbedState from drand48's default state
     (after `skip` draws); returns the NN prefactors it consumed."""
     R = ref()
-    R.ref_seed48(0x330E, 0xABCD, 0x1234)
+    R.ref_seed48(0, 0, 0)
     for _ in range(skip):
         R.ref_drand48()
     NN = snaps.shape[0]
     R.ref_perturb(_dp(fin), _dp(np.ascontiguousarray(snaps)), eps, NN, fin.size)
-    R.ref_seed48(0x330E, 0xABCD, 0x1234)
+    R.ref_seed48(0, 0, 0)
     for _ in range(skip):
         R.ref_drand48()
     return np.array([R.ref_drand48() for _ in range(NN)])

@@ -14,11 +14,12 @@ Fixtures (numpy .npz, FP64 unless noted):
   ref_perturb_L4.npz
       generatePerturbedState (src/perturbation_routines/
       generatePerturbedState.cpp) on a seeded state with 3 snapshots and
-      drand48 in its default state; the prefactors it drew are recorded.
+      drand48 in its never-seeded state (X = 0); the prefactors it drew are
+      recorded.
   glibc_streams.npz
       rand() after srand(1) and 1000 discards (the master rank's stream,
-      src/mpi_LBM_GK_pops_perturb.cpp:176-178), and drand48() from its
-      default state (never seeded by the reference).
+      src/mpi_LBM_GK_pops_perturb.cpp:176-178), and drand48() in a fresh
+      process that never seeds it (the reference never does).
   bodyforce_L4.npz
       every output of the reference's warm-up program (src/lbm_src/
       main_bodyforce.cpp, built by `make -C oracle ref` into
@@ -40,8 +41,22 @@ Fixtures (numpy .npz, FP64 unless noted):
       (streamingAndCollisionComputeMacroBodyForce) and spatial with a map
       (...Spatial), src/lbm_src/streamCollCompute.cpp:17-26,44-45.
 
+  ref_driver_L4.npz
+      the reference's production cloning driver (src/mpi_LBM_GK_pops_perturb
+      .cpp, built by `make -C oracle ref` into oracle/_ref/tlgk_pops_perturb
+      with the macros FORCE_IO, ZERO_MEAN, _SMART_PERTURB) run under
+      `mpiexec -n p` on square-side-4 lattices: five cases (Nc, p, alpha) =
+      (6, 1, 10), (8, 2, 20), (9, 3, 40), (9, 1, 20), (6, 3, 10), each 8
+      cloning intervals of 20 steps from the same ten seeded pops files
+      (`pops`, also the ten perturbation snapshots).  Per case `c<i>_`:
+      params [Nc, p, T, dT, eps, L, tau, U0, alpha], labels.dat, rvalues.dat,
+      every slot's data_force.datout, and from output_TLGK.out the
+      nbCreatedCopies after the prune/add and the copy pairs (flat, with
+      npairs per interval).  The alphas were chosen so that every case has
+      an interval that prunes and one that adds two or more clones.
+
 Run from the repository root: python tests/golden/make_golden.py [name ...]
-(names: steps perturb glibc bodyforce forced macro; default all).
+(names: steps perturb glibc bodyforce forced macro driver; default all).
 """
 import os
 import shutil
@@ -106,9 +121,18 @@ def perturb_fixture():
                         snaps=snaps, eps=0.25, prefactors=a, out=out)
 
 
+NEVER_SEEDED = ("import ctypes; c = ctypes.CDLL('libc.so.6'); "
+                "c.drand48.restype = ctypes.c_double; "
+                "print(' '.join(repr(c.drand48()) for _ in range(64)))")
+
+
 def glibc_fixture():
     r = O.glibc_rand(1, 64, discard=1000)
-    d = O.glibc_drand48(64)
+    # drand48 in a fresh process that never seeds it, as the reference's
+    # ranks never do
+    d = np.array([float(v) for v in subprocess.run(
+        [sys.executable, "-c", NEVER_SEEDED], check=True, capture_output=True,
+        text=True).stdout.split()])
     np.savez_compressed(os.path.join(OUT, "glibc_streams.npz"),
                         rand_seed1_after1000=r, drand48_default=d)
 
@@ -170,6 +194,53 @@ def macro_fixture(L=4, eps=0.7, seed=31):
         beta=g.beta, S0=s0, map=fmap, eps=eps, uniform=uni, spatial=spa)
 
 
+DRIVER_CASES = [(6, 1, 10.0), (8, 2, 20.0), (9, 3, 40.0), (9, 1, 20.0),
+                (6, 3, 10.0)]
+DRIVER_RUN = dict(T=2.0, dT=0.25, eps=0.02, L=4, tau=0.53, U0=0.05)
+
+
+def driver_fixture():
+    import pathlib
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refloop import make_pops, parse_output, write_case
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "tlgk_pops_perturb")
+    mpiexec = os.path.join(os.environ.get("MPI_HOME", "/opt/conda"), "bin",
+                           "mpiexec")
+    if not os.path.exists(exe):
+        sys.exit("oracle/_ref/tlgk_pops_perturb missing: run `make -C oracle "
+                 "ref` where /root/reference and MPI exist")
+    c = DRIVER_RUN
+    pops = make_pops(O, c["L"], 10, 2024, c["tau"], c["U0"])
+    M = int(c["dT"] * c["L"] / c["U0"])
+    nT = int(c["T"] / c["dT"])
+    out = dict(pops=pops)
+    for i, (Nc, p, alpha) in enumerate(DRIVER_CASES):
+        run = pathlib.Path(ROOT, "oracle", "_ref", f"driver_run{i}")
+        shutil.rmtree(run, ignore_errors=True)
+        run.mkdir(parents=True)
+        _, _, inp = write_case(run, pops, Nc, c["T"], c["dT"], c["eps"],
+                               c["L"], c["tau"], c["U0"], alpha)
+        subprocess.run([mpiexec, "-n", str(p), exe, str(inp)], cwd=run,
+                       check=True, stdout=subprocess.DEVNULL, timeout=600)
+        res = run / "run"
+        nb, pairs = parse_output((run / "output_TLGK.out").read_text(), Nc)
+        assert len(nb) == nT and len(pairs) == nT
+        out[f"c{i}_params"] = np.array([Nc, p, c["T"], c["dT"], c["eps"],
+                                        c["L"], c["tau"], c["U0"], alpha])
+        out[f"c{i}_labels"] = np.fromfile(res / "labels.dat",
+                                          np.int32).reshape(nT, Nc)
+        out[f"c{i}_rvalues"] = np.fromfile(res / "rvalues.dat")
+        out[f"c{i}_forces"] = np.stack([
+            np.fromfile(res / f"clone_{j}" / "data_force.datout")
+            for j in range(Nc)]).reshape(Nc, nT, M)
+        out[f"c{i}_nb_after"] = np.array(nb)
+        out[f"c{i}_npairs"] = np.array([len(q) for q in pairs], np.int32)
+        out[f"c{i}_pairs"] = np.concatenate(pairs).astype(np.int32)
+    np.savez_compressed(os.path.join(OUT, "ref_driver_L4.npz"), **out)
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libref.so missing: run `make -C oracle ref` "
@@ -182,6 +253,7 @@ def main():
         "bodyforce": bodyforce_fixture,
         "forced": forced_fixture,
         "macro": macro_fixture,
+        "driver": driver_fixture,
     }
     for name in sys.argv[1:] or list(jobs):
         jobs[name]()

@@ -10,7 +10,9 @@ from collections import deque
 import numpy as np
 import pytest
 
+import refloop
 from conftest import PKG
+from refloop import make_pops, parse_output
 
 pytestmark = pytest.mark.gpu
 
@@ -20,104 +22,28 @@ RECON = os.path.join(PKG, "post", "reconstruct")
 
 def write_case(tmp, O, L, Nc, T, dT, eps, tau, U0, alpha, extra=""):
     g = O.ref_box_geom(L, tau, U0)
-    rng = np.random.default_rng(L * 100 + Nc)
-    data = tmp / "data"
-    data.mkdir()
-    names = []
-    for i in range(max(Nc, 10)):
-        f = np.empty((g.Lx, g.Ly, 9))
-        for x in range(g.Lx):
-            for y in range(g.Ly):
-                f[x, y] = O.feq(1 + 2e-3 * rng.standard_normal(),
-                                U0 * (1 + 0.2 * rng.standard_normal()),
-                                2e-3 * rng.standard_normal())
-        f[g.xmin + 1:g.xmax, g.ymin + 1:g.ymax] = O.W
-        name = f"pops_{i}.datout"
-        f.tofile(data / name)
-        names.append(name)
-    (data / "popfiles_list.dat").write_text("\n".join(names) + "\n")
-    inp = tmp / "input"
-    inp.write_text(f"{Nc}\n{T}\n{dT}\n{eps}\n{L}\n{tau}\n{U0}\n{alpha}\n"
-                   f"{data}/\n{tmp}/run/\n1000\n{extra}")
-    return g, data, names, inp
+    pops = make_pops(O, L, max(Nc, 10), L * 100 + Nc, tau, U0)
+    data, _, inp = refloop.write_case(tmp, pops, Nc, T, dT, eps, L, tau, U0,
+                                      alpha, extra)
+    return g, pops, inp, data
 
 
-def reference_loop(O, g, data, names, Nc, T, dT, eps, U0, alpha, L, p=1):
-    """The reference driver's interval loop with p MPI ranks of Nc/p clones,
-    on the oracle (src/mpi_LBM_GK_pops_perturb.cpp:251-319,387-714): the
-    master hands the first Nc - Nc/p file names to ranks 1.. and keeps the
-    next Nc/p; every rank reads the first 10 names as snapshots and draws
-    its marked clones' prefactors from its own drand48 (never seeded, so
-    each rank's stream starts from the default state); the master draws the
-    floor-step uniforms from its drand48 after its own prefactors, the
-    prune/add ints from its rand() (srand(1), 1000 discarded), and sums
-    total_R in rank blocks."""
-    T0 = L / U0
-    F0 = U0 * U0 * L * 0.5
-    dt = 1.0 / T0
-    M = int(math.floor(dT * T0))
-    nT = int(math.floor(T / dT))
-    local = Nc // p
-    first = [j - local if j >= local else (Nc - local) + j for j in range(Nc)]
-    states = [np.fromfile(data / names[first[j]]).reshape(g.Lx, g.Ly, 9)
-              for j in range(Nc)]
-    snaps = np.stack([np.fromfile(data / names[n]).reshape(g.Lx, g.Ly, 9)
-                      for n in range(10)])
-    lib = O.lib()
-    lib.orc_glibc_srand(1)
-    tmpi = np.zeros(1000, np.int32)
-    lib.orc_glibc_rand_n(1000, O._ip(tmpi))
-    lib.orc_glibc_seed48_default()
-    default48 = np.zeros(100000)  # drand48's sequence from its default state
-    lib.orc_glibc_drand48_n(default48.size, O._dp(default48))
-    pos = [0] * p
-
-    def drand(n, rank=0):
-        out = default48[pos[rank]:pos[rank] + n]
-        pos[rank] += n
-        return out
-
-    rq = deque()
-    marks = np.zeros(Nc, bool)
-    forces = [[] for _ in range(Nc)]
-    labels_all, R = [], []
-    for t in range(nT):
-        for j in range(Nc):
-            if marks[j]:
-                O.perturb(states[j], snaps, eps, drand(10, j // local))
-        acc = np.zeros(Nc)
-        for j in range(Nc):
-            acc[j], A = O.run_steps(g, states[j], M, 1.0 / F0, 1.775, dt)
-            forces[j].append(A)
-        s = np.array([math.exp(alpha * a) for a in acc])
-        u = drand(Nc)
-        while len(rq) < Nc:
-            tmp1 = np.zeros(1, np.int32)
-            lib.orc_glibc_rand_n(1, O._ip(tmp1))
-            rq.append(int(tmp1[0]))
-        sel = O.select(s, u, np.array(list(rq)[:Nc], np.int32), p_blocks=p)
-        for _ in range(sel["n_rint"]):
-            rq.popleft()
-        labels_all.append(sel["labels"])
-        R.append(sel["total_R"])
-        marks[:] = False
-        for src, dst in sel["pairs"]:
-            states[dst] = states[src].copy()
-            marks[dst] = True
-    return labels_all, R, forces, states
+def reference_loop(O, g, pops, Nc, T, dT, eps, U0, alpha, L, p=1):
+    r = refloop.reference_loop(O, g, pops, Nc, T, dT, eps, U0, alpha, L, p)
+    return r["labels"], r["R"], r["forces"], r["states"]
 
 
 @pytest.mark.parametrize("Nc,alpha", [(6, 40.0), (8, 0.0)])
 def test_driver_matches_reference_loop(tmp_path, O, Nc, alpha):
     L, T, dT, eps, tau, U0 = 4, 1.0, 0.25, 0.02, 0.53, 0.05
-    g, data, names, inp = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
-                                     alpha, extra="dump_states = 1\n")
+    g, pops, inp, _ = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
+                                 alpha, extra="dump_states = 1\n")
     print(f"case: driver REF L={L} Nc={Nc} alpha={alpha}", flush=True)
     r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr + r.stdout
     labels_ref, R_ref, forces_ref, states_ref = reference_loop(
-        O, g, data, names, Nc, T, dT, eps, U0, alpha, L)
+        O, g, pops, Nc, T, dT, eps, U0, alpha, L)
     run = tmp_path / "run"
     labels = np.fromfile(run / "labels.dat", dtype=np.int32).reshape(-1, Nc)
     assert np.array_equal(labels, np.array(labels_ref))
@@ -160,15 +86,14 @@ def test_driver_reproduces_a_multi_rank_run(tmp_path, O, Nc, p):
     streams, the master's draws and the rank-blocked total_R of a p-rank
     run of the reference driver."""
     L, T, dT, eps, tau, U0, alpha = 4, 1.0, 0.25, 0.02, 0.53, 0.05, 40.0
-    g, data, names, inp = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
-                                     alpha, extra=f"ranks = {p}\n"
-                                                  "dump_states = 1\n")
+    g, pops, inp, _ = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
+                                 alpha, extra=f"ranks = {p}\ndump_states = 1\n")
     print(f"case: driver REF L={L} Nc={Nc} ranks={p}", flush=True)
     r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr + r.stdout
     labels_ref, R_ref, forces_ref, states_ref = reference_loop(
-        O, g, data, names, Nc, T, dT, eps, U0, alpha, L, p=p)
+        O, g, pops, Nc, T, dT, eps, U0, alpha, L, p=p)
     run = tmp_path / "run"
     labels = np.fromfile(run / "labels.dat", dtype=np.int32).reshape(-1, Nc)
     assert np.array_equal(labels, np.array(labels_ref))
@@ -184,6 +109,47 @@ def test_driver_reproduces_a_multi_rank_run(tmp_path, O, Nc, p):
         fin = np.fromfile(run / f"clone_{j}" / "pops_final.datout").reshape(
             g.Lx, g.Ly, 9)
         np.testing.assert_allclose(fin, states_ref[j], rtol=1e-11, atol=0)
+
+
+REF_RUN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                       "ref_driver_L4.npz")
+
+
+@pytest.mark.parametrize("i", range(5))
+def test_driver_reproduces_the_reference_driver_run(tmp_path, i):
+    """The reference's own TLGK_LBM_POPS_PERTURB, run under mpiexec with
+    p = 1, 2, 3 ranks (fixture ref_driver_L4.npz): the driver with
+    `ranks = p` on the same input and pops files writes the same labels.dat
+    and the same copy tables, rvalues.dat within 1e-12, and every slot's
+    first-interval forces bit for bit (later intervals start from perturbed
+    copies, whose norms the device sums in another order: 1e-10)."""
+    z = np.load(REF_RUN)
+    Nc, p, T, dT, eps, L, tau, U0, alpha = z[f"c{i}_params"]
+    Nc, p, L = int(Nc), int(p), int(L)
+    _, _, inp = refloop.write_case(tmp_path, z["pops"], Nc, T, dT, eps, L,
+                                   tau, U0, alpha, extra=f"ranks = {p}\n")
+    print(f"case: reference run {i}: Nc={Nc} ranks={p} alpha={alpha}",
+          flush=True)
+    r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    run = tmp_path / "run"
+    labels = np.fromfile(run / "labels.dat", dtype=np.int32).reshape(-1, Nc)
+    assert np.array_equal(labels, z[f"c{i}_labels"])
+    np.testing.assert_allclose(np.fromfile(run / "rvalues.dat"),
+                               z[f"c{i}_rvalues"], rtol=1e-12, atol=0)
+    nb, pairs = parse_output((tmp_path / "output_TLGK.out").read_text(), Nc)
+    assert np.array_equal(np.array(nb), z[f"c{i}_nb_after"])
+    want = np.split(z[f"c{i}_pairs"], np.cumsum(z[f"c{i}_npairs"])[:-1])
+    assert len(pairs) == len(want)
+    for got, ref in zip(pairs, want):
+        assert np.array_equal(got, ref)
+    F = z[f"c{i}_forces"]
+    for j in range(Nc):
+        Fj = np.fromfile(run / f"clone_{j}" / "data_force.datout").reshape(
+            F.shape[1], F.shape[2])
+        assert np.array_equal(Fj[0], F[j][0]), f"slot {j}"
+        np.testing.assert_allclose(Fj, F[j], rtol=1e-10, atol=1e-12)
 
 
 def _channel_input(tmp, Nc, extra):
@@ -340,8 +306,8 @@ def test_forcing_driver_matches_reference_forcing_loop(tmp_path, O):
     generator on the same Philox normals."""
     L, Nc, T, dT, dT0, eps, tau, U0, alpha = 4, 6, 1.0, 0.25, 0.1, 0.5, 0.53, \
         0.05, 20.0
-    g, data, names, _ = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
-                                   alpha)
+    g, pops, _, data = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
+                                  alpha)
     inp = tmp_path / "input_forcing"
     inp.write_text(f"{Nc}\n{T}\n{dT}\n{dT0}\n{eps}\n{L}\n{tau}\n{U0}\n{alpha}\n"
                    f"{data}/\n{tmp_path}/frun/\n"
@@ -364,9 +330,8 @@ def test_forcing_driver_matches_reference_forcing_loop(tmp_path, O):
             want = O.field_of_clone(lam, 0, j, t, 0, g.Lx, g.Ly)
             assert np.abs(fields[t][j] - want).max() < 1e-5, (t, j)
     # the reference loop on the oracle with those fields: rank 0's rand()
-    # after srand(1) and 1000 discards, drand48 from its default state
-    states = [np.fromfile(data / names[j]).reshape(g.Lx, g.Ly, 9)
-              for j in range(Nc)]
+    # after srand(1) and 1000 discards, drand48 from its never-seeded state
+    states = [pops[j].copy() for j in range(Nc)]
     lib = O.lib()
     lib.orc_glibc_srand(1)
     tmpi = np.zeros(1000, np.int32)
@@ -432,8 +397,8 @@ def test_driver_vtk_frames_match_reference_loop(tmp_path, O):
     404-455).  The first interval's frames equal the reference's text for the
     oracle's states byte for byte."""
     L, Nc, T, dT, eps, tau, U0, alpha = 4, 3, 0.5, 0.25, 0.02, 0.53, 0.05, 5.0
-    g, data, names, _ = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
-                                   alpha)
+    g, pops, _, data = write_case(tmp_path, O, L, Nc, T, dT, eps, tau, U0,
+                                  alpha)
     inp = tmp_path / "input_vtk"
     inp.write_text(f"{Nc}\n{T}\n{dT}\n{eps}\n{L}\n{tau}\n{U0}\n{alpha}\n"
                    f"{data}/\n{tmp_path}/vrun/\n5\n")
@@ -443,7 +408,7 @@ def test_driver_vtk_frames_match_reference_loop(tmp_path, O):
     assert r.returncode == 0, r.stderr + r.stdout
     run = tmp_path / "vrun"
     for j in range(Nc):
-        a = np.fromfile(data / names[j]).reshape(g.Lx, g.Ly, 9)
+        a = pops[j].copy()
         b = np.zeros_like(a)
         for n in range(20):
             if n % 5 == 0:

@@ -80,7 +80,7 @@ def test_oracle_matches_live_reference(O, L, tau, U0, seed):
 # ---------------------------------------------------------------- RNG pins
 def _drand48_closed_form(n):
     a, c, m = 0x5DEECE66D, 0xB, 1 << 48
-    x = 0x1234ABCD330E
+    x = 0  # glibc's never-seeded state
     out = []
     for _ in range(n):
         x = (a * x + c) % m
@@ -106,6 +106,21 @@ def _glibc_random_r_closed_form(seed, n, discard):
         if k >= 310 + discard:
             out.append(r[i] >> 1)
     return np.array(out, dtype=np.int64)
+
+
+def test_never_seeded_drand48_is_the_zero_state(O):
+    """A fresh process that never seeds drand48, as each reference rank,
+    draws 11/2^48 first: glibc's static state starts at X = 0."""
+    import subprocess
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_golden import NEVER_SEEDED
+    out = subprocess.run([sys.executable, "-c", NEVER_SEEDED], check=True,
+                         capture_output=True, text=True).stdout.split()
+    d = np.array([float(v) for v in out])
+    assert d[0] == 11 / 2 ** 48
+    assert np.array_equal(d, _drand48_closed_form(64))
+    assert np.array_equal(d, O.glibc_drand48(64))
 
 
 def test_glibc_streams(O):
