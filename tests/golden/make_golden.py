@@ -62,6 +62,7 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
@@ -217,9 +218,8 @@ def driver_fixture():
     nT = int(c["T"] / c["dT"])
     out = dict(pops=pops)
     for i, (Nc, p, alpha) in enumerate(DRIVER_CASES):
-        run = pathlib.Path(ROOT, "oracle", "_ref", f"driver_run{i}")
-        shutil.rmtree(run, ignore_errors=True)
-        run.mkdir(parents=True)
+        tmpdir = tempfile.TemporaryDirectory()
+        run = pathlib.Path(tmpdir.name)
         _, _, inp = write_case(run, pops, Nc, c["T"], c["dT"], c["eps"],
                                c["L"], c["tau"], c["U0"], alpha)
         subprocess.run([mpiexec, "-n", str(p), exe, str(inp)], cwd=run,
@@ -238,6 +238,7 @@ def driver_fixture():
         out[f"c{i}_nb_after"] = np.array(nb)
         out[f"c{i}_npairs"] = np.array([len(q) for q in pairs], np.int32)
         out[f"c{i}_pairs"] = np.concatenate(pairs).astype(np.int32)
+        tmpdir.cleanup()
     np.savez_compressed(os.path.join(OUT, "ref_driver_L4.npz"), **out)
 
 

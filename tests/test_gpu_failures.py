@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from conftest import PKG
-from test_gpu_driver import write_case
+from refloop import make_pops, write_case
 
 pytestmark = pytest.mark.gpu
 
@@ -89,8 +89,9 @@ def test_equal_weights_give_identity_and_no_copies(lbc):
 def test_driver_reports_missing_and_short_population_files(tmp_path, O):
     print("case: driver with missing index file, then a short pops file",
           flush=True)
-    g, data, names, inp = write_case(tmp_path, O, 4, 4, 0.6, 0.3, 0.1, 0.53,
-                                     0.05, 1.0)
+    pops = make_pops(O, 4, 10, 404)
+    data, names, inp = write_case(tmp_path, pops, 4, 0.6, 0.3, 0.1, 4, 0.53,
+                                  0.05, 1.0)
     os.rename(data / "popfiles_list.dat", data / "moved.dat")
     r = subprocess.run([DRIVER, str(inp)], cwd=tmp_path, capture_output=True,
                        text=True, timeout=120)
