@@ -174,6 +174,26 @@ def prepare(s: "lbc.Sampler", specs: list, amp: float = 1e-3) -> None:
             s.set_snapshots(i, s.store_states(i, 0, sp.n_snapshots))
 
 
+def hbm_sweeps(M: int, P: int, M1: int = 0, E: int = 0) -> tuple[int, int]:
+    """HBM sweeps of one M-step interval run P passes per sweep, and how many
+    of them collide forced steps (M1 forced steps; output every E steps).
+    Pass 0 collides only, pass M streams only, pass n collides step n; a
+    sweep groups P passes 1..M-1 unless the group would straddle the end of
+    the forced steps or an output step (the kernel's own grouping rule,
+    fused_group_at in csrc/lbc_internal.h)."""
+    def grouped(n):
+        return (n >= 1 and n + P - 1 <= M - 1 and
+                (n + P - 1 < M1 or n >= M1) and
+                (E <= 0 or (n + E - 1) // E * E > n + P - 1))
+    n = sweeps = forced = 0
+    while n <= M:
+        sweeps += 1
+        if n < M1 and n < M:
+            forced += 1
+        n += P if grouped(n) else 1
+    return sweeps, forced
+
+
 def time_storage(storage: int, K: int, W: int, specs: list, mode: int,
                  seed: int = 0, shape=None, G: int = 3) -> dict:
     """W untimed warm-up intervals then K timed ones (graph replays), then
@@ -227,6 +247,16 @@ def time_storage(storage: int, K: int, W: int, specs: list, mode: int,
     alg_bytes = lup1 * bpl + sum(map_b * sp.n_clones * sp.sites *
                                  sp.forced_steps for sp in specs)
     achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else float("nan")
+    # the bytes the blocked kernel moves at least: every HBM sweep (P LBM
+    # passes) reads and writes every population once, and a forced sweep
+    # reads the clone's field map once
+    sweep_bytes = 0
+    for sp in specs:
+        n_sw, n_forced = hbm_sweeps(sp.steps, passes, sp.forced_steps,
+                                    sp.macro_every)
+        # bpl = 2*9*b: one read and one write of the nine populations
+        sweep_bytes += sp.n_clones * sp.sites * (bpl * n_sw + map_b * n_forced)
+    sweep_gbs = sweep_bytes / (kms * 1e-3) / 1e9 if kms > 0 else float("nan")
     return dict(
         storage=lbc.STORAGE_NAMES[storage],
         clone_mlups=lups / secs / 1e6,
@@ -238,6 +268,9 @@ def time_storage(storage: int, K: int, W: int, specs: list, mode: int,
         alg_bytes_per_launch=alg_bytes,
         achieved_gbs=achieved,
         roofline_frac=achieved / HBM_PEAK_GBS,
+        sweep_bytes_per_launch=sweep_bytes,
+        sweep_gbs=sweep_gbs,
+        sweep_frac=sweep_gbs / HBM_PEAK_GBS,
         threads=threads,
         sites_per_thread=vec,
         nontemporal=nt,
@@ -365,6 +398,39 @@ def accuracy(storage: int, workload: str, n: int = 256, warm: int = 10,
                                   if psi64 else None)))
 
 
+def host_cpus() -> dict:
+    """What the host offers this process: logical CPUs (nproc), the CPUs in
+    its affinity mask, the cgroup's CPU quota (cpu.max, as CPUs) and the
+    CPU model."""
+    out = {"nproc": os.cpu_count() or 1}
+    try:
+        out["affinity"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        out["affinity"] = out["nproc"]
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        out["cgroup_cpus"] = (None if quota == "max"
+                              else round(int(quota) / int(period), 2))
+    except (OSError, ValueError):
+        out["cgroup_cpus"] = None
+    out["model"] = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    out["model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return out
+
+
+# CPU share of one GPU on the GPU boxes: worker pools there are sized to it,
+# whatever nproc shows
+GPU_BOX_CPU_SHARE = 16
+
+
 def cpu_baseline(workload: str, lups_per_core: float = 2.4e8,
                  steps: int = 200) -> dict:
     """The CPU path timed on the box's host cores, one worker thread per
@@ -395,7 +461,13 @@ def cpu_baseline(workload: str, lups_per_core: float = 2.4e8,
     kind = "reference" if ref_mode and O.ref_available() else "port"
     sample_clones_per_core = max(1, round(lups_per_core /
                                           (g.Lx * g.Ly * steps)))
-    cores = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpus()
+    # one worker per CPU this process may run on (affinity mask, cgroup
+    # quota), within the GPU's CPU share of the box
+    usable = host["affinity"]
+    if host["cgroup_cpus"]:
+        usable = min(usable, max(1, int(host["cgroup_cpus"])))
+    cores = max(1, min(usable, GPU_BOX_CPU_SHARE))
     states = [np.stack([base] * sample_clones_per_core) for _ in range(cores)]
     O.lib()
     if kind == "reference":
@@ -415,19 +487,11 @@ def cpu_baseline(workload: str, lups_per_core: float = 2.4e8,
         t.join()
     secs = time.perf_counter() - t0
     lups = cores * sample_clones_per_core * g.Lx * g.Ly * steps
-    cpu = "unknown"
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
+    cpu = host["model"]
     what = ("the reference's own step kernels in its driver's step loop, "
             "FP64 AoS" if kind == "reference" else "FP64 AoS push oracle")
     out = dict(value=lups / secs / 1e6, unit="clone-MLUPS", cores=cores,
-               kind=kind,
+               kind=kind, host=host, cpu_share=GPU_BOX_CPU_SHARE,
                sample=(f"{cores} threads x {sample_clones_per_core} clones x "
                        f"{steps} steps of {g.Lx}x{g.Ly} ({what}), "
                        f"{secs:.1f} s on {cpu}"))
@@ -464,21 +528,52 @@ def calibrate_port(O, L: int = 50, steps: int = 60) -> dict | None:
                 port_over_reference=round(rates["port"] / rates["reference"], 3))
 
 
-def measured_traffic(run: dict, workload: str) -> int | None:
+KERNEL_SOURCES = ("csrc/lbc_step.h", "csrc/lbc_internal.h",
+                  "csrc/lbc_kernels.hip", "csrc/lbc_forced.hip")
+
+
+def kernel_source_sha16() -> str:
+    """Hash of the device sources the step kernel is compiled from; a
+    traffic summary under profiles/ records the hash of the build it
+    measured, and its figures are used only while the hash matches."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "cloning-lbm-mpi_amd", rel), "rb") as f:
+            h.update(rel.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(run: dict, workload: str) -> tuple[int | None, dict]:
     """HBM bytes per step-kernel launch for this workload, storage and
     number of LBM passes per sweep, from the FETCH_SIZE / WRITE_SIZE
-    summaries under profiles/ (copy-calibrated there), or None when none
-    covers it."""
+    summaries under profiles/ (copy-calibrated there), and where they came
+    from.  A summary counts only if its `source_sha16` line names the
+    current kernel sources; otherwise the bytes are None and the reason is
+    given."""
     key = (f"workload={workload} storage={run['storage']} "
            f"passes={run['passes_per_sweep']}")
     pat = re.compile(r"^traffic_bytes_per_launch (.*): (\d+)\s*$")
+    want = kernel_source_sha16()
+    stale = None
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
+        sha, hit = None, None
         with open(path) as f:
             for line in f:
-                m = pat.match(line.strip())
+                line = line.strip()
+                if line.startswith("source_sha16:"):
+                    sha = line.split(":", 1)[1].strip()
+                m = pat.match(line)
                 if m and m.group(1) == key:
-                    return int(m.group(2))
-    return None
+                    hit = int(m.group(2))
+        rel = os.path.relpath(path, ROOT)
+        if hit is not None and sha == want:
+            return hit, {"file": rel, "source_sha16": sha}
+        if hit is not None:
+            stale = {"file": rel, "source_sha16": sha,
+                     "dropped": f"profiled build {sha} is not this build {want}"}
+    return None, stale or {"dropped": f"no traffic summary for {key}"}
 
 
 def stream_ceilings() -> dict | None:
@@ -519,25 +614,43 @@ def live_stream_ceilings() -> dict | None:
 
 
 def roofline(run: dict, workload: str, live: dict | None = None) -> dict:
-    """HBM roofline of the step kernel.  achieved = algorithmic bytes
-    (2*9*b per lattice update) / average kernel time; with P LBM passes
-    per HBM sweep the kernel moves about 1/P of that, so the measured
-    traffic (copy-calibrated FETCH_SIZE + WRITE_SIZE per launch) and the
-    DRAM throughput it implies are given beside it."""
-    traffic = measured_traffic(run, workload)
+    """HBM roofline of the step kernel.
+
+    achieved: the bytes the kernel's algorithm must move per launch (one
+    read and one write of every population per HBM sweep of P LBM passes,
+    plus the field map once per forced sweep) over its average launch time
+    from HIP events; frac = achieved / 8 TB/s, which cannot exceed 1.
+    traffic: HBM bytes per launch counted by rocprofv3 (copy-calibrated
+    FETCH_SIZE + WRITE_SIZE) for this build, with dram_achieved / dram_frac
+    from them.  alg_achieved / alg_frac: the per-lattice-update model
+    (2*9*b bytes per update); with P passes per sweep the kernel moves
+    about 1/P of those bytes, so alg_frac exceeds 1 and is not a roofline
+    fraction."""
+    traffic, source = measured_traffic(run, workload)
+    frac = run["sweep_frac"]
+    if not frac <= 1.0:
+        raise SystemExit(f"bench.py: roofline fraction {frac:.3f} > 1: the "
+                         "byte model does not describe this kernel")
     out = {
         "bound": "hbm",
-        "achieved": round(run["achieved_gbs"], 1),
+        "achieved": round(run["sweep_gbs"], 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(run["roofline_frac"], 4),
+        "frac": round(frac, 4),
         "traffic": traffic,
+        "traffic_source": source,
+        "bytes_per_launch": run["sweep_bytes_per_launch"],
+        "passes_per_sweep": run["passes_per_sweep"],
         "alg_bytes_per_launch": run["alg_bytes_per_launch"],
+        "alg_achieved": round(run["achieved_gbs"], 1),
+        "alg_frac": round(run["roofline_frac"], 4),
     }
     if traffic:
         dram = traffic / (run["kernel_ms"] * 1e-3) / 1e9
         out["dram_achieved"] = round(dram, 1)
         out["dram_frac"] = round(dram / HBM_PEAK_GBS, 4)
+        out["traffic_over_bytes"] = round(traffic / run["sweep_bytes_per_launch"],
+                                          4)
         ceil = live or stream_ceilings()
         if ceil:
             out["stream_ceilings"] = ceil

@@ -1111,6 +1111,10 @@ int lbc_set_field_maps(lbc_ctx* c, int inst, const double* maps) {
       for (int y = 0; y < d.Ly; y++)
         tmp[j * plane + (size_t)y * d.Lx + x] =
             maps[j * plane + y + (size_t)d.Ly * x];
+  // the shifted storages step with the FP32 copy: keep its widening, so
+  // that lbc_get_field_maps reports the values the interval runs with
+  if (d.field_map32)
+    for (double& v : tmp) v = (double)(float)v;
   HIPC(c, hipMemcpyAsync(d.field_map, tmp.data(), sizeof(double) * tmp.size(),
                          hipMemcpyHostToDevice, c->stream));
   HIPC(c, lbc::launch_field_narrow(c->idd, inst, d.n_clones, d.plane,

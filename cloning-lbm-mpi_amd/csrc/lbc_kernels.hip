@@ -640,13 +640,14 @@ hipError_t launch_select(const InstDev* insts, Counters* counters, int inst0,
                          hipStream_t stream) {
   const int threads = 1024;
   const size_t lds = sizeof(int) * (4 * (size_t)n_max + threads);
-  static size_t lds_set = 0;
-  if (lds > 65536 && lds > lds_set) {
+  // above 64 KiB the kernel needs the attribute on the current device; it
+  // is set on every such launch (a cheap call) rather than cached, so that
+  // contexts on other devices or threads never skip it
+  if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void*)k_select,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;
-    lds_set = lds;
   }
   hipLaunchKernelGGL(k_select, dim3(n_inst), dim3(threads), lds, stream, insts,
                      counters, inst0, weights_from_host, draws_from_host, lo,

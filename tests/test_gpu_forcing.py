@@ -234,6 +234,8 @@ def test_shifted_storage_reads_fp32_maps(O, lbc, storage):
     a = run(maps)
     b = run(maps.astype(np.float32).astype(np.float64))
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # the maps reported are the FP32 values the interval ran with
+    assert np.array_equal(a[2], maps.astype(np.float32).astype(np.float64))
     c = run(maps + 1e-3)  # the maps are used
     assert not np.array_equal(a[1], c[1])
     drawn = run(None)
