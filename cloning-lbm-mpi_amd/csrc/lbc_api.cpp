@@ -23,6 +23,8 @@ using lbc::WorkItem;
 namespace {
 
 thread_local std::string g_err;
+// error text of lbc_select_static, which must not allocate (string literals)
+thread_local const char* g_err_static = nullptr;
 
 constexpr int kMaxClones = 8192;
 constexpr int kMaxIntervals = 1 << 16;
@@ -121,6 +123,12 @@ class DeviceScope {
 int fail(lbc_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   g_err = msg;
+  g_err_static = nullptr;
+  return code;
+}
+
+int fail_static(int code, const char* msg) {
+  g_err_static = msg;
   return code;
 }
 
@@ -536,6 +544,7 @@ extern "C" {
 
 const char* lbc_last_error(const lbc_ctx* ctx) {
   if (ctx && !ctx->err.empty()) return ctx->err.c_str();
+  if (g_err_static) return g_err_static;
   return g_err.c_str();
 }
 
@@ -1199,6 +1208,46 @@ int lbc_gather_kernel_ms(lbc_ctx* c, double* ms, double* bytes) {
   HIPC(c, hipEventElapsedTime(&t, c->gev[0], c->gev[1]));
   *ms = t;
   if (bytes) *bytes = c->gather_bytes;
+  return LBC_OK;
+}
+
+int lbc_select_static(int n, int rank_blocks, const double* logw,
+                      const double* u, const int32_t* rint, int32_t* nb_before,
+                      int32_t* nb_after, int32_t* pairs, int32_t* labels,
+                      uint8_t* marks, int32_t* info, double* R, void* stream) {
+  // host-side checks first: nothing is launched for a rejected call
+  if (n < 1 || n > LBC_STATIC_MAX_CLONES)
+    return fail_static(LBC_E_ARG, "lbc_select_static: n must be in "
+                                  "[1, LBC_STATIC_MAX_CLONES]");
+  if (rank_blocks < 1 || n % rank_blocks != 0)
+    return fail_static(LBC_E_ARG, "lbc_select_static: rank_blocks must be "
+                                  "positive and divide n");
+  if (!logw || !u || !rint || !nb_before || !nb_after || !pairs || !labels ||
+      !marks || !info || !R)
+    return fail_static(LBC_E_ARG, "lbc_select_static: null device array");
+  void* w = nullptr;
+  if (lbc::static_scratch(&w) != hipSuccess)
+    return fail_static(LBC_E_HIP, "lbc_select_static: scratch address lookup "
+                                  "failed");
+  lbc::InstDev I{};
+  I.n_clones = n;
+  I.rank_blocks = rank_blocks;
+  I.logw = const_cast<double*>(logw);
+  I.weights = static_cast<double*>(w);
+  // read only: the kernel writes the draws only when it draws them itself
+  I.u = const_cast<double*>(u);
+  I.rint = const_cast<int32_t*>(rint);
+  I.nb_before = nb_before;
+  I.nb_after = nb_after;
+  I.pairs = pairs;
+  I.labels = labels;
+  I.marks = marks;
+  I.sel_info = info;
+  I.sel_R = R;
+  I.max_intervals = 0;  // no R history
+  if (lbc::launch_select_static(I, static_cast<hipStream_t>(stream)) !=
+      hipSuccess)
+    return fail_static(LBC_E_HIP, "lbc_select_static: launch failed");
   return LBC_OK;
 }
 

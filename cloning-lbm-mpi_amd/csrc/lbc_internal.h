@@ -65,6 +65,10 @@ struct Counters {
 };
 
 constexpr int kMaxInstances = 64;
+// Selection without allocation (lbc_select_static): largest population, and
+// devices whose scratch address is kept.
+constexpr int kStaticMaxClones = 8192;
+constexpr int kMaxDevices = 64;
 
 // Work item of the step kernel: one clone of one instance.
 struct WorkItem {
@@ -150,6 +154,13 @@ constexpr int fused_sweeps(int M, int P, int M1 = 0, int E = 0) {
 }
 size_t fused_lds_bytes(int storage, int ring_max, int rch, int passes);
 hipError_t launch_flip(const Launch& L, int n_inst, const int* flip_mask_dev);
+// The device address of the selection scratch (kStaticMaxClones doubles) on
+// the current device, looked up on the first call per device.
+hipError_t static_scratch(void** out);
+// One selection of I (a descriptor held by the caller, passed to the kernel
+// by value; I.weights must be the current device's scratch) with the
+// caller's draws, on `stream`.
+hipError_t launch_select_static(const InstDev& I, hipStream_t stream);
 hipError_t launch_scgf(const InstDev* insts, int inst, int n, double dT,
                        double* lam, hipStream_t stream);
 // Random fields of the forcing mode (lbc_forced.hip), for clones

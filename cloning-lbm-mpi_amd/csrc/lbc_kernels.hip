@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "lbc_internal.h"
 #include "lbc_step.h"
 
@@ -83,16 +85,18 @@ __device__ int block_scan_excl(int* v, int n, int* part) {
   return total;
 }
 
-// Selection of the cloning step (src/mpi_LBM_GK_pops_perturb.cpp:489-632).
-// LDS: nb[N], temp[2N], work[N], part[blockDim].
-__global__ void __launch_bounds__(1024)
-    k_select(const InstDev* __restrict__ insts, Counters* counters, int inst0,
-             int weights_from_host, int draws_from_host, uint32_t seed_lo,
-             uint32_t seed_hi) {
+// Selection of the cloning step (src/mpi_LBM_GK_pops_perturb.cpp:489-632)
+// for instance `inst` described by I, in one workgroup.  `interval` keys the
+// device draws and the R history slot; interval_out (if not null) receives
+// interval + 1.  LDS: nb[N], temp[2N], work[N], part[blockDim].
+__device__ __forceinline__ void select_body(const InstDev& I, int inst,
+                                            int interval, int* interval_out,
+                                            int weights_from_host,
+                                            int draws_from_host,
+                                            uint32_t seed_lo,
+                                            uint32_t seed_hi) {
   extern __shared__ int sm[];
-  const int inst = inst0 + blockIdx.x;
-  const InstDev* __restrict__ I = insts + inst;
-  const int N = I->n_clones;
+  const int N = I.n_clones;
   const int t = threadIdx.x, nt = blockDim.x;
   int* nb = sm;
   int* temp = nb + N;
@@ -100,7 +104,6 @@ __global__ void __launch_bounds__(1024)
   int* part = work + N;
   __shared__ double sh_total, sh_shift;
   __shared__ int sh_bad, sh_used;
-  const int interval = counters->interval[inst];
 
   // weights and draws
   if (t == 0) sh_bad = 0;
@@ -108,7 +111,7 @@ __global__ void __launch_bounds__(1024)
   double lmax = -1e308, lmin = 1e308;
   if (!weights_from_host) {
     for (int j = t; j < N; j += nt) {
-      const double l = I->logw[j];
+      const double l = I.logw[j];
       lmax = fmax(lmax, l);
       lmin = fmin(lmin, l);
     }
@@ -142,7 +145,7 @@ __global__ void __launch_bounds__(1024)
     }
     __syncthreads();
     const double sh = sh_shift;
-    for (int j = t; j < N; j += nt) I->weights[j] = exp(I->logw[j] - sh);
+    for (int j = t; j < N; j += nt) I.weights[j] = exp(I.logw[j] - sh);
   } else if (t == 0) {
     sh_shift = 0.0;
   }
@@ -150,8 +153,8 @@ __global__ void __launch_bounds__(1024)
     for (int j = t; j < N; j += nt) {
       const U4 r = philox((uint32_t)j, (uint32_t)interval, (uint32_t)inst,
                           0x5E1EC7u, seed_lo, seed_hi);
-      I->u[j] = u53(r.x, r.y);
-      I->rint[j] = (int32_t)(r.z >> 1);
+      I.u[j] = u53(r.x, r.y);
+      I.rint[j] = (int32_t)(r.z >> 1);
     }
   }
   __syncthreads();
@@ -159,12 +162,12 @@ __global__ void __launch_bounds__(1024)
   // total_R: sequential, in rank blocks (src/mpi_LBM_GK_pops_perturb.cpp:489-497,512)
   if (t == 0) {
 #pragma clang fp contract(off)
-    const int p = I->rank_blocks > 0 ? I->rank_blocks : 1;
+    const int p = I.rank_blocks > 0 ? I.rank_blocks : 1;
     const int local = N / p;
     double total = 0.0;
     for (int r = 0; r < p; r++) {
       double R = 0.0;
-      for (int j = r * local; j < (r + 1) * local; j++) R += I->weights[j];
+      for (int j = r * local; j < (r + 1) * local; j++) R += I.weights[j];
       if (r == 0) total = R;
       else total += R;
     }
@@ -177,12 +180,12 @@ __global__ void __launch_bounds__(1024)
 
   // floor step (:514-520)
   for (int j = t; j < N; j += nt) {
-    const double v = floor(I->weights[j] / total + I->u[j]);
+    const double v = floor(I.weights[j] / total + I.u[j]);
     int c = 0;
     if (v >= 0.0 && v <= 2.0 * N) c = (int)v;
     else sh_bad = 1;
     nb[j] = c;
-    I->nb_before[j] = c;
+    I.nb_before[j] = c;
     work[j] = c;
   }
   __syncthreads();
@@ -191,16 +194,16 @@ __global__ void __launch_bounds__(1024)
   if (sh_bad || ncp < 1 || ncp > 2 * N) {
     // non-finite or inconsistent weights: keep every clone, report the error
     for (int j = t; j < N; j += nt) {
-      I->nb_after[j] = 1;
-      I->labels[j] = j;
-      I->marks[j] = 0;
+      I.nb_after[j] = 1;
+      I.labels[j] = j;
+      I.marks[j] = 0;
     }
     if (t == 0) {
-      I->sel_info[0] = 0;
-      I->sel_info[1] = 0;
-      I->sel_info[2] = -5;
-      I->sel_R[0] = total;
-      I->sel_R[1] = log(total) + sh_shift;
+      I.sel_info[0] = 0;
+      I.sel_info[1] = 0;
+      I.sel_info[2] = -5;
+      I.sel_R[0] = total;
+      I.sel_R[1] = log(total) + sh_shift;
     }
     return;
   }
@@ -215,14 +218,14 @@ __global__ void __launch_bounds__(1024)
     int used = 0;
     if (dN > 0) {
       for (int i = 0; i < dN; i++) {
-        const int idx = I->rint[used++] % (ncp - i);
+        const int idx = I.rint[used++] % (ncp - i);
         const int c = temp[idx];
         temp[idx] = temp[ncp - i - 1];
         nb[c]--;
       }
     } else if (dN < 0) {
       for (int i = 0; i < -dN; i++) {
-        const int idx = I->rint[used++] % ncp;
+        const int idx = I.rint[used++] % ncp;
         const int c = temp[idx];
         temp[ncp + i] = c;
         nb[c]++;
@@ -234,7 +237,7 @@ __global__ void __launch_bounds__(1024)
   // copy table (:590-619): the j-th extra copy (parents ascending) goes to
   // the j-th empty slot (ascending)
   for (int j = t; j < N; j += nt) {
-    I->nb_after[j] = nb[j];
+    I.nb_after[j] = nb[j];
     work[j] = nb[j] > 1 ? nb[j] - 1 : 0;
     temp[j] = nb[j] == 0 ? 1 : 0;
   }
@@ -245,33 +248,49 @@ __global__ void __launch_bounds__(1024)
   for (int j = t; j < N; j += nt)
     if (nb[j] == 0) temp[N + temp[j]] = j;
   __syncthreads();
-  for (int j = t; j < N; j += nt) I->labels[j] = j;
+  for (int j = t; j < N; j += nt) I.labels[j] = j;
   __syncthreads();
   for (int j = t; j < N; j += nt) {
     const int e = nb[j] > 1 ? nb[j] - 1 : 0;
     for (int i = 0; i < e; i++) {
       const int pidx = work[j] + i;
       const int dst = temp[N + pidx];
-      I->pairs[2 * pidx] = j;
-      I->pairs[2 * pidx + 1] = dst;
-      I->labels[dst] = j;
+      I.pairs[2 * pidx] = j;
+      I.pairs[2 * pidx + 1] = dst;
+      I.labels[dst] = j;
     }
   }
   __syncthreads();
-  for (int j = t; j < N; j += nt) I->marks[j] = (I->labels[j] != j) ? 1 : 0;
+  for (int j = t; j < N; j += nt) I.marks[j] = (I.labels[j] != j) ? 1 : 0;
   if (t == 0) {
     const double lr = log(total) + sh_shift;
-    I->sel_info[0] = n_pairs;
-    I->sel_info[1] = sh_used;
-    I->sel_info[2] = 0;
-    I->sel_R[0] = sh_shift == 0.0 ? total : exp(lr);
-    I->sel_R[1] = lr;
-    if (interval < I->max_intervals) {
-      I->R_hist[2 * interval] = I->sel_R[0];
-      I->R_hist[2 * interval + 1] = lr;
+    I.sel_info[0] = n_pairs;
+    I.sel_info[1] = sh_used;
+    I.sel_info[2] = 0;
+    I.sel_R[0] = sh_shift == 0.0 ? total : exp(lr);
+    I.sel_R[1] = lr;
+    if (interval < I.max_intervals) {
+      I.R_hist[2 * interval] = I.sel_R[0];
+      I.R_hist[2 * interval + 1] = lr;
     }
-    counters->interval[inst] = interval + 1;
+    if (interval_out) *interval_out = interval + 1;
   }
+}
+
+__global__ void __launch_bounds__(1024)
+    k_select(const InstDev* __restrict__ insts, Counters* counters, int inst0,
+             int weights_from_host, int draws_from_host, uint32_t seed_lo,
+             uint32_t seed_hi) {
+  const int inst = inst0 + blockIdx.x;
+  select_body(insts[inst], inst, counters->interval[inst],
+              &counters->interval[inst], weights_from_host, draws_from_host,
+              seed_lo, seed_hi);
+}
+
+// The same selection for a descriptor passed by value (no device-side
+// descriptor or counters), with the caller's draws: lbc_select_static.
+__global__ void __launch_bounds__(1024) k_select_static(InstDev I) {
+  select_body(I, 0, 0, nullptr, 0, 1, 0u, 0u);
 }
 
 // ---------------------------------------------------------------- gather
@@ -652,6 +671,48 @@ hipError_t launch_select(const InstDev* insts, Counters* counters, int inst0,
   hipLaunchKernelGGL(k_select, dim3(n_inst), dim3(threads), lds, stream, insts,
                      counters, inst0, weights_from_host, draws_from_host, lo,
                      hi);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------ selection without allocation
+// The one scratch buffer of lbc_select_static: the FP64 weights of at most
+// kStaticMaxClones clones.  Every device the library runs on holds its own
+// copy until the process ends; calls on one device share it.
+static __device__ __align__(256) double scratch[kStaticMaxClones];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t static_scratch(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    // the array's device address on the current device (the host-side name
+    // of the array is not a device address)
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+hipError_t launch_select_static(const InstDev& I, hipStream_t stream) {
+  const int threads = 1024;
+  const size_t lds = sizeof(int) * (4 * (size_t)I.n_clones + threads);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_select_static,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_select_static, dim3(1), dim3(threads), lds, stream, I);
   return hipGetLastError();
 }
 

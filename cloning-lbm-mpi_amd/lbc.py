@@ -41,7 +41,10 @@ EXPORTS = (
     "lbc_get_field_maps", "lbc_get_macro", "lbc_sync", "lbc_step_kernel_ms",
     "lbc_gather_kernel_ms", "lbc_threads",
     "lbc_sites_per_thread", "lbc_step_shape", "lbc_step_host",
+    "lbc_select_static",
 )
+
+STATIC_MAX_CLONES = 8192  # LBC_STATIC_MAX_CLONES
 
 
 class Instance(C.Structure):
@@ -127,6 +130,7 @@ def lib() -> C.CDLL:
     L.lbc_sites_per_thread.argtypes = [vp]
     L.lbc_step_shape.argtypes = [vp]
     L.lbc_step_host.argtypes = [C.c_int, C.POINTER(Instance), dp, dp, dp]
+    L.lbc_select_static.argtypes = [C.c_int, C.c_int] + [vp] * 11
     _lib = L
     return L
 
@@ -513,6 +517,19 @@ def step_host(mode: int, spec: InstanceSpec, fin: np.ndarray):
         raise LbcError(f"lbc_step_host failed ({rc}): "
                        f"{lib().lbc_last_error(None).decode()}")
     return a, out, float(F[0])
+
+
+def select_static(n: int, rank_blocks: int, logw: int, u: int, rint: int,
+                  nb_before: int, nb_after: int, pairs: int, labels: int,
+                  marks: int, info: int, R: int, stream: int = 0) -> None:
+    """lbc_select_static on device arrays given by address (the caller owns
+    and sizes them; see include/lbc.h); enqueued on `stream`."""
+    rc = lib().lbc_select_static(n, rank_blocks, logw, u, rint, nb_before,
+                                 nb_after, pairs, labels, marks, info, R,
+                                 stream or None)
+    if rc != 0:
+        raise LbcError(f"lbc_select_static failed ({rc}): "
+                       f"{lib().lbc_last_error(None).decode()}")
 
 
 def bytes_per_lup(storage: int) -> int:

@@ -263,6 +263,41 @@ int lbc_sites_per_thread(const lbc_ctx* ctx);
  * per HBM sweep in bits 5-6 (1, 2 or 3).  0 for a null context. */
 int lbc_step_shape(const lbc_ctx* ctx);
 
+/* Largest population of lbc_select_static. */
+#define LBC_STATIC_MAX_CLONES 8192
+
+/* The selection of lbc_select (weights NULL, caller's draws) for one
+ * population of n clones whose inputs and outputs all live in device memory
+ * of the current device, without a context and without allocating: it calls
+ * no hipMalloc, hipFree or other allocating routine.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight sampling loops that hold their own
+ * buffers).
+ *   logw:   n log-weights; the weights are exp(logw), shifted by the largest
+ *           log-weight only where exp would leave its range (as lbc_select)
+ *   u:      n floor-step uniforms in [0,1);  rint: n non-negative draws for
+ *           the prune/add steps (as lbc_select's u and rint)
+ *   rank_blocks: total_R summed in that many sequential blocks
+ *           (lbc_instance.rank_blocks)
+ *   outputs: nb_before[n], nb_after[n], pairs[2n] (the first info[0] pairs
+ *           are set), labels[n], marks[n] (1 for an overwritten slot),
+ *           info[4] = {n_pairs, n_rint_used, status (0, or LBC_E_NUMERIC
+ *           for non-finite weights: every clone kept), 0 (unused)} and
+ *           R[2] = {total_R, log total_R}.
+ * n is at most LBC_STATIC_MAX_CLONES (larger n: LBC_E_ARG, nothing is
+ * launched).  The kernel keeps the FP64 weights in the library's static
+ * device scratch (LBC_STATIC_MAX_CLONES doubles, 64 KiB, held on every
+ * device the library runs on until the process ends), which every call on
+ * a device shares: calls that overlap on one device race, so order them on
+ * one stream.  The work is enqueued on `stream` (a hipStream_t; NULL for
+ * the null stream) and the call returns without waiting; errors of argument
+ * checks and launch come back as the return code (text from
+ * lbc_last_error(NULL)), numeric ones in info[2] once the stream has run. */
+int lbc_select_static(int n, int rank_blocks, const double* logw,
+                      const double* u, const int32_t* rint, int32_t* nb_before,
+                      int32_t* nb_after, int32_t* pairs, int32_t* labels,
+                      uint8_t* marks, int32_t* info, double* R, void* stream);
+
 /* One reference LBM step on one lattice held in host memory (FP64):
  * fin (post-stream state) becomes the post-collision state, fout receives
  * the next post-stream state, *force the raw drag on it.  Parity hook for

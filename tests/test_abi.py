@@ -93,3 +93,18 @@ def test_host_arrays_are_checked_before_the_library_reads_them(lbc):
         lbc._dp(np.zeros((4, 4))[:, 0])
     with pytest.raises(TypeError):
         lbc._ip(np.zeros(4, np.int64))
+
+
+def test_select_static_rejects_without_launching(lbc):
+    """lbc_select_static checks its arguments on the host before it looks up
+    its scratch or launches anything, so these run without a GPU."""
+    fake = 1 << 20  # never dereferenced: every call below is rejected
+    args = [fake] * 10
+    for n, p, what in ((0, 1, "n must"), (lbc.STATIC_MAX_CLONES + 1, 1,
+                                          "n must"), (10, 3, "rank_blocks")):
+        with pytest.raises(lbc.LbcError) as e:
+            lbc.select_static(n, p, *args)
+        assert "(-1)" in str(e.value) and what in str(e.value)
+    with pytest.raises(lbc.LbcError) as e:
+        lbc.select_static(8, 1, fake, 0, *args[2:])
+    assert "null device array" in str(e.value)
