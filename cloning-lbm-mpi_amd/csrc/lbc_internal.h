@@ -152,6 +152,12 @@ constexpr int fused_sweeps(int M, int P, int M1 = 0, int E = 0) {
   for (int n = 0; n <= M; s++) n += fused_group_at(n, M, P, M1, E) ? P : 1;
   return s;
 }
+// LDS bytes of one ring of the multi-pass step kernel: nine direction
+// planes of rch slots plus two mirror slots each, elements of es bytes,
+// rounded up to 16 bytes.
+__host__ __device__ constexpr size_t ring_bytes(int rch, int es) {
+  return ((size_t)9 * (size_t)(rch + 2) * (size_t)es + 15) / 16 * 16;
+}
 size_t fused_lds_bytes(int storage, int ring_max, int rch, int passes);
 hipError_t launch_flip(const Launch& L, int n_inst, const int* flip_mask_dev);
 // The device address of the selection scratch (kStaticMaxClones doubles) on

@@ -601,7 +601,7 @@ size_t fused_lds_bytes(int storage, int ring_max, int rch, int passes) {
   if (rch > 8192 || passes < 2) return 0;
   const size_t es = storage == kF64 ? 8 : storage == kF32 ? 4 : 2;
   const size_t b = sizeof(double) * 2 * passes * (size_t)ring_max +
-                   (passes - 1) * 9 * (size_t)rch * es;
+                   (passes - 1) * ring_bytes(rch, (int)es);
   return b <= 160 * 1024 ? b : 0;
 }
 

@@ -87,6 +87,81 @@ struct Traits<kBF16> {
   __device__ static C rest(int) { return 0.0f; }
 };
 
+// ---------------------------------------------------------------- f16 in place
+// An FP16 population as loaded: its half in the low 16 bits of a VGPR.  The
+// collision consumes it through gfx950's mixed-precision FMA v_fma_mix_f32,
+// which reads f16 operands exactly, so no separate conversion is issued.
+// Each form rounds once, as the same operation on the converted value does
+// (checked on the device against v_cvt_f32_f16 for every f16 bit pattern,
+// subnormals included); the product takes a -0.0 addend so that a zero
+// product keeps its sign.
+struct H16 {
+  unsigned v;
+};
+
+__device__ inline float to_f(float a) { return a; }
+__device__ inline float to_f(H16 a) {
+  return (float)__builtin_bit_cast(_Float16, (unsigned short)a.v);
+}
+// a + b, a - b, x + a, s * a on populations a, b (float or f16 in place)
+__device__ inline float padd(float a, float b) { return a + b; }
+__device__ inline float psub(float a, float b) { return a - b; }
+__device__ inline float paddf(float a, float x) { return x + a; }
+__device__ inline float pmul(float s, float a) { return s * a; }
+__device__ inline float padd(H16 a, H16 b) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a.v), "v"(b.v));
+  return r;
+}
+__device__ inline float psub(H16 a, H16 b) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a.v), "v"(b.v));
+  return r;
+}
+__device__ inline float paddf(H16 a, float x) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(a.v), "v"(x));
+  return r;
+}
+__device__ inline float pmul(float s, H16 a) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,1,0]"
+      : "=v"(r) : "v"(s), "v"(a.v), "s"(-0.0f));
+  return r;
+}
+
+// Register form of a pulled population: f16 in place for FP16 storage, the
+// compute type of the storage otherwise.
+template <int ST>
+struct PopT {
+  using P = typename Traits<ST>::C;
+};
+template <>
+struct PopT<kF16> {
+  using P = H16;
+};
+template <int ST>
+using pop_t = typename PopT<ST>::P;
+
+// The compute-type value of a population in register form, and its storage
+// element (exact: an f16 in place is its own element).
+template <int ST>
+__device__ inline typename Traits<ST>::C pop_c(typename Traits<ST>::C v) {
+  return v;
+}
+template <int ST>
+__device__ inline float pop_c(H16 v) {
+  return to_f(v);
+}
+template <int ST>
+__device__ inline typename Traits<ST>::S pop_pack(typename Traits<ST>::C v) {
+  return Traits<ST>::pack(v);
+}
+template <int ST>
+__device__ inline _Float16 pop_pack(H16 v) {
+  return __builtin_bit_cast(_Float16, (unsigned short)v.v);
+}
+
 // ---------------------------------------------------------------- Philox
 // Philox4x32-10 (Salmon et al., SC'11).  Each round's two 32x32 -> 64-bit
 // products are one v_mad_u64_u32 each (low and high word together) and
@@ -207,28 +282,60 @@ __device__ inline typename Traits<ST>::S fma_store(float a, float b, float c) {
   }
 }
 
-template <int MODE, int ST>
-__device__ inline void collide_shifted(const float (&d)[9],
+// Channel x boundary of one lane, folded into the shifted storages'
+// collision (channel mode, no reference counterpart): the inlet (kind 1,
+// x = 0) and the outlet (kind 2, x = Lx-1) set the site to the equilibrium
+// of their boundary moments (channel_x_boundary states the rules), and the
+// collision of an equilibrium is that equilibrium, so the lane takes those
+// moments and omega = 1: d_q' = w_q (t +/- s) of (rho_b, u_b), rounded once.
+struct LaneBC {
+  int kind;        // 0: interior, 1: inlet, 2: outlet
+  float u_in, kin;  // inlet velocity and 1 / (1 - u_in)
+};
+
+template <int MODE, int ST, bool BC = false, class P>
+__device__ inline void collide_shifted(const P (&d)[9],
                                        typename Traits<ST>::S (&o)[9],
-                                       float omega, float beta) {
+                                       float omega, float beta,
+                                       LaneBC bc = LaneBC{0, 0.0f, 0.0f}) {
   // Every fused multiply-add is written out (the file compiles with
   // contraction off), so the rounding is fixed by this source and every
-  // kernel that inlines it produces the same bits.
-  const float drho = ((d[0] + d[1]) + (d[2] + d[3])) +
-                     ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
-  const float jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
-  const float jy = (d[2] - d[4]) + (d[5] + d[6]) - (d[7] + d[8]);
-  const float rho = 1.0f + drho;
+  // kernel that inlines it produces the same bits, with populations given
+  // as floats or as f16 in place.
+  float drho = paddf(d[8], (padd(d[0], d[1]) + padd(d[2], d[3])) +
+                               (padd(d[4], d[5]) + padd(d[6], d[7])));
+  const float jx = (psub(d[1], d[3]) + psub(d[5], d[6])) + psub(d[8], d[7]);
+  const float jy = (psub(d[2], d[4]) + padd(d[5], d[6])) - padd(d[7], d[8]);
+  float rho = 1.0f + drho;
   const float inv = __builtin_amdgcn_rcpf(rho);
   float ux = jx * inv;
   if (MODE == kModeRef) ux = fmaf(0.5f * beta, inv, ux);
-  const float uy = jy * inv;
+  float uy = jy * inv;
+  float om = omega;
+  if constexpr (BC) {
+    if (bc.kind == 1) {
+      // rho_in - 1 = (D + u_in) / (1 - u_in) from the populations that
+      // streamed in from the domain (the weights of the known directions
+      // sum to 1 - (w1 + w5 + w8) and the w-shifts cancel)
+      const float D = ((to_f(d[0]) + to_f(d[2])) + to_f(d[4])) +
+                      2.0f * ((to_f(d[3]) + to_f(d[6])) + to_f(d[7]));
+      drho = (D + bc.u_in) * bc.kin;
+      ux = bc.u_in;
+      uy = 0.0f;
+    } else if (bc.kind == 2) {
+      drho = 0.0f;  // the outlet holds the reference density, own velocity
+    }
+    if (bc.kind != 0) {
+      rho = 1.0f + drho;
+      om = 1.0f;
+    }
+  }
   const float usq = fmaf(ux, ux, uy * uy);
   const float base = fmaf(-1.5f * rho, usq, drho);
   const float r45 = 4.5f * rho, r3 = 3.0f * rho;
-  const float om1 = 1.0f - omega;
-  const float o0 = omega * (4.0f / 9.0f), o1 = omega * (1.0f / 9.0f),
-              o2 = omega * (1.0f / 36.0f);
+  const float om1 = 1.0f - om;
+  const float o0 = om * (4.0f / 9.0f), o1 = om * (1.0f / 9.0f),
+              o2 = om * (1.0f / 36.0f);
   // eu of directions 1, 2, 5, 6 (their opposites 3, 4, 7, 8 have -eu)
   const float e1 = ux, e2 = uy, e5 = ux + uy, e6 = uy - ux;
   const float t1 = fmaf(r45 * e1, e1, base), s1 = r3 * e1;
@@ -241,11 +348,12 @@ __device__ inline void collide_shifted(const float (&d)[9],
                        t5 + s5, t6 + s6, t5 - s5, t6 - s6};
   if (MODE != kModeRef) {
 #pragma unroll
-    for (int q = 0; q < 9; q++) o[q] = fma_store<ST>(oq[q], tq[q], om1 * d[q]);
+    for (int q = 0; q < 9; q++)
+      o[q] = fma_store<ST>(oq[q], tq[q], pmul(om1, d[q]));
   } else {
     float n[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) n[q] = fmaf(oq[q], tq[q], om1 * d[q]);
+    for (int q = 0; q < 9; q++) n[q] = fmaf(oq[q], tq[q], pmul(om1, d[q]));
     // Guo forcing w_q (1 - omega/2) beta (3 (cx - ux) + 9 cx eu)
     const float G = (1.0f - 0.5f * omega) * beta;
     const float g0 = (4.0f / 9.0f) * G, g1 = (1.0f / 9.0f) * G,
@@ -441,6 +549,7 @@ struct BufIO<kF64> {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
   }
   __device__ static double cvt(R v) { return v; }
+  __device__ static double pop(R v) { return v; }
   __device__ static double ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
     return raw(r, vo, so);
   }
@@ -464,6 +573,7 @@ struct BufIO<kF32> {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
   }
   __device__ static float cvt(R v) { return v; }
+  __device__ static float pop(R v) { return v; }
   __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
     return raw(r, vo, so);
   }
@@ -486,6 +596,7 @@ struct BufIO<kF16> {
     return __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
   }
   __device__ static float cvt(R v) { return (float)__builtin_bit_cast(_Float16, v); }
+  __device__ static H16 pop(R v) { return H16{v}; }
   __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
     return cvt(raw(r, vo, so));
   }
@@ -508,6 +619,7 @@ struct BufIO<kBF16> {
     return __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
   }
   __device__ static float cvt(R v) { return __uint_as_float((uint32_t)v << 16); }
+  __device__ static float pop(R v) { return cvt(v); }
   __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
     return cvt(raw(r, vo, so));
   }
@@ -544,19 +656,35 @@ struct HbmIO {
 // The rings sit from LDS address 0 (the step kernels declare no static LDS,
 // so their dynamic LDS starts there), ring k at k * BYTES, and are addressed
 // absolutely: ring and direction plane go into the instruction's offset
-// field and no base is added.
+// field and no base is added.  Each direction plane holds its RCH slots
+// followed by copies of slots 0 and 1 (written with them), so that the
+// three source columns of a pulled row are one masked base plus an
+// immediate offset of 0, 1 or 2 sites.
 template <int ST, int RCH, int RING = 0>
 struct LdsIO {
   typedef __attribute__((address_space(3))) typename Traits<ST>::S LS;
   static constexpr int ES = (int)sizeof(typename Traits<ST>::S);
   static constexpr int BMASK = RCH * ES - 1;  // ring bytes per direction - 1
-  static constexpr int BYTES = 9 * RCH * ES;  // one ring
-  __device__ static LS* at(int q, int boff) {
-    return (LS*)(size_t)(unsigned)(RING * BYTES + q * RCH * ES + (boff & BMASK));
+  static constexpr int PLANEB = (RCH + 2) * ES;  // one direction plane
+  static constexpr int BYTES = (int)ring_bytes(RCH, ES);  // one ring
+  __device__ static LS* at_abs(int q, int b) {
+    return (LS*)(size_t)(unsigned)(RING * BYTES + q * PLANEB + b);
   }
+  __device__ static LS* at(int q, int boff) { return at_abs(q, boff & BMASK); }
   __device__ typename Traits<ST>::C ld_byte(int q, int boff) const {
     typename Traits<ST>::S e = *at(q, boff);
     return Traits<ST>::ld(&e);
+  }
+  // register form of the element at byte b of plane q (b in [0, PLANEB))
+  __device__ pop_t<ST> pop_abs(int q, int b) const {
+    typename Traits<ST>::S e = *at_abs(q, b);
+    if constexpr (ST == kF16)
+      return H16{(unsigned)__builtin_bit_cast(unsigned short, e)};
+    else
+      return Traits<ST>::ld(&e);
+  }
+  __device__ pop_t<ST> pop_byte(int q, int boff) const {
+    return pop_abs(q, boff & BMASK);
   }
   __device__ typename Traits<ST>::C ld(int q, int site) const {
     return ld_byte(q, site * ES);
@@ -567,12 +695,13 @@ struct LdsIO {
   }
 };
 
-// Collision of one site's populations in the storage's compute type; the
-// result comes out as storage elements.
-template <int MODE, int ST>
-__device__ inline void collide_any(const typename Traits<ST>::C (&f)[9],
+// Collision of one site's populations (register form: the compute type,
+// or f16 in place); the result comes out as storage elements.  BC: the
+// lane's channel x boundary folded in (shifted storages).
+template <int MODE, int ST, bool BC = false, class P>
+__device__ inline void collide_any(const P (&f)[9],
                                    typename Traits<ST>::S (&o)[9], double beta,
-                                   double tau) {
+                                   double tau, LaneBC bc = LaneBC{0, 0.0f, 0.0f}) {
   using T = Traits<ST>;
   if constexpr (!T::shifted) {
     double ff[9];
@@ -582,7 +711,7 @@ __device__ inline void collide_any(const typename Traits<ST>::C (&f)[9],
 #pragma unroll
     for (int q = 0; q < 9; q++) o[q] = ff[q];
   } else {
-    collide_shifted<MODE, ST>(f, o, (float)(1.0 / tau), (float)beta);
+    collide_shifted<MODE, ST, BC>(f, o, (float)(1.0 / tau), (float)beta, bc);
   }
 }
 
@@ -705,15 +834,26 @@ __device__ inline void store_macro(const typename Traits<ST>::C (&f)[9],
 
 // Storage elements of a site: collided (any pass but stream-only) or packed
 // as they are.
-template <int MODE, int ST>
-__device__ inline void finish_site(const typename Traits<ST>::C (&f)[9],
+template <int MODE, int ST, bool BC = false, class P>
+__device__ inline void finish_site(const P (&f)[9],
                                    typename Traits<ST>::S (&o)[9], int PH,
-                                   double beta, double tau) {
+                                   double beta, double tau,
+                                   LaneBC bc = LaneBC{0, 0.0f, 0.0f}) {
   if (PH != kStream) {
-    collide_any<MODE, ST>(f, o, beta, tau);
+    collide_any<MODE, ST, BC>(f, o, beta, tau, bc);
+  } else if constexpr (BC) {
+    // no collision to fold the x boundary into: a boundary lane stores its
+    // boundary equilibrium
+    typename Traits<ST>::C fc[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) fc[q] = pop_c<ST>(f[q]);
+    if (bc.kind != 0)
+      channel_x_boundary<ST>(fc, bc.kind == 1, (double)bc.u_in, (double)bc.kin);
+#pragma unroll
+    for (int q = 0; q < 9; q++) o[q] = Traits<ST>::pack(fc[q]);
   } else {
 #pragma unroll
-    for (int q = 0; q < 9; q++) o[q] = Traits<ST>::pack(f[q]);
+    for (int q = 0; q < 9; q++) o[q] = pop_pack<ST>(f[q]);
   }
 }
 
@@ -788,14 +928,14 @@ __device__ inline void join9(R (&v)[9]) {
                "+v"(v[5]), "+v"(v[6]), "+v"(v[7]), "+v"(v[8]));
 }
 
-// Pulls of a fast-path site.  From HBM: one buffer load per direction at
-// the site's byte offset with the wave-uniform soffset q*plane - cy*Lx - cx;
-// from the LDS ring: the ring slot of site - cy*Lx - cx.  voL / voR (sL / sR)
-// fold in the x boundary for directions with cx = +1 / -1.
+// Pulls of a site.  From HBM: one buffer load per direction at the site's
+// byte offset with the wave-uniform soffset q*plane - cy*Lx - cx; from the
+// LDS ring: the ring slot of site - cy*Lx - cx.  voL / voR (adjL / adjR)
+// fold in the x boundary for directions with cx = +1 / -1 (waves holding
+// an x = 0 or x = Lx-1 site); pull9_rows serves every other wave.
 template <int ST, int RCH, int RING>
-__device__ inline void pull9(const LdsIO<ST, RCH, RING>& io,
-                             typename Traits<ST>::C (&f)[9], int site, int adjL,
-                             int adjR, int rowb) {
+__device__ inline void pull9(const LdsIO<ST, RCH, RING>& io, pop_t<ST> (&f)[9],
+                             int site, int adjL, int adjR, int rowb) {
   constexpr int ES = LdsIO<ST, RCH, RING>::ES;
   // source byte offsets: column x - cx (x boundary folded in), then the row
   // y - cy one row-length (rowb bytes) away
@@ -804,8 +944,24 @@ __device__ inline void pull9(const LdsIO<ST, RCH, RING>& io,
   const int col[9] = {b0, bP, b0, bM, b0, bP, bM, bM, bP};
 #pragma unroll
   for (int q = 0; q < 9; q++)
-    f[q] = io.ld_byte(q, CY[q] > 0 ? col[q] - rowb
-                                   : (CY[q] < 0 ? col[q] + rowb : col[q]));
+    f[q] = io.pop_byte(q, CY[q] > 0 ? col[q] - rowb
+                                    : (CY[q] < 0 ? col[q] + rowb : col[q]));
+}
+
+// The same pulls away from the x boundary: one masked ring offset per
+// source row (y - 1, y, y + 1), one site left of the site's column, and the
+// columns x - 1, x, x + 1 at 0, 1, 2 sites from it (the plane's two mirror
+// slots cover a row that wraps the ring).
+template <int ST, int RCH, int RING>
+__device__ inline void pull9_rows(const LdsIO<ST, RCH, RING>& io,
+                                  pop_t<ST> (&f)[9], int site, int rowb) {
+  using IO = LdsIO<ST, RCH, RING>;
+  constexpr int ES = IO::ES;
+  const int b = site * ES - ES;
+  const int row[3] = {(b - rowb) & IO::BMASK, b & IO::BMASK,
+                      (b + rowb) & IO::BMASK};
+#pragma unroll
+  for (int q = 0; q < 9; q++) f[q] = io.pop_abs(q, row[1 - CY[q]] + (1 - CX[q]) * ES);
 }
 
 template <int ST>
@@ -844,8 +1000,15 @@ template <int ST, bool NT, int RCH, int RING>
 __device__ inline void put9(const LdsIO<ST, RCH, RING>& io,
                             const typename Traits<ST>::S (&o)[9], int site,
                             const PassConst<ST>&) {
+  using IO = LdsIO<ST, RCH, RING>;
 #pragma unroll
   for (int q = 0; q < 9; q++) io.template st<NT>(q, site, o[q]);
+  // slots 0 and 1 are also kept after the plane's last slot
+  const int slot = site & (RCH - 1);
+  if (slot < 2) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) *IO::at_abs(q, (RCH + slot) * IO::ES) = o[q];
+  }
 }
 
 // Per-pass constants of the pull and store addressing: two wave-uniform
@@ -952,48 +1115,85 @@ __device__ inline void pull_issue(const Geo& g, const WaveGeo& w, int lane,
                           pc.soff(q));
 }
 
-// The nine pulled populations of a site.
+// The nine pulled populations of a site, in register form.
 template <int ST>
 __device__ inline void pull_site(const Geo& g, const WaveGeo& w, int lane,
                                  int site, const HbmIO<ST>& src,
-                                 const PassConst<ST>& pc,
-                                 typename Traits<ST>::C (&f)[9]) {
+                                 const PassConst<ST>& pc, pop_t<ST> (&f)[9]) {
   typename BufIO<ST>::R v[9];
   pull_issue(g, w, lane, site, src, pc, v);
   join9(v);
 #pragma unroll
-  for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(v[q]);
+  for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::pop(v[q]);
 }
 
 template <int ST, int RCH, int RING>
 __device__ inline void pull_site(const Geo& g, const WaveGeo& w, int lane,
                                  int site, const LdsIO<ST, RCH, RING>& src,
-                                 const PassConst<ST>& pc,
-                                 typename Traits<ST>::C (&f)[9]) {
-  int aL, aR;
-  x_adjust(g, w, lane, pc, aL, aR);
-  pull9(src, f, site, aL, aR, pc.rowb);
+                                 const PassConst<ST>& pc, pop_t<ST> (&f)[9]) {
+  if (w.edge) {
+    int aL, aR;
+    x_adjust(g, w, lane, pc, aL, aR);
+    pull9(src, f, site, aL, aR, pc.rowb);
+  } else {
+    pull9_rows(src, f, site, pc.rowb);
+  }
+}
+
+// A site's own populations in register form (the collide-only pass).
+template <int ST>
+__device__ inline void own_pop(const HbmIO<ST>& io, pop_t<ST> (&f)[9], int site,
+                               const PassConst<ST>& pc) {
+  constexpr int ES = HbmIO<ST>::ES;
+  typename BufIO<ST>::R v[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) v[q] = BufIO<ST>::raw(io.r, site * ES, pc.sown(q));
+  join9(v);
+#pragma unroll
+  for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::pop(v[q]);
+}
+
+template <int ST, int RCH, int RING>
+__device__ inline void own_pop(const LdsIO<ST, RCH, RING>& io, pop_t<ST> (&f)[9],
+                               int site, const PassConst<ST>&) {
+#pragma unroll
+  for (int q = 0; q < 9; q++) f[q] = io.pop_byte(q, site * LdsIO<ST, RCH, RING>::ES);
 }
 
 // The rest of one site of one pass, given its nine pulled populations f
 // (own populations in the collide-only pass).  Waves holding wall or square
 // lanes load each lane's own populations and take the bounce-back ones
 // from there (ring lanes add their drag term, solid lanes store the rest
-// state); channel x-boundary lanes apply the boundary rule.  Collision
-// (except in the stream-only pass) and nine stores follow.  Boundary waves
-// and the rest run separate copies of the collide-and-store tail, so the
-// common path carries neither the solid-lane select nor register copies
-// where the two would merge.
-template <int MODE, int ST, bool NT, class SRC, class DST, class DV>
-__device__ inline void collide_store(typename Traits<ST>::C (&f)[9], bool solid,
-                                     const DST& dst, const PassConst<ST>& pc,
-                                     const DV& dv, double beta, double tau,
-                                     int site, int plane, int PH) {
+// state); channel x-boundary lanes apply the boundary rule (folded into the
+// collision in the shifted storages).  Collision (except in the
+// stream-only pass) and nine stores follow.  Boundary waves and the rest
+// run separate copies of the collide-and-store tail, so the common path
+// carries neither the solid-lane select nor register copies where the two
+// would merge.
+template <int MODE, int ST, bool NT, class SRC, bool BC = false, class DST,
+          class DV, class P>
+__device__ inline void collide_store(P (&f)[9], bool solid, const DST& dst,
+                                     const PassConst<ST>& pc, const DV& dv,
+                                     double beta, double tau, int site,
+                                     int plane, int PH,
+                                     LaneBC bc = LaneBC{0, 0.0f, 0.0f}) {
   using T = Traits<ST>;
-  if constexpr (DV::kFrame)
-    if (dv.out) store_macro<MODE, ST>(f, solid, beta, dv.out, site, plane);
+  if constexpr (DV::kFrame) {
+    if (dv.out) {
+      typename T::C fc[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) fc[q] = pop_c<ST>(f[q]);
+      if constexpr (BC) {
+        // the boundary lane's fields are those of its boundary equilibrium
+        if (bc.kind != 0)
+          channel_x_boundary<ST>(fc, bc.kind == 1, (double)bc.u_in,
+                                 (double)bc.kin);
+      }
+      store_macro<MODE, ST>(fc, solid, beta, dv.out, site, plane);
+    }
+  }
   typename T::S o[9];
-  finish_site<MODE, ST>(f, o, PH, beta, tau);
+  finish_site<MODE, ST, BC>(f, o, PH, beta, tau, bc);
   if (solid) {
     // the reference resets the solid interior to w every step; its
     // post-collision values never reach a fluid node
@@ -1005,7 +1205,7 @@ __device__ inline void collide_store(typename Traits<ST>::C (&f)[9], bool solid,
 
 template <int MODE, int ST, bool NT, class SRC, class DST, class DV>
 __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
-                                   int site, typename Traits<ST>::C (&f)[9],
+                                   int site, pop_t<ST> (&f)[9],
                                    const SRC& src, const DST& dst,
                                    const PassConst<ST>& pc,
                                    double* __restrict__ ring, const DV& dv,
@@ -1013,10 +1213,18 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
                                    const InstDev* __restrict__ I, int PH) {
   using T = Traits<ST>;
   using C = typename T::C;
+  // channel x boundary folded into the collision (shifted storages)
+  constexpr bool kFold = MODE == kModeChannel && T::shifted;
   bool solid = false;
   if (PH != kCollide && (w.edge || w.special)) {
+    C fc[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) fc[q] = pop_c<ST>(f[q]);
     int x, y;
     lane_xy(g, w, lane, x, y);
+    LaneBC bc{0, (float)I->u_in, (float)I->inlet_k};
+    const bool xb = MODE == kModeChannel && (x == 0 || x == g.Lx - 1);
+    bool in_box = false;
     if (w.special) {
       C own[9];
       own9(src, own, site, pc);
@@ -1024,23 +1232,23 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
                     square_mask(x, y, g.x0, g.x1, g.y0, g.y1);
 #pragma unroll
       for (int q = 0; q < 9; q++)
-        if ((m >> q) & 1) f[q] = own[OPP[q]];
-      const bool in_box = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1;
+        if ((m >> q) & 1) fc[q] = own[OPP[q]];
+      in_box = x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1;
       solid = in_box && x > g.x0 && x < g.x1 && y > g.y0 && y < g.y1;
-      if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1))
-        channel_x_boundary<ST>(f, x == 0, I->u_in, I->inlet_k);
-      if (in_box && !solid) ring_drag<MODE, ST>(g, x, y, f, ring, tau);
-    } else if (MODE == kModeChannel && (x == 0 || x == g.Lx - 1)) {
-      channel_x_boundary<ST>(f, x == 0, I->u_in, I->inlet_k);
     }
-    collide_store<MODE, ST, NT, SRC>(f, solid, dst, pc, dv, beta, tau, site,
-                                     g.plane,
-                                     PH);
+    if (xb) {
+      if constexpr (kFold)
+        bc.kind = (x == 0) ? 1 : 2;
+      else
+        channel_x_boundary<ST>(fc, x == 0, I->u_in, I->inlet_k);
+    }
+    if (in_box && !solid) ring_drag<MODE, ST>(g, x, y, fc, ring, tau);
+    collide_store<MODE, ST, NT, SRC, kFold>(fc, solid, dst, pc, dv, beta, tau,
+                                            site, g.plane, PH, bc);
     return;
   }
   collide_store<MODE, ST, NT, SRC>(f, false, dst, pc, dv, beta, tau, site,
-                                   g.plane,
-                                   PH);
+                                   g.plane, PH);
 }
 
 // One site of one pass: the collide-only pass loads the site's own
@@ -1053,9 +1261,9 @@ __device__ inline void site_step(const Geo& g, const WaveGeo& w, int lane,
                                  double* __restrict__ ring, const DV& dv,
                                  double beta, double tau,
                                  const InstDev* __restrict__ I, int PH) {
-  typename Traits<ST>::C f[9];
+  pop_t<ST> f[9];
   if (PH == kCollide)
-    own9(src, f, site, pc);
+    own_pop(src, f, site, pc);
   else
     pull_site(g, w, lane, site, src, pc, f);
   site_finish<MODE, ST, NT>(g, w, lane, site, f, src, dst, pc, ring, dv, beta,
@@ -1102,13 +1310,14 @@ __device__ inline void lds_barrier() {
 
 // NP (2 or 3) stream-and-collide passes in one sweep over the lattice.  The
 // first pass reads HBM and writes its post-collision populations into LDS
-// ring 0; each later pass runs L chunks behind the one before, reads the
-// previous ring, and writes the next ring or (the last pass) HBM.  A chunk
-// is T consecutive sites (one per thread); L*T >= the pull reach, so every
-// pull finds its source already in the ring, and a ring (>= 2L + 2 chunks)
-// is never overwritten while a later pass still needs it.  One LDS barrier
-// between consecutive passes of a chunk; the sweep's caller ends it with a
-// full barrier.  drag[k*ring_max] holds pass k's drag slots.
+// ring 0; each later pass runs D chunks behind the one before (D = L for
+// two passes, L + 1 for three), reads the previous ring, and writes the
+// next ring or (the last pass) HBM.  A chunk is T consecutive sites (one per
+// thread); L*T >= the pull reach, so every pull finds its source already in
+// the ring, and a ring (>= 2L + 2 chunks) is never overwritten while a later
+// pass still needs it.  One LDS barrier per chunk step (between the two
+// passes, or after the three); the sweep's caller ends it with a full
+// barrier.  drag[k*ring_max] holds pass k's drag slots.
 template <int MODE, int ST, bool NT, int RCH, int NP, class DV>
 __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
                                    const HbmIO<ST>& src, const HbmIO<ST>& dst,
@@ -1125,7 +1334,14 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
   const int CH = nthr;
   const int L = (pull_reach(MODE, g.Lx) + CH - 1) / CH;
   const int nch = (g.plane + CH - 1) / CH;
-  const int total = nch + (NP - 1) * L;  // chunk steps of the sweep
+  // lag in chunks of each pass behind the one before.  Two passes: L, with
+  // the chunk step's barrier between the passes.  Three passes: L + 1, so
+  // every pull reads a ring chunk written in an earlier chunk step and one
+  // barrier at the end of the step orders them all (the rings hold the same
+  // 2L + 2 chunks: a pass's pulls in step c reach chunks c-2L-1 .. c-1 of
+  // the ring the pass before writes chunk c of)
+  const int D = NP == 3 ? L + 1 : L;
+  const int total = nch + (NP - 1) * D;  // chunk steps of the sweep
   const int lane = tid & 63;
   const int wblk = __builtin_amdgcn_readfirstlane(tid) >> 6;  // wave in chunk
   const int bpc = CH >> 6;                                    // waves per chunk
@@ -1171,29 +1387,28 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
   auto step = [&](int c, R (&pre)[9]) {
     pc.renew();
     const double b0 = beta_at(c);
-    const double b1 = NP == 3 ? beta_at(c - L) : 0.0;
-    const double b2 = beta_at(c - (NP - 1) * L);
+    const double b1 = NP == 3 ? beta_at(c - D) : 0.0;
+    const double b2 = beta_at(c - (NP - 1) * D);
     if (PF > 1 || c < nch) {
       const int s = c * CH + tid;
-      typename Traits<ST>::C f[9];
+      pop_t<ST> f[9];
 #pragma unroll
-      for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::cvt(pre[q]);
-      // the converted values first, then chunk c + PF's loads into the
-      // same registers (loads cannot move above a memory clobber)
-      asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]), "+v"(f[3]),
-                   "+v"(f[4]), "+v"(f[5]), "+v"(f[6]), "+v"(f[7]), "+v"(f[8])
-                   :: "memory");
-      const int cp = min(c + PF, nch - 1);
-      pull_issue(g, wave_of(I, cp * bpc + wblk), lane, cp * CH + tid, src, pc,
-                 pre);
+      for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::pop(pre[q]);
       if (c < nch && s < g.plane)
         site_finish<MODE, ST, false>(g, wave_of(I, c * bpc + wblk), lane, s, f,
                                      src, r0, pc, drag, dv, beta_of(b0, s), tau, I,
                                      kBoth);
+      // then chunk c + PF's loads into the same registers, once their values
+      // are consumed (loads cannot move above a memory clobber), so no
+      // register copies are needed
+      asm volatile("" ::: "memory");
+      const int cp = min(c + PF, nch - 1);
+      pull_issue(g, wave_of(I, cp * bpc + wblk), lane, cp * CH + tid, src, pc,
+                 pre);
     }
-    lds_barrier();
+    if constexpr (NP == 2) lds_barrier();
     if constexpr (NP == 3) {
-      const int cm = c - L;
+      const int cm = c - D;
       if (cm >= 0 && cm < nch) {
         const int s = cm * CH + tid;
         if (s < g.plane)
@@ -1202,9 +1417,8 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
                                      tau, I,
                                      kBoth);
       }
-      lds_barrier();
     }
-    const int cz = c - (NP - 1) * L;
+    const int cz = c - (NP - 1) * D;
     if (cz >= 0 && cz < nch) {
       const int s = cz * CH + tid;
       if (s < g.plane) {
@@ -1219,6 +1433,7 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
                                   kBoth);
       }
     }
+    if constexpr (NP == 3) lds_barrier();
   };
   // a multiple of PF steps; steps past the last do nothing
   for (int c = 0; c < total; c += PF) {
