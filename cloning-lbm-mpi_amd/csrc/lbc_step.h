@@ -96,12 +96,12 @@ struct Traits<kBF16> {
 // subnormals included); the product takes a -0.0 addend so that a zero
 // product keeps its sign.
 struct H16 {
-  unsigned v;
+  unsigned short v;  // a 16-bit operand: the upper half of its VGPR is unused
 };
 
 __device__ inline float to_f(float a) { return a; }
 __device__ inline float to_f(H16 a) {
-  return (float)__builtin_bit_cast(_Float16, (unsigned short)a.v);
+  return (float)__builtin_bit_cast(_Float16, a.v);
 }
 // a + b, a - b, x + a, s * a on populations a, b (float or f16 in place)
 __device__ inline float padd(float a, float b) { return a + b; }
@@ -159,7 +159,7 @@ __device__ inline typename Traits<ST>::S pop_pack(typename Traits<ST>::C v) {
 }
 template <int ST>
 __device__ inline _Float16 pop_pack(H16 v) {
-  return __builtin_bit_cast(_Float16, (unsigned short)v.v);
+  return __builtin_bit_cast(_Float16, v.v);
 }
 
 // ---------------------------------------------------------------- Philox
@@ -678,8 +678,11 @@ struct LdsIO {
   // register form of the element at byte b of plane q (b in [0, PLANEB))
   __device__ pop_t<ST> pop_abs(int q, int b) const {
     typename Traits<ST>::S e = *at_abs(q, b);
+    return elem_pop(e);
+  }
+  __device__ static pop_t<ST> elem_pop(typename Traits<ST>::S e) {
     if constexpr (ST == kF16)
-      return H16{(unsigned)__builtin_bit_cast(unsigned short, e)};
+      return H16{__builtin_bit_cast(unsigned short, e)};
     else
       return Traits<ST>::ld(&e);
   }
@@ -946,6 +949,7 @@ __device__ inline void pull9(const LdsIO<ST, RCH, RING>& io, pop_t<ST> (&f)[9],
   for (int q = 0; q < 9; q++)
     f[q] = io.pop_byte(q, CY[q] > 0 ? col[q] - rowb
                                     : (CY[q] < 0 ? col[q] + rowb : col[q]));
+  asm volatile("" ::: "memory");  // see pull9_rows
 }
 
 // The same pulls away from the x boundary: one masked ring offset per
@@ -953,15 +957,20 @@ __device__ inline void pull9(const LdsIO<ST, RCH, RING>& io, pop_t<ST> (&f)[9],
 // columns x - 1, x, x + 1 at 0, 1, 2 sites from it (the plane's two mirror
 // slots cover a row that wraps the ring).
 template <int ST, int RCH, int RING>
-__device__ inline void pull9_rows(const LdsIO<ST, RCH, RING>& io,
+__device__ inline void pull9_rows(const LdsIO<ST, RCH, RING>&,
                                   pop_t<ST> (&f)[9], int site, int rowb) {
   using IO = LdsIO<ST, RCH, RING>;
   constexpr int ES = IO::ES;
   const int b = site * ES - ES;
-  const int row[3] = {(b - rowb) & IO::BMASK, b & IO::BMASK,
-                      (b + rowb) & IO::BMASK};
+  // the three rows as element pointers into plane 0 of the ring; plane and
+  // column are constant element indices (the instruction's offset field)
+  typename IO::LS* row[3] = {IO::at(0, b - rowb), IO::at(0, b), IO::at(0, b + rowb)};
 #pragma unroll
-  for (int q = 0; q < 9; q++) f[q] = io.pop_abs(q, row[1 - CY[q]] + (1 - CX[q]) * ES);
+  for (int q = 0; q < 9; q++)
+    f[q] = IO::elem_pop(row[1 - CY[q]][q * (RCH + 2) + (1 - CX[q])]);
+  // keeps these loads in this branch: merged with the edge waves' pulls
+  // they would take a per-direction address register each
+  asm volatile("" ::: "memory");
 }
 
 template <int ST>
@@ -1324,11 +1333,12 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
                                    double* drag, int ring_max, const DV& dv,
                                    double tau, const InstDev* __restrict__ I) {
   static_assert(NP == 2 || NP == 3, "two or three passes per sweep");
-  // HBM prefetch distance in chunks: each wave keeps two chunks' first-pass
-  // loads in flight (measured: 1-2% faster than one at two passes per
-  // sweep, 9% at three, where the rings leave half the waves; three chunks
-  // gain nothing more)
-  constexpr int PF = 2;
+  // HBM prefetch distance in chunks: each wave keeps PF chunks' first-pass
+  // loads in flight.  Measured at 400x100 in FP16: two chunks are 1-2%
+  // faster than one at two passes per sweep (three gain nothing more); at
+  // three passes, where the rings leave half the waves, three chunks are
+  // 3.5% faster than two (four gain nothing more)
+  constexpr int PF = NP == 3 ? 3 : 2;
   using R = typename BufIO<ST>::R;
   PassConst<ST> pc(g, MODE);
   const int CH = nthr;
