@@ -528,8 +528,9 @@ def calibrate_port(O, L: int = 50, steps: int = 60) -> dict | None:
                 port_over_reference=round(rates["port"] / rates["reference"], 3))
 
 
-KERNEL_SOURCES = ("csrc/lbc_step.h", "csrc/lbc_internal.h",
-                  "csrc/lbc_kernels.hip", "csrc/lbc_forced.hip")
+# the step kernel's device code (lbc_kernels.hip and lbc_forced.hip only
+# instantiate and launch it)
+KERNEL_SOURCES = ("csrc/lbc_step.h", "csrc/lbc_internal.h")
 
 
 def kernel_source_sha16() -> str:
