@@ -339,11 +339,14 @@ __global__ void __launch_bounds__(256)
     for (; i < nv; i += bd) vd[i] = vs[i];
     return;
   }
-  for (int s = threadIdx.x; s < plane; s += blockDim.x) {
+  // site coordinates advance by the stride without a division per site
+  const int bd = blockDim.x;
+  const int dy = bd / Lx, dx = bd - dy * Lx;
+  int x = threadIdx.x % Lx, y = threadIdx.x / Lx;
+  for (int s = threadIdx.x; s < plane; s += bd) {
     typename T::C f[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) f[q] = T::ld(src + q * plane + s);
-    const int x = s % Lx, y = s / Lx;
     const bool interior = x > I->xmin && x < I->xmax && y > I->ymin && y < I->ymax;
     if (!interior) {
 #pragma unroll
@@ -368,6 +371,12 @@ __global__ void __launch_bounds__(256)
     }
 #pragma unroll
     for (int q = 0; q < 9; q++) T::st(dst + q * plane + s, f[q]);
+    x += dx;
+    y += dy;
+    if (x >= Lx) {
+      x -= Lx;
+      y += 1;
+    }
   }
 }
 
