@@ -1613,8 +1613,17 @@ __device__ inline void interval_fused(const InstDev* __restrict__ insts,
   if (tid == 0) finish_clone<MODE>(I, wi, acc);
 }
 
+// Largest workgroup of the multi-pass kernel in storage ST.  An FP64 ring
+// of 1024-site chunks (at least four of them, 295 KB) never fits the CU's
+// LDS, so FP64 runs at most 512 threads, and the bound lets its waves hold
+// up to 256 VGPRs instead of spilling at 128.
+template <int ST>
+constexpr int fused_max_threads() {
+  return ST == kF64 ? 512 : 1024;
+}
+
 template <int MODE, int ST, bool NT, int RCH, int NP, bool FRC>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(fused_max_threads<ST>())
     k_interval_fused(const InstDev* __restrict__ insts,
                      const WorkItem* __restrict__ items,
                      const Counters* __restrict__ counters, int ring_max) {
@@ -1647,7 +1656,7 @@ hipError_t raise_lds_limit(K kernel, size_t lds) {
 template <int MODE, int ST, bool NT, int RCH, int NP, bool FRC>
 hipError_t launch_fused(const Launch& L) {
   const size_t lds = fused_lds_bytes(ST, L.ring_max, RCH, NP);
-  if (!lds) return hipErrorInvalidValue;
+  if (!lds || L.threads > fused_max_threads<ST>()) return hipErrorInvalidValue;
   if constexpr (sizeof(typename Traits<ST>::S) == 2 && NP == 2) {
     hipError_t e =
         raise_lds_limit(k_interval_fused_w8<MODE, ST, NT, RCH, FRC>, lds);
