@@ -679,13 +679,17 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     if (hipMemcpy(d.labels, ident.data(), sizeof(int32_t) * n,
                   hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(c, LBC_E_HIP, "label init failed"));
-    // wave table: one word per block of 64 sites, plus two chunks' worth of
-    // blocks past the lattice (the last, partial chunk and the step kernel's
-    // one-chunk lookahead)
-    std::vector<uint32_t> waves((d.plane + 63) / 64 + 2 * 1024 / 64, 0u);
-    for (int b = 0; b * 64 < d.plane; b++)
-      waves[b] = lbc::wave_word(d.Lx, d.Ly, d.xmin, d.xmax, d.ymin, d.ymax,
-                                64 * b);
+    // wave table: two words per block of 64 sites, plus two chunks' worth
+    // of blocks past the lattice (the last, partial chunk and the step
+    // kernel's one-chunk lookahead)
+    std::vector<uint32_t> waves(2 * ((d.plane + 63) / 64 + 2 * 1024 / 64), 0u);
+    for (size_t b = 0; 2 * b < waves.size(); b++) {
+      if ((long long)b * 64 < d.plane)
+        waves[2 * b] = lbc::wave_word(d.Lx, d.Ly, d.xmin, d.xmax, d.ymin,
+                                      d.ymax, 64 * (int)b);
+      waves[2 * b + 1] = lbc::wave_lanes(d.Lx, d.Ly, d.xmin, d.xmax, d.ymin,
+                                         d.ymax, d.plane, 64 * (int)b);
+    }
     uint32_t* wt = nullptr;
     if (dalloc(c, &wt, sizeof(uint32_t) * waves.size()))
       return bail(LBC_E_NOMEM);

@@ -41,7 +41,8 @@ struct InstDev {
   double* prefactors;      // [n_clones][n_snapshots]
   double* snapshots;       // [n_snapshots][9*plane] FP64 SoA
   double* norms;           // [n_clones][2] REF perturbation norms
-  const uint32_t* wave_info;  // one word per block of 64 sites (wave_word)
+  const uint32_t* wave_info;  // two words per block of 64 sites (wave_word,
+                              // wave_lanes)
   // random-field body force (forcing mode); forced_steps = 0: off
   int forced_steps;        // leading steps of an interval that use it
   int fmx, fmy;            // circulant embedding (2Lx-1) x (2Ly-1)
@@ -131,6 +132,68 @@ inline uint32_t wave_word(int Lx, int Ly, int x0, int x1, int y0, int y1,
   }
   return (uint32_t)x | ((uint32_t)edge << kWaveEdgeBit) | ((uint32_t)y << 16) |
          ((uint32_t)special << kWaveSpecialBit);
+}
+
+// Second word of a block's wave-table entry: how the step kernel runs the
+// block (bits 30-31) and, for lattices at least 64 wide, the lane holding
+// the block's x = 0 site (bits 0-6) and the lane holding its x = Lx-1 site
+// (bits 8-14), kWaveNoLane where there is none (such a block holds at most
+// one of each).
+//   kWavePlain:   64 lattice sites, none of them on a wall row, in the
+//                 square's box or at an x boundary;
+//   kWaveXEdge:   64 lattice sites, x-boundary lanes but no wall-row or box
+//                 site;
+//   kWaveGeneric: wall-row or box sites, or lanes past the lattice's last
+//                 site (Lx >= 64);
+//   kWaveNarrow:  Lx < 64: several x-boundary lanes per block, found from
+//                 the lanes' coordinates.
+constexpr uint32_t kWaveNoLane = 0x7Fu;
+constexpr int kWaveKindShift = 30;
+constexpr int kWavePlain = 0, kWaveXEdge = 1, kWaveGeneric = 2, kWaveNarrow = 3;
+
+inline uint32_t wave_lanes(int Lx, int Ly, int x0, int x1, int y0, int y1,
+                           int plane, int site0) {
+  if (Lx < 64) return (uint32_t)kWaveNarrow << kWaveKindShift;
+  uint32_t l0 = kWaveNoLane, l1 = kWaveNoLane;
+  for (int l = 0; l < 64 && site0 + l < plane; l++) {
+    const int x = (site0 + l) % Lx;
+    if (x == 0) l0 = (uint32_t)l;
+    if (x == Lx - 1) l1 = (uint32_t)l;
+  }
+  int kind = kWavePlain;
+  if (site0 + 64 > plane) {
+    kind = kWaveGeneric;
+  } else {
+    const uint32_t w = wave_word(Lx, Ly, x0, x1, y0, y1, site0);
+    if ((w >> kWaveSpecialBit) & 1u)
+      kind = kWaveGeneric;
+    else if ((w >> kWaveEdgeBit) & 1u)
+      kind = kWaveXEdge;
+  }
+  return l0 | (l1 << 8) | ((uint32_t)kind << kWaveKindShift);
+}
+
+// Bounce-back directions of the square ring at (x,y) (the writes of
+// computeSquareBounceBack_TEST, src/lbm_src/square.cpp:24-50).
+__host__ __device__ inline int square_mask(int x, int y, int x0, int x1,
+                                           int y0, int y1) {
+  if (x < x0 || x > x1 || y < y0 || y > y1) return 0;
+  const bool inx = (x > x0 && x < x1), iny = (y > y0 && y < y1);
+  if (inx && iny) return 0;
+  if (y == y0 && inx) return (1 << 4) | (1 << 7) | (1 << 8);
+  if (y == y1 && inx) return (1 << 2) | (1 << 5) | (1 << 6);
+  if (x == x0 && iny) return (1 << 3) | (1 << 6) | (1 << 7);
+  if (x == x1 && iny) return (1 << 1) | (1 << 5) | (1 << 8);
+  if (x == x0) return (y == y0) ? (1 << 7) : (1 << 6);
+  return (y == y1) ? (1 << 5) : (1 << 8);
+}
+
+// Wall bounce-back directions (src/lbm_src/domain_noSlipWalls.cpp:10-21).
+__host__ __device__ inline int wall_mask(int y, int Ly) {
+  int m = 0;
+  if (y == 0) m |= (1 << 2) | (1 << 5) | (1 << 6);
+  if (y == Ly - 1) m |= (1 << 4) | (1 << 7) | (1 << 8);
+  return m;
 }
 
 // Grouping of the M+1 passes of an interval into HBM sweeps with P passes

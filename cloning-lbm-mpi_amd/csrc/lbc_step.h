@@ -95,13 +95,17 @@ struct Traits<kBF16> {
 // (checked on the device against v_cvt_f32_f16 for every f16 bit pattern,
 // subnormals included); the product takes a -0.0 addend so that a zero
 // product keeps its sign.
+// The register is held as the 32-bit value the zero-extending loads
+// (buffer_load_ushort, ds_read_u16) leave, upper half zero: typed as a
+// 16-bit value it would be masked with 0xffff before every use as an
+// asm operand, nine VALU instructions per site.
 struct H16 {
-  unsigned short v;  // a 16-bit operand: the upper half of its VGPR is unused
+  uint32_t v;  // the half in the low 16 bits, zeros above
 };
 
 __device__ inline float to_f(float a) { return a; }
 __device__ inline float to_f(H16 a) {
-  return (float)__builtin_bit_cast(_Float16, a.v);
+  return (float)__builtin_bit_cast(_Float16, (unsigned short)a.v);
 }
 // a + b, a - b, x + a, s * a on populations a, b (float or f16 in place)
 __device__ inline float padd(float a, float b) { return a + b; }
@@ -159,7 +163,7 @@ __device__ inline typename Traits<ST>::S pop_pack(typename Traits<ST>::C v) {
 }
 template <int ST>
 __device__ inline _Float16 pop_pack(H16 v) {
-  return __builtin_bit_cast(_Float16, v.v);
+  return __builtin_bit_cast(_Float16, (unsigned short)v.v);
 }
 
 // ---------------------------------------------------------------- Philox
@@ -426,28 +430,6 @@ __device__ inline void node_stress_shifted(const float (&d)[9], float& drho_o,
   pxy_o = pxy;
 }
 
-// Bounce-back directions of the square ring at (x,y) (the writes of
-// computeSquareBounceBack_TEST, src/lbm_src/square.cpp:24-50).
-__device__ inline int square_mask(int x, int y, int x0, int x1, int y0, int y1) {
-  if (x < x0 || x > x1 || y < y0 || y > y1) return 0;
-  const bool inx = (x > x0 && x < x1), iny = (y > y0 && y < y1);
-  if (inx && iny) return 0;
-  if (y == y0 && inx) return (1 << 4) | (1 << 7) | (1 << 8);
-  if (y == y1 && inx) return (1 << 2) | (1 << 5) | (1 << 6);
-  if (x == x0 && iny) return (1 << 3) | (1 << 6) | (1 << 7);
-  if (x == x1 && iny) return (1 << 1) | (1 << 5) | (1 << 8);
-  if (x == x0) return (y == y0) ? (1 << 7) : (1 << 6);
-  return (y == y1) ? (1 << 5) : (1 << 8);
-}
-
-// Wall bounce-back directions (src/lbm_src/domain_noSlipWalls.cpp:10-21).
-__device__ inline int wall_mask(int y, int Ly) {
-  int m = 0;
-  if (y == 0) m |= (1 << 2) | (1 << 5) | (1 << 6);
-  if (y == Ly - 1) m |= (1 << 4) | (1 << 7) | (1 << 8);
-  return m;
-}
-
 // Channel x boundaries on the post-stream populations of one site
 // (no reference counterpart; the oracle states the same rules):
 //  inlet  x = 0:    feq(rho_in, (u_in, 0)), rho_in = (f0 + f2 + f4 +
@@ -591,11 +573,13 @@ struct BufIO<kF32> {
 
 template <>
 struct BufIO<kF16> {
-  using R = unsigned short;
+  using R = uint32_t;  // zero-extended (buffer_load_ushort)
   __device__ static R raw(__amdgpu_buffer_rsrc_t r, int vo, int so) {
-    return __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
   }
-  __device__ static float cvt(R v) { return (float)__builtin_bit_cast(_Float16, v); }
+  __device__ static float cvt(R v) {
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)v);
+  }
   __device__ static H16 pop(R v) { return H16{v}; }
   __device__ static float ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
     return cvt(raw(r, vo, so));
@@ -676,15 +660,17 @@ struct LdsIO {
     return Traits<ST>::ld(&e);
   }
   // register form of the element at byte b of plane q (b in [0, PLANEB))
-  __device__ pop_t<ST> pop_abs(int q, int b) const {
-    typename Traits<ST>::S e = *at_abs(q, b);
-    return elem_pop(e);
-  }
-  __device__ static pop_t<ST> elem_pop(typename Traits<ST>::S e) {
-    if constexpr (ST == kF16)
-      return H16{__builtin_bit_cast(unsigned short, e)};
-    else
+  __device__ pop_t<ST> pop_abs(int q, int b) const { return elem_pop(at_abs(q, b)); }
+  // register form of the element at p (FP16: its bits zero-extended by
+  // the load itself)
+  __device__ static pop_t<ST> elem_pop(const LS* p) {
+    if constexpr (ST == kF16) {
+      typedef __attribute__((address_space(3))) const unsigned short LU;
+      return H16{(uint32_t)*(LU*)p};
+    } else {
+      typename Traits<ST>::S e = *p;
       return Traits<ST>::ld(&e);
+    }
   }
   __device__ pop_t<ST> pop_byte(int q, int boff) const {
     return pop_abs(q, boff & BMASK);
@@ -950,6 +936,15 @@ __device__ inline void pull9(const LdsIO<ST, RCH, RING>& io, pop_t<ST> (&f)[9],
     f[q] = io.pop_byte(q, CY[q] > 0 ? col[q] - rowb
                                     : (CY[q] < 0 ? col[q] + rowb : col[q]));
   asm volatile("" ::: "memory");  // see pull9_rows
+  if constexpr (ST == kF16) {
+    // FP16: the values leave this branch through an (empty) asm, not as
+    // zero-extended loads, so the compiler does not narrow their merge with
+    // pull9_rows' values to 16 bits, which would mask them with 0xffff
+    // before every use on the common path
+    asm volatile("" : "+v"(f[0].v), "+v"(f[1].v), "+v"(f[2].v), "+v"(f[3].v),
+                 "+v"(f[4].v), "+v"(f[5].v), "+v"(f[6].v), "+v"(f[7].v),
+                 "+v"(f[8].v));
+  }
 }
 
 // The same pulls away from the x boundary: one masked ring offset per
@@ -967,7 +962,7 @@ __device__ inline void pull9_rows(const LdsIO<ST, RCH, RING>&,
   typename IO::LS* row[3] = {IO::at(0, b - rowb), IO::at(0, b), IO::at(0, b + rowb)};
 #pragma unroll
   for (int q = 0; q < 9; q++)
-    f[q] = IO::elem_pop(row[1 - CY[q]][q * (RCH + 2) + (1 - CX[q])]);
+    f[q] = IO::elem_pop(&row[1 - CY[q]][q * (RCH + 2) + (1 - CX[q])]);
   // keeps these loads in this branch: merged with the edge waves' pulls
   // they would take a per-direction address register each
   asm volatile("" ::: "memory");
@@ -1055,22 +1050,30 @@ struct PassConst {
 // wave-uniform, so the per-site branches on it are scalar and most waves
 // compute no lane coordinates at all.
 struct WaveGeo {
+  int kind;      // kWavePlain, kWaveXEdge, kWaveGeneric or kWaveNarrow
   int x, y;      // first site
   bool edge;     // holds an x = 0 or x = Lx-1 site
   bool special;  // holds a wall-row site or a site of the square's box
+  int l0, l1;    // Lx >= 64: the lanes of the x = 0 and x = Lx-1 sites
+                 // (kWaveNoLane: none)
 };
 
 __device__ inline WaveGeo wave_of(const InstDev* __restrict__ I, int block) {
   const __attribute__((address_space(4))) uint32_t* t =
       (const __attribute__((address_space(4))) uint32_t*)I->wave_info;
-  const uint32_t w = t[block];
+  const unsigned i = 2u * (unsigned)block;
+  const uint32_t w = t[i], wl = t[i + 1];
   WaveGeo g;
+  g.kind = (int)(wl >> kWaveKindShift);
   g.x = (int)(w & kWaveCoordMask);
   g.edge = (w >> kWaveEdgeBit) & 1u;
   g.y = (int)((w >> 16) & kWaveCoordMask);
   g.special = (w >> kWaveSpecialBit) & 1u;
+  g.l0 = (int)(wl & kWaveNoLane);
+  g.l1 = (int)((wl >> 8) & kWaveNoLane);
   return g;
 }
+
 
 // Lane coordinates inside a wave (edge / special waves only).
 __device__ inline void lane_xy(const Geo& g, const WaveGeo& w, int lane, int& x,
@@ -1096,11 +1099,14 @@ template <int ST>
 __device__ inline void x_adjust(const Geo& g, const WaveGeo& w, int lane,
                                 const PassConst<ST>& pc, int& aL, int& aR) {
   aL = aR = 0;
-  if (w.edge) {
+  if (w.kind == kWaveNarrow) {
     int x, y;
     lane_xy(g, w, lane, x, y);
     aL = (x == 0) ? pc.adjL : 0;
     aR = (x == g.Lx - 1) ? pc.adjR : 0;
+  } else if (w.kind != kWavePlain) {
+    aL = (lane == w.l0) ? pc.adjL : 0;
+    aR = (lane == w.l1) ? pc.adjR : 0;
   }
 }
 
@@ -1136,16 +1142,59 @@ __device__ inline void pull_site(const Geo& g, const WaveGeo& w, int lane,
   for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::pop(v[q]);
 }
 
+// Makes a reloaded FP16 population opaque to the compiler (no instruction):
+// merged as a zero-extended load with the row pulls' values it would be
+// narrowed to 16 bits and masked before every use.
+template <class P>
+__device__ inline void opaque_pop(P&) {}
+__device__ inline void opaque_pop(H16& v) { asm volatile("" : "+v"(v.v)); }
+
+// The x-boundary lanes of a wave pulled by rows (Lx >= 64, at most one lane
+// of each kind): the x = 0 lane reloads its cx = +1 directions from column
+// adjL - 1, the x = Lx-1 lane its cx = -1 directions from column
+// Lx + adjR (the periodic wrap in REF mode, the lane's own column, i.e. the
+// zero-gradient ghost, in channel mode).
+template <int ST, int RCH, int RING>
+__device__ inline void pull_edge_lanes(const LdsIO<ST, RCH, RING>& io,
+                                       pop_t<ST> (&f)[9], int site,
+                                       const WaveGeo& w, int lane,
+                                       const PassConst<ST>& pc) {
+  constexpr int ES = LdsIO<ST, RCH, RING>::ES;
+  if (lane == w.l0) {
+    const int b = (site - 1 + pc.adjL) * ES;
+    pop_t<ST> v1 = io.pop_byte(1, b), v5 = io.pop_byte(5, b - pc.rowb),
+              v8 = io.pop_byte(8, b + pc.rowb);
+    opaque_pop(v1);
+    opaque_pop(v5);
+    opaque_pop(v8);
+    f[1] = v1;
+    f[5] = v5;
+    f[8] = v8;
+  }
+  if (lane == w.l1) {
+    const int b = (site + 1 + pc.adjR) * ES;
+    pop_t<ST> v3 = io.pop_byte(3, b), v6 = io.pop_byte(6, b - pc.rowb),
+              v7 = io.pop_byte(7, b + pc.rowb);
+    opaque_pop(v3);
+    opaque_pop(v6);
+    opaque_pop(v7);
+    f[3] = v3;
+    f[6] = v6;
+    f[7] = v7;
+  }
+}
+
 template <int ST, int RCH, int RING>
 __device__ inline void pull_site(const Geo& g, const WaveGeo& w, int lane,
                                  int site, const LdsIO<ST, RCH, RING>& src,
                                  const PassConst<ST>& pc, pop_t<ST> (&f)[9]) {
-  if (w.edge) {
+  if (w.kind == kWaveNarrow) {
     int aL, aR;
     x_adjust(g, w, lane, pc, aL, aR);
     pull9(src, f, site, aL, aR, pc.rowb);
   } else {
     pull9_rows(src, f, site, pc.rowb);
+    if (w.kind != kWavePlain) pull_edge_lanes(src, f, site, w, lane, pc);
   }
 }
 
@@ -1212,20 +1261,141 @@ __device__ inline void collide_store(P (&f)[9], bool solid, const DST& dst,
   put9<ST, NT>(dst, o, site, pc);
 }
 
-template <int MODE, int ST, bool NT, class SRC, class DST, class DV>
+// x or y at bit-select precision: mk all ones picks a, zero picks b (one
+// v_bfi_b32 on the 32-bit register, for floats and f16 in place alike).
+__device__ inline float pick(uint32_t mk, float a, float b) {
+  return __uint_as_float((__float_as_uint(a) & mk) | (__float_as_uint(b) & ~mk));
+}
+__device__ inline H16 pick(uint32_t mk, H16 a, H16 b) {
+  return H16{(a.v & mk) | (b.v & ~mk)};
+}
+
+// Bounce-back directions of the square's ring for a site at (dx, dy) from
+// the box's lower-left corner (the writes of computeSquareBounceBack_TEST,
+// src/lbm_src/square.cpp:24-50, as square_mask), without branches: a
+// 16-bit field of a 64-bit table per column class (left edge, right edge,
+// strictly inside), selected by the row class (bottom, top, strictly
+// inside) of the site.
+__device__ inline int square_mask_sel(int dx, int dy, int sx, int sy) {
+  constexpr uint64_t kLeft = (0x80ull << 16) | (0x40ull << 32) | (0xC8ull << 48);
+  constexpr uint64_t kRight = (0x100ull << 16) | (0x20ull << 32) | (0x122ull << 48);
+  constexpr uint64_t kIn = (0x190ull << 16) | (0x64ull << 32);
+  const bool ix = (unsigned)(dx - 1) < (unsigned)(sx - 1);
+  const bool iy = (unsigned)(dy - 1) < (unsigned)(sy - 1);
+  const uint64_t t = dx == 0 ? kLeft : (dx == sx ? kRight : (ix ? kIn : 0ull));
+  const int sh = dy == 0 ? 16 : (dy == sy ? 32 : (iy ? 48 : 0));
+  return (int)((t >> sh) & 0x1FFu);
+}
+
+// The boundary lanes of a channel wave on shifted storage (wall rows, the
+// square's box, x boundaries; Lx >= 64), computed for every lane with
+// selects instead of per-lane branches: bounce-back from the site's own
+// populations `own` (src/lbm_src/domain_noSlipWalls.cpp:10-21,
+// src/lbm_src/square.cpp:24-50), the ring nodes' drag terms (momentum
+// exchange, FP32 in direction order, as ring_drag), the inlet and outlet
+// folded into the collision, the solid interior stored at rest.  The same
+// results as the general path, bit for bit.
+template <int MODE, int ST, bool NT, class SRC, class DST, class DV, class P>
+__device__ inline void channel_boundary_lanes(
+    const Geo& g, const WaveGeo& w, int lane, int site, const P (&f)[9],
+    const P (&own)[9], const DST& dst, const PassConst<ST>& pc,
+    double* __restrict__ ring, const DV& dv, double beta, double tau,
+    const InstDev* __restrict__ I, int PH) {
+  int x = w.x + lane, y = w.y;
+  if (x >= g.Lx) {  // the block's second row
+    x -= g.Lx;
+    y += 1;
+  }
+  const int dx = x - g.x0, dy = y - g.y0;
+  const int sm = square_mask_sel(dx, dy, g.sx, g.sy);
+  const int m = sm | wall_mask(y, g.Ly);
+  const bool solid = (unsigned)(dx - 1) < (unsigned)(g.sx - 1) &&
+                     (unsigned)(dy - 1) < (unsigned)(g.sy - 1);
+  P fb[9];
+  fb[0] = f[0];
+#pragma unroll
+  for (int q = 1; q < 9; q++)
+    fb[q] = pick((uint32_t)__builtin_amdgcn_sbfe(m, q, 1), own[OPP[q]], f[q]);
+  if (sm != 0) {
+    // ring node: F = sum over its bounced directions of -2 cx f_q
+    float tf = 0.0f;
+#pragma unroll
+    for (int q = 1; q < 9; q++) {
+      if (CX[q] == 0) continue;  // their terms are zeros
+      const uint32_t mk = (uint32_t)__builtin_amdgcn_sbfe(sm, q, 1);
+      tf += __uint_as_float(__float_as_uint(pmul((float)(-2 * CX[q]), fb[q])) & mk);
+    }
+    const int slot =
+        dx == 0 ? dy
+                : (dx == g.sx ? (g.sy + 1) + dy
+                              : (dy == g.sy ? 2 * (g.sy + 1) + dx
+                                            : 2 * (g.sy + 1) + (g.sx + 1) + dx));
+    ring[slot] = (double)tf;
+  }
+  if (w.l0 == (int)kWaveNoLane && w.l1 == (int)kWaveNoLane) {
+    collide_store<MODE, ST, NT, SRC>(fb, solid, dst, pc, dv, beta, tau, site,
+                                     g.plane, PH);
+  } else {
+    const LaneBC bc{lane == w.l0 ? 1 : (lane == w.l1 ? 2 : 0),
+                    (float)I->u_in, (float)I->inlet_k};
+    collide_store<MODE, ST, NT, SRC, true>(fb, solid, dst, pc, dv, beta, tau,
+                                           site, g.plane, PH, bc);
+  }
+}
+
+// OP: the site's own populations were loaded ahead into own_pre (the HBM
+// pass of a sweep); otherwise the boundary lanes load them from src.
+template <int MODE, int ST, bool NT, bool OP = false, class SRC, class DST,
+          class DV>
 __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
                                    int site, pop_t<ST> (&f)[9],
                                    const SRC& src, const DST& dst,
                                    const PassConst<ST>& pc,
                                    double* __restrict__ ring, const DV& dv,
                                    double beta, double tau,
-                                   const InstDev* __restrict__ I, int PH) {
+                                   const InstDev* __restrict__ I, int PH,
+                                   const pop_t<ST>* own_pre = nullptr) {
   using T = Traits<ST>;
   using C = typename T::C;
+  if (w.kind == kWavePlain) {
+    // 64 lattice sites with no boundary lane: the common path
+    collide_store<MODE, ST, NT, SRC>(f, false, dst, pc, dv, beta, tau, site,
+                                     g.plane, PH);
+    return;
+  }
+  if (site >= g.plane) return;  // a lane past the lattice's last site
   // channel x boundary folded into the collision (shifted storages)
   constexpr bool kFold = MODE == kModeChannel && T::shifted;
   bool solid = false;
-  if (PH != kCollide && (w.edge || w.special)) {
+  if constexpr (kFold) {
+    // waves whose only boundary lanes are their (at most two) x-boundary
+    // lanes: the inlet or outlet rule folded into the collision of those
+    // lanes, no lane coordinates
+    if (PH != kCollide && w.kind == kWaveXEdge) {
+      const LaneBC bc{lane == w.l0 ? 1 : (lane == w.l1 ? 2 : 0),
+                      (float)I->u_in, (float)I->inlet_k};
+      collide_store<MODE, ST, NT, SRC, true>(f, false, dst, pc, dv, beta, tau,
+                                             site, g.plane, PH, bc);
+      return;
+    }
+    if (PH != kCollide && w.kind == kWaveGeneric) {
+      pop_t<ST> own[9];
+      if constexpr (OP) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) own[q] = own_pre[q];
+      } else {
+        own_pop(src, own, site, pc);
+      }
+      channel_boundary_lanes<MODE, ST, NT, SRC>(g, w, lane, site, f, own, dst,
+                                                pc, ring, dv, beta, tau, I, PH);
+      return;
+    }
+  }
+  // REF-mode x-boundary lanes need nothing beyond their pulls (the periodic
+  // wrap), so wide waves with only those take the common path
+  const bool edge_rule =
+      w.edge && (MODE == kModeChannel || w.kind == kWaveNarrow);
+  if (PH != kCollide && (edge_rule || w.special)) {
     C fc[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) fc[q] = pop_c<ST>(f[q]);
@@ -1294,6 +1464,8 @@ __device__ inline void site_pass(const Geo& g, int tid, int nthr,
   int site = tid;
   for (int c = 0; c < nch; c++) {
     pc.renew();
+    // (site_finish skips lanes past the lattice; the guard here keeps the
+    // body force's map read inside the clone's map)
     if (site < g.plane)
       site_step<MODE, ST, NT>(g, wave_of(I, c * bpc + wblk), lane, site, src,
                               dst, pc, ring, dv, dv.at(site), tau, I, PH);
@@ -1373,6 +1545,23 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
   for (int u = 0; u < PF; u++)
     pull_issue(g, wave_of(I, min(u, nch - 1) * bpc + wblk), lane,
                min(u, nch - 1) * CH + tid, src, pc, pre[u]);
+  // Channel boundary blocks on shifted storage take bounce-back values from
+  // the sites' own populations; for the HBM pass they are loaded one chunk
+  // step ahead (issued before that step's pulls, so waiting for them leaves
+  // the newest chunk's pulls in flight).
+  constexpr bool kOwnAhead = MODE == kModeChannel && Traits<ST>::shifted;
+  R ownr[9];
+  auto own_issue = [&](int cc) {
+    if constexpr (kOwnAhead) {
+      if (wave_of(I, cc * bpc + wblk).kind == kWaveGeneric) {
+        constexpr int ES = HbmIO<ST>::ES;
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+          ownr[q] = BufIO<ST>::raw(src.r, (cc * CH + tid) * ES, pc.sown(q));
+      }
+    }
+  };
+  own_issue(0);
   // the body force of the site a pass handles in chunk cc.  With field maps
   // in FP64 or FP32 storage the map load is issued at the top of a step,
   // ahead of the pulls and LDS barriers its collision waits behind; the
@@ -1388,9 +1577,13 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
       return (cc >= 0 && cc < nch && s < g.plane) ? dv.at(s) : 0.0;
     }
   };
+  // (lanes past the lattice, which site_finish skips, read the map at the
+  // last site instead of past the clone's map)
   auto beta_of = [&](double early, int s) {
     if constexpr (kEarly)
       return early;
+    else if constexpr (DV::kMap)
+      return dv.at(min(s, g.plane - 1));
     else
       return dv.at(s);
   };
@@ -1404,14 +1597,19 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
       pop_t<ST> f[9];
 #pragma unroll
       for (int q = 0; q < 9; q++) f[q] = BufIO<ST>::pop(pre[q]);
-      if (c < nch && s < g.plane)
-        site_finish<MODE, ST, false>(g, wave_of(I, c * bpc + wblk), lane, s, f,
-                                     src, r0, pc, drag, dv, beta_of(b0, s), tau, I,
-                                     kBoth);
+      if ((unsigned)c < (unsigned)nch) {
+        pop_t<ST> own[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) own[q] = BufIO<ST>::pop(ownr[q]);
+        site_finish<MODE, ST, false, kOwnAhead>(
+            g, wave_of(I, c * bpc + wblk), lane, s, f, src, r0, pc, drag, dv,
+            beta_of(b0, s), tau, I, kBoth, own);
+      }
       // then chunk c + PF's loads into the same registers, once their values
       // are consumed (loads cannot move above a memory clobber), so no
-      // register copies are needed
+      // register copies are needed; the next chunk's own populations first
       asm volatile("" ::: "memory");
+      if ((unsigned)(c + 1) < (unsigned)nch) own_issue(c + 1);
       const int cp = min(c + PF, nch - 1);
       pull_issue(g, wave_of(I, cp * bpc + wblk), lane, cp * CH + tid, src, pc,
                  pre);
@@ -1419,36 +1617,34 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
     if constexpr (NP == 2) lds_barrier();
     if constexpr (NP == 3) {
       const int cm = c - D;
-      if (cm >= 0 && cm < nch) {
+      if ((unsigned)cm < (unsigned)nch) {
         const int s = cm * CH + tid;
-        if (s < g.plane)
-          site_step<MODE, ST, false>(g, wave_of(I, cm * bpc + wblk), lane, s, r0,
-                                     r1, pc, drag + ring_max, dv, beta_of(b1, s),
-                                     tau, I,
-                                     kBoth);
+        site_step<MODE, ST, false>(g, wave_of(I, cm * bpc + wblk), lane, s, r0,
+                                   r1, pc, drag + ring_max, dv, beta_of(b1, s),
+                                   tau, I, kBoth);
       }
     }
     const int cz = c - (NP - 1) * D;
-    if (cz >= 0 && cz < nch) {
+    if ((unsigned)cz < (unsigned)nch) {
       const int s = cz * CH + tid;
-      if (s < g.plane) {
-        double* dz = drag + (NP - 1) * ring_max;
-        if constexpr (NP == 3)
-          site_step<MODE, ST, NT>(g, wave_of(I, cz * bpc + wblk), lane, s, r1,
-                                  dst, pc, dz, dv, beta_of(b2, s), tau, I,
-                                  kBoth);
-        else
-          site_step<MODE, ST, NT>(g, wave_of(I, cz * bpc + wblk), lane, s, r0,
-                                  dst, pc, dz, dv, beta_of(b2, s), tau, I,
-                                  kBoth);
-      }
+      double* dz = drag + (NP - 1) * ring_max;
+      if constexpr (NP == 3)
+        site_step<MODE, ST, NT>(g, wave_of(I, cz * bpc + wblk), lane, s, r1,
+                                dst, pc, dz, dv, beta_of(b2, s), tau, I, kBoth);
+      else
+        site_step<MODE, ST, NT>(g, wave_of(I, cz * bpc + wblk), lane, s, r0,
+                                dst, pc, dz, dv, beta_of(b2, s), tau, I, kBoth);
     }
     if constexpr (NP == 3) lds_barrier();
   };
-  // a multiple of PF steps; steps past the last do nothing
+  // a multiple of PF steps; steps past the last do nothing.  The PF steps
+  // are written out: a loop over them, which the compiler may leave rolled
+  // once the body is large, would index the register sets indirectly.
+  static_assert(PF == 2 || PF == 3, "two or three register sets");
   for (int c = 0; c < total; c += PF) {
-#pragma unroll
-    for (int u = 0; u < PF; u++) step(c + u, pre[u]);
+    step(c, pre[0]);
+    step(c + 1, pre[1]);
+    if constexpr (PF == 3) step(c + 2, pre[2]);
   }
 }
 
