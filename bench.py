@@ -237,6 +237,27 @@ def time_storage(storage: int, K: int, W: int, specs: list, mode: int,
                 s.gather()
                 gms, gbytes = s.gather_kernel_ms()
                 runs.append((gms, gbytes, copies))
+        # the same copies collided into the next interval's pass 0 (channel
+        # noise, as lbc_advance does between intervals): alternating
+        # selections without copies (equal weights) and copying every other
+        # clone, each followed by gather_fused and an interval, both kernels
+        # timed with events; the copy cost is the difference of their sums
+        fused = None
+        if mode == lbc.MODE_CHANNEL and all(sp.noise_sigma > 0
+                                            for sp in specs):
+            rec = {False: [], True: []}
+            for half in (False, True) * G:
+                copies = 0
+                for i, sp in enumerate(specs):
+                    w = (np.where(np.arange(sp.n_clones) % 2 == 0, 1.0, 1e-9)
+                         if half else np.ones(sp.n_clones))
+                    copies += len(s.select(i, weights=w).pairs)
+                s.gather_fused()
+                gms, _ = s.gather_kernel_ms()
+                s.run_interval()
+                sms, _ = s.step_kernel_ms()
+                rec[half].append((gms, sms, copies))
+            fused = fused_copy_stats(rec)
     secs = t1 - t0
     lup1 = sum(sp.n_clones * sp.sites * sp.steps for sp in specs)
     lups = lup1 * K
@@ -278,7 +299,25 @@ def time_storage(storage: int, K: int, W: int, specs: list, mode: int,
         logR_last=float(math.log(R[-1])) if len(R) else None,
         gather_workload=gather_stats(gather),
         gather=gather_stats(gather_half),
+        gather_fused=fused,
     )
+
+
+def fused_copy_stats(rec: dict) -> dict:
+    """Copies collided into pass 0 (lbc_gather_fused): mean copy-kernel and
+    step-kernel ms without copies and with them, and what the copies add to
+    the two together, also scaled to 2048 copies."""
+    def mean(v, k):
+        return sum(r[k] for r in v) / len(v)
+    none, half = rec[False], rec[True]
+    n_copies = mean(half, 2)
+    extra = (mean(half, 0) + mean(half, 1)) - (mean(none, 0) + mean(none, 1))
+    return dict(
+        copies=int(round(n_copies)), repeats=len(half),
+        gather_ms_no_copies=mean(none, 0), gather_ms=mean(half, 0),
+        step_ms_no_copies=mean(none, 1), step_ms=mean(half, 1),
+        copy_cost_ms=extra,
+        copy_cost_ms_per_2048=extra * 2048.0 / n_copies if n_copies else None)
 
 
 def gather_stats(runs: list) -> dict | None:
@@ -787,6 +826,7 @@ def main() -> int:
         "config": config,
         "roofline": roofline(head, wl, live),
         "gather_roofline": head.get("gather"),
+        "gather_fused": head.get("gather_fused"),
         "cpu_baseline": cpu,
         "paths": runs,
         "accuracy_vs_fp64": acc,

@@ -71,8 +71,10 @@ struct lbc_ctx {
   std::string err;
   std::vector<hipEvent_t> ev;         // pairs around step-kernel launches
   int ev_used = 0;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};  // step kernel | the rest
-  hipGraph_t graphs[2] = {nullptr, nullptr};
+  // step kernel | the rest | the rest with the copies collided into the
+  // next interval's pass 0
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  hipGraph_t graphs[3] = {nullptr, nullptr, nullptr};
   bool timing_from_graph = false;
   std::vector<float> graph_ms;
   // forcing mode, per instance
@@ -85,6 +87,9 @@ struct lbc_ctx {
   hipEvent_t gev[2] = {nullptr, nullptr};
   bool gather_timed = false;
   double gather_bytes = 0.0;
+  // lbc_gather_fused left copies collided into the next interval's pass 0:
+  // the states are not readable until an interval has run
+  bool pass0_pending = false;
 };
 
 namespace {
@@ -410,6 +415,16 @@ int check_inst(lbc_ctx* c, int inst) {
   return LBC_OK;
 }
 
+// Calls that read, replace or reselect the states refuse while copies wait
+// in the next interval's pass 0 (lbc_gather_fused).
+int check_no_pending(lbc_ctx* c) {
+  if (c && c->pass0_pending)
+    return fail(c, LBC_E_STATE,
+                "the copies of lbc_gather_fused are pending in the next "
+                "interval's pass 0: run an interval first");
+  return LBC_OK;
+}
+
 // Buffer flips of one interval: one per HBM sweep (one per pass with one
 // pass per sweep).
 int interval_flip(const lbc_ctx* c, int i) {
@@ -561,7 +576,7 @@ void lbc_destroy(lbc_ctx* c) {
   if (!c) return;
   ON_CTX_DEVICE(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (int i = 0; i < 2; i++) {
+  for (int i = 0; i < 3; i++) {
     if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
     if (c->graphs[i]) (void)hipGraphDestroy(c->graphs[i]);
   }
@@ -774,6 +789,7 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
 int lbc_load_states(lbc_ctx* c, int inst, int clone0, int n, const double* aos) {
   ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
+  if (int rc = check_no_pending(c)) return rc;
   const lbc_instance& s = c->inst[inst];
   if (!aos || n < 1 || clone0 < 0 || clone0 + n > s.n_clones)
     return fail(c, LBC_E_ARG, "clone range out of bounds");
@@ -790,6 +806,7 @@ static int store_which(lbc_ctx* c, int inst, int which, int clone0, int n,
                        double* aos) {
   ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
+  if (int rc = check_no_pending(c)) return rc;
   const lbc_instance& s = c->inst[inst];
   if (!aos || n < 1 || clone0 < 0 || clone0 + n > s.n_clones)
     return fail(c, LBC_E_ARG, "clone range out of bounds");
@@ -809,6 +826,7 @@ int lbc_store_states(lbc_ctx* c, int inst, int clone0, int n, double* aos) {
 int lbc_init_equilibrium(lbc_ctx* c, int inst, double amp) {
   ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
+  if (int rc = check_no_pending(c)) return rc;
   if (!std::isfinite(amp) || amp < 0) return fail(c, LBC_E_ARG, "bad amplitude");
   lbc::Launch L = make_launch(c);
   HIPC(c, lbc::launch_init_eq(L, inst, c->inst[inst].n_clones, amp, seed_lo(c),
@@ -875,6 +893,7 @@ int lbc_run_interval(lbc_ctx* c, const double* prefactors) {
   if (int rc = enqueue_perturb(c, philox)) return rc;
   if (int rc = enqueue_fields(c)) return rc;
   if (int rc = enqueue_interval(c, true)) return rc;
+  c->pass0_pending = false;
   return LBC_OK;
 }
 
@@ -907,6 +926,7 @@ int lbc_select(lbc_ctx* c, int inst, const double* weights, const double* u,
                const int32_t* rint, lbc_selection* out) {
   ON_CTX_DEVICE(c);
   if (int rc = check_inst(c, inst)) return rc;
+  if (int rc = check_no_pending(c)) return rc;
   const InstDev& d = c->idh[inst];
   const size_t n = (size_t)d.n_clones;
   if ((u == nullptr) != (rint == nullptr))
@@ -961,9 +981,10 @@ int lbc_select(lbc_ctx* c, int inst, const double* weights, const double* u,
   return LBC_OK;
 }
 
-int lbc_gather(lbc_ctx* c) {
+static int gather_impl(lbc_ctx* c, bool fuse0) {
   ON_CTX_DEVICE(c);
   if (!c) return fail(nullptr, LBC_E_ARG, "null context");
+  if (int rc = check_no_pending(c)) return rc;
   for (int i = 0; i < c->cfg.n_instances; i++)
     if (!c->selected[i])
       return fail(c, LBC_E_STATE, "lbc_gather: every instance needs lbc_select first");
@@ -982,7 +1003,7 @@ int lbc_gather(lbc_ctx* c) {
   }
   lbc::Launch L = make_launch(c);
   HIPC(c, hipEventRecord(c->gev[0], c->stream));
-  HIPC(c, lbc::launch_gather(L, seed_lo(c), seed_hi(c)));
+  HIPC(c, lbc::launch_gather(L, seed_lo(c), seed_hi(c), fuse0 ? 1 : 0));
   HIPC(c, hipEventRecord(c->gev[1], c->stream));
   // every copy reads the parent's populations and writes the destination's
   double bytes = 0.0;
@@ -992,19 +1013,27 @@ int lbc_gather(lbc_ctx* c) {
   c->gather_bytes = bytes;
   c->gather_timed = true;
   for (int i = 0; i < c->cfg.n_instances; i++) c->selected[i] = 0;
+  if (fuse0 && c->cfg.mode == LBC_MODE_CHANNEL) c->pass0_pending = true;
   HIPC(c, hipStreamSynchronize(c->stream));
   return LBC_OK;
 }
 
+int lbc_gather(lbc_ctx* c) { return gather_impl(c, false); }
+
+int lbc_gather_fused(lbc_ctx* c) { return gather_impl(c, true); }
+
 namespace {
 
-// Capture one cloning step of every instance as two graphs: [step kernel]
-// and [parity flip, selection, gather, perturbation of the new copies
-// (REF)].  Ordinary events recorded between the two replays time the step
-// kernel alone.
+// Capture one cloning step of every instance as graphs: [step kernel] and
+// [parity flip, selection, gather, perturbation of the new copies (REF)],
+// the latter twice: with the copies stored (graph 1) and, for the channel
+// mode's noisy copies, with them collided into the next interval's pass 0
+// (graph 2; k_gather's fuse0), which only a following step kernel may
+// replay after.  Ordinary events recorded between the replays time the
+// step kernel alone.
 int capture_step(lbc_ctx* c) {
   lbc::Launch L = make_launch(c);
-  for (int part = 0; part < 2; part++) {
+  for (int part = 0; part < 3; part++) {
     HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     bool ok = true;
     if (part == 0) {
@@ -1015,7 +1044,9 @@ int capture_step(lbc_ctx* c) {
         ok = lbc::launch_select(c->idd, c->counters, 0, c->cfg.n_instances,
                                 c->n_max, 0, 0, seed_lo(c), seed_hi(c),
                                 c->stream) == hipSuccess;
-      if (ok) ok = lbc::launch_gather(L, seed_lo(c), seed_hi(c)) == hipSuccess;
+      if (ok)
+        ok = lbc::launch_gather(L, seed_lo(c), seed_hi(c), part == 2) ==
+             hipSuccess;
       bool snapshots = false;
       for (const auto& s : c->inst) snapshots |= s.n_snapshots > 0;
       if (ok && c->cfg.mode == LBC_MODE_REF && snapshots)
@@ -1044,7 +1075,9 @@ int lbc_advance(lbc_ctx* c, int n_intervals) {
   ON_CTX_DEVICE(c);
   if (!c) return fail(nullptr, LBC_E_ARG, "null context");
   if (n_intervals < 0) return fail(c, LBC_E_ARG, "negative interval count");
-  if (!c->gexec[0] || !c->gexec[1])
+  if (c->pass0_pending && n_intervals == 0)
+    return check_no_pending(c);
+  if (!c->gexec[0] || !c->gexec[1] || !c->gexec[2])
     if (int rc = capture_step(c)) return rc;
   c->timing_from_graph = false;
   c->ev_used = 0;
@@ -1063,10 +1096,13 @@ int lbc_advance(lbc_ctx* c, int n_intervals) {
     HIPC(c, hipEventRecord(c->ev[c->ev_used], c->stream));
     HIPC(c, hipGraphLaunch(c->gexec[0], c->stream));
     HIPC(c, hipEventRecord(c->ev[c->ev_used + 1], c->stream));
-    HIPC(c, hipGraphLaunch(c->gexec[1], c->stream));
+    // the last interval's copies are stored, so the states are whole when
+    // the call returns; the others go into the next interval's pass 0
+    HIPC(c, hipGraphLaunch(c->gexec[k + 1 < n_intervals ? 2 : 1], c->stream));
     c->ev_used += 2;
     for (int i = 0; i < c->cfg.n_instances; i++)
       c->parity[i] ^= interval_flip(c, i);
+    c->pass0_pending = false;
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   return LBC_OK;

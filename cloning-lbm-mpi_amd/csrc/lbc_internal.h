@@ -260,7 +260,10 @@ hipError_t launch_select(const InstDev* insts, Counters* counters, int inst0,
                          int n_inst, int n_max, int weights_from_host, int draws_from_host,
                          uint32_t seed_lo, uint32_t seed_hi,
                          hipStream_t stream);
-hipError_t launch_gather(const Launch& L, uint32_t seed_lo, uint32_t seed_hi);
+// fuse0: channel copies with noise are collided into the next interval's
+// pass 0 (k_gather).
+hipError_t launch_gather(const Launch& L, uint32_t seed_lo, uint32_t seed_hi,
+                         int fuse0 = 0);
 hipError_t launch_init_eq(const Launch& L, int inst, int n_clones, double amp,
                           uint32_t seed_lo, uint32_t seed_hi);
 hipError_t launch_aos_to_soa(int storage, const InstDev* inst_dev,

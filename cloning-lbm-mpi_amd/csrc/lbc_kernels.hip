@@ -298,12 +298,24 @@ __global__ void __launch_bounds__(1024) k_select_static(InstDev I) {
 // by site, adding Gaussian noise to every population outside the solid
 // interior; otherwise as a plain block copy.  The interval counter has
 // already been advanced by the selection.
+//
+// fuse0 (channel mode with noise on shifted storage, no macroscopic
+// output): the copy goes no further than the next interval's collide-only
+// pass 0.  The slot's noisy state, rounded to storage as the plain copy
+// stores it, is collided here and written where pass 0 writes
+// (buf[parity ^ 1]); the slot's label
+// becomes -1 - parent, which tells the step kernel that pass 0 is done (and
+// which it turns back into the parent).  The parents' buf[parity] stays
+// unwritten until the step kernel, which runs after this launch, so no
+// copy reads a state being overwritten.  Results are those of the plain
+// copy followed by pass 0, bit for bit; the copy's own read and write of
+// the slot's populations are gone.
 template <int MODE, int ST>
 __global__ void __launch_bounds__(256)
     k_gather(const InstDev* __restrict__ insts,
              const WorkItem* __restrict__ items,
              const Counters* __restrict__ counters, uint32_t seed_lo,
-             uint32_t seed_hi) {
+             uint32_t seed_hi, int fuse0) {
   using T = Traits<ST>;
   using S = typename T::S;
   const WorkItem wi = items[blockIdx.x];
@@ -317,6 +329,19 @@ __global__ void __launch_bounds__(256)
   const int Lx = I->Lx, plane = I->plane;
   const double sigma = I->sigma;
   const bool noisy = (MODE == kModeChannel) && sigma > 0.0;
+  // (FP64 stores its copies: its noise, on libm, makes the fused copy longer
+  // than the pass 0 it saves)
+  const bool fused = ST != kF64 && noisy && fuse0 && I->macro_every <= 0;
+  // pass 0 of the next interval: the slot's post-collision populations
+  Geo g{};
+  g.Lx = Lx;
+  g.plane = plane;
+  const PassConst<ST> pc(g, MODE);
+  const HbmIO<ST> out0{
+      clone_rsrc((S*)I->buf[par ^ 1] + (long long)wi.clone * I->clone_stride,
+                 (int)(I->clone_stride * (long long)sizeof(S))),
+      plane};
+  const DriveOf<false, ST> drive{I->beta};
   if (!noisy) {
     // a plain copy of the whole clone block, padding included: clone_stride
     // is a multiple of 128 elements, so the block is a whole number of
@@ -349,34 +374,43 @@ __global__ void __launch_bounds__(256)
     for (int q = 0; q < 9; q++) f[q] = T::ld(src + q * plane + s);
     const bool interior = x > I->xmin && x < I->xmax && y > I->ymin && y < I->ymax;
     if (!interior) {
+      const U4 r = philox((uint32_t)s, (uint32_t)wi.clone, (uint32_t)interval,
+                          (uint32_t)wi.inst, seed_lo, seed_hi);
+      if constexpr (!T::shifted) {
+        double nrm[10];
+        site_normals_d(r, nrm);
 #pragma unroll
-      for (int j = 0; j < 3; j++) {
-        const U4 r = philox(3u * (uint32_t)s + (uint32_t)j, (uint32_t)wi.clone,
-                            (uint32_t)interval, (uint32_t)wi.inst, seed_lo,
-                            seed_hi);
-        if (!T::shifted) {
-          double nrm[4];
-          box_muller_d(r, nrm);
+        for (int q = 0; q < 9; q++) f[q] += sigma * nrm[q];
+      } else {
+        float nrm[10];
+        site_normals_f(r, nrm);
 #pragma unroll
-          for (int i = 0; i < 4; i++)
-            if (4 * j + i < 9) f[4 * j + i] += sigma * nrm[i];
-        } else {
-          float nrm[4];
-          box_muller_f(r, nrm);
-#pragma unroll
-          for (int i = 0; i < 4; i++)
-            if (4 * j + i < 9) f[4 * j + i] += (float)sigma * nrm[i];
-        }
+        for (int q = 0; q < 9; q++) f[q] += (float)sigma * nrm[q];
       }
     }
+    if (fused) {
+      // the stored copy's elements, in the register form pass 0 loads
+      pop_t<ST> p[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) T::st(dst + q * plane + s, f[q]);
+      for (int q = 0; q < 9; q++)
+        p[q] = elem_pop<ST>(T::pack(f[q]));
+      collide_store<MODE, ST, false, HbmIO<ST>>(p, false, out0, pc, drive,
+                                                I->beta, I->tau, s, plane,
+                                                kCollide);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; q++) T::st(dst + q * plane + s, f[q]);
+    }
     x += dx;
     y += dy;
     if (x >= Lx) {
       x -= Lx;
       y += 1;
     }
+  }
+  if (fused) {
+    __syncthreads();  // every thread has read the label
+    if (threadIdx.x == 0) I->labels[wi.clone] = -1 - parent;
   }
 }
 
@@ -579,19 +613,21 @@ __global__ void k_snap_to_soa(int n, const double* __restrict__ aos, int Lx,
 
 
 template <int MODE, int ST>
-hipError_t launch_gather_t(const Launch& L, uint32_t lo, uint32_t hi) {
+hipError_t launch_gather_t(const Launch& L, uint32_t lo, uint32_t hi,
+                           int fuse0) {
   hipLaunchKernelGGL((k_gather<MODE, ST>), dim3(L.n_items), dim3(256), 0,
-                     L.stream, L.insts, L.items, L.counters, lo, hi);
+                     L.stream, L.insts, L.items, L.counters, lo, hi, fuse0);
   return hipGetLastError();
 }
 
 template <int MODE>
-hipError_t launch_gather_m(const Launch& L, uint32_t lo, uint32_t hi) {
+hipError_t launch_gather_m(const Launch& L, uint32_t lo, uint32_t hi,
+                           int fuse0) {
   switch (L.storage) {
-    case kF64: return launch_gather_t<MODE, kF64>(L, lo, hi);
-    case kF32: return launch_gather_t<MODE, kF32>(L, lo, hi);
-    case kF16: return launch_gather_t<MODE, kF16>(L, lo, hi);
-    case kBF16: return launch_gather_t<MODE, kBF16>(L, lo, hi);
+    case kF64: return launch_gather_t<MODE, kF64>(L, lo, hi, fuse0);
+    case kF32: return launch_gather_t<MODE, kF32>(L, lo, hi, fuse0);
+    case kF16: return launch_gather_t<MODE, kF16>(L, lo, hi, fuse0);
+    case kBF16: return launch_gather_t<MODE, kBF16>(L, lo, hi, fuse0);
   }
   return hipErrorInvalidValue;
 }
@@ -725,10 +761,11 @@ hipError_t launch_select_static(const InstDev& I, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_gather(const Launch& L, uint32_t lo, uint32_t hi) {
+hipError_t launch_gather(const Launch& L, uint32_t lo, uint32_t hi,
+                         int fuse0) {
   if (L.n_items <= 0) return hipSuccess;
-  return L.mode == kModeRef ? launch_gather_m<kModeRef>(L, lo, hi)
-                            : launch_gather_m<kModeChannel>(L, lo, hi);
+  return L.mode == kModeRef ? launch_gather_m<kModeRef>(L, lo, hi, 0)
+                            : launch_gather_m<kModeChannel>(L, lo, hi, fuse0);
 }
 
 hipError_t launch_init_eq(const Launch& L, int inst, int n_clones, double amp,

@@ -166,6 +166,15 @@ __device__ inline _Float16 pop_pack(H16 v) {
   return __builtin_bit_cast(_Float16, (unsigned short)v.v);
 }
 
+// Register form of a storage element (FP16: its bits zero-extended).
+template <int ST>
+__device__ inline pop_t<ST> elem_pop(typename Traits<ST>::S e) {
+  if constexpr (ST == kF16)
+    return H16{(uint32_t)__builtin_bit_cast(unsigned short, e)};
+  else
+    return Traits<ST>::ld(&e);
+}
+
 // ---------------------------------------------------------------- Philox
 // Philox4x32-10 (Salmon et al., SC'11).  Each round's two 32x32 -> 64-bit
 // products are one v_mad_u64_u32 each (low and high word together) and
@@ -217,22 +226,45 @@ __device__ inline void box_muller_d(U4 r, double n[4]) {
   }
 }
 
-// FP32 Box-Muller (noise on shifted storage) on gfx950's transcendental
-// instructions: v_log_f32 (base 2), v_sqrt_f32, and v_sin_f32 / v_cos_f32,
-// which take the angle in revolutions, so cos(2 pi u2) is one instruction.
-// Each is good to about 1e-6 relative, far inside the FP32 (or narrower)
-// populations the noise is added to.
-__device__ inline void box_muller_f(U4 r, float n[4]) {
-  const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
+// Copy noise of one site (channel mode; no reference counterpart): nine
+// normals from one Philox4x32-10 draw, as five Box-Muller pairs.  Pair p
+// takes bits [25p, 25p + 25) of the 128-bit draw (words x, y, z, w from the
+// least significant bit): a 13-bit a, u1 = (a + 1) / 2^13 in (0, 1], then a
+// 12-bit b, u2 = b / 2^12 in [0, 1); n[2p] = sqrt(-2 ln u1) cos(2 pi u2) and
+// n[2p + 1] = sqrt(-2 ln u1) sin(2 pi u2).  n[9] is not used.
+__device__ inline uint32_t draw_bits(const U4& r, int off, int len) {
+  const uint32_t w[5] = {r.x, r.y, r.z, r.w, 0u};
+  const int i = off >> 5, sh = off & 31;
+  const uint64_t two = ((uint64_t)w[i + 1] << 32) | w[i];
+  return (uint32_t)(two >> sh) & ((1u << len) - 1u);
+}
+
+__device__ inline void site_normals_d(const U4& r, double n[10]) {
+  const double twopi = 6.283185307179586;
 #pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const float u1 = ((float)(rr[2 * h] >> 8) + 1.0f) * 0x1p-24f;  // (0, 1]
-    const float u2 = (float)(rr[2 * h + 1] >> 8) * 0x1p-24f;       // [0, 1)
+  for (int p = 0; p < 5; p++) {
+    const double u1 = ((double)draw_bits(r, 25 * p, 13) + 1.0) * 0x1p-13;
+    const double u2 = (double)draw_bits(r, 25 * p + 13, 12) * 0x1p-12;
+    const double rad = sqrt(-2.0 * log(u1));
+    const double ang = twopi * u2;
+    n[2 * p] = rad * cos(ang);
+    n[2 * p + 1] = rad * sin(ang);
+  }
+}
+
+// The same normals in FP32 on gfx950's transcendental instructions
+// (v_log_f32 in base 2, v_sqrt_f32, v_cos_f32 / v_sin_f32 taking the angle
+// in revolutions), each good to about 1e-6 relative; u1 and u2 are exact.
+__device__ inline void site_normals_f(const U4& r, float n[10]) {
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    const float u1 = ((float)draw_bits(r, 25 * p, 13) + 1.0f) * 0x1p-13f;
+    const float u2 = (float)draw_bits(r, 25 * p + 13, 12) * 0x1p-12f;
     // -2 ln(u1) = -2 ln(2) log2(u1)
     const float rad =
         __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
-    n[2 * h] = rad * __builtin_amdgcn_cosf(u2);
-    n[2 * h + 1] = rad * __builtin_amdgcn_sinf(u2);
+    n[2 * p] = rad * __builtin_amdgcn_cosf(u2);
+    n[2 * p + 1] = rad * __builtin_amdgcn_sinf(u2);
   }
 }
 
@@ -1741,6 +1773,8 @@ __global__ void __launch_bounds__(1024)
   const HbmIO<ST> ioA{clone_rsrc(k.A, k.clone_bytes), k.g.plane};
   const HbmIO<ST> ioB{clone_rsrc(k.B, k.clone_bytes), k.g.plane};
   const int M1 = FRC ? I->forced_steps : 0;
+  // a copy collided by k_gather (label -1 - parent) has had its pass 0
+  const bool done0 = I->labels[wi.clone] < 0;
   double acc = 0.0;
   for (int n = 0; n <= k.M; ++n) {
     const HbmIO<ST>& src = (n & 1) ? ioB : ioA;
@@ -1749,11 +1783,15 @@ __global__ void __launch_bounds__(1024)
     const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
     const LonePass<DriveOf<FRC, ST>> lp{drive_of<FRC, ST>(I, wi.clone, n < M1),
                                   macro_out(I, wi.clone, n, k.M)};
-    site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, ring, lp, k.tau, I, ph);
+    if (n > 0 || !done0)
+      site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, ring, lp, k.tau, I, ph);
     __syncthreads();
     if (n >= 1 && tid == 0) take_drag<MODE>(k, ring, n, acc);
   }
-  if (tid == 0) finish_clone<MODE>(I, wi, acc);
+  if (tid == 0) {
+    finish_clone<MODE>(I, wi, acc);
+    if (done0) I->labels[wi.clone] = -1 - I->labels[wi.clone];
+  }
 }
 
 // The same interval with NP passes per HBM sweep where it can: the
@@ -1782,6 +1820,9 @@ __device__ inline void interval_fused(const InstDev* __restrict__ insts,
   double* const slots = lds + (NP - 1) * (LdsIO<ST, RCH>::BYTES / 8);
   const int M1 = FRC ? I->forced_steps : 0;
   const int E = I->macro_every;
+  // a copy collided by k_gather (label -1 - parent) has had its pass 0,
+  // which is a sweep of its own
+  const bool done0 = I->labels[wi.clone] < 0;
   double acc = 0.0;
   int sweep = 0;
   for (int n = 0; n <= k.M; sweep++) {
@@ -1800,13 +1841,18 @@ __device__ inline void interval_fused(const InstDev* __restrict__ insts,
     } else {
       const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
       const LonePass<DriveOf<FRC, ST>> lp{dv, macro_out(I, wi.clone, n, k.M)};
-      site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, drag, lp, k.tau, I, ph);
+      if (n > 0 || !done0)
+        site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, drag, lp, k.tau, I,
+                                ph);
       __syncthreads();
       if (n >= 1 && tid == 0) take_drag<MODE>(k, drag, n, acc);
       n += 1;
     }
   }
-  if (tid == 0) finish_clone<MODE>(I, wi, acc);
+  if (tid == 0) {
+    finish_clone<MODE>(I, wi, acc);
+    if (done0) I->labels[wi.clone] = -1 - I->labels[wi.clone];
+  }
 }
 
 // Largest workgroup of the multi-pass kernel in storage ST.  An FP64 ring

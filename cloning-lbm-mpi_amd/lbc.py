@@ -36,7 +36,8 @@ EXPORTS = (
     "lbc_create", "lbc_destroy", "lbc_last_error", "lbc_load_states",
     "lbc_store_states", "lbc_init_equilibrium", "lbc_set_snapshots",
     "lbc_set_marks", "lbc_get_marks", "lbc_run_interval", "lbc_get_drag",
-    "lbc_get_logweights", "lbc_select", "lbc_gather", "lbc_advance",
+    "lbc_get_logweights", "lbc_select", "lbc_gather", "lbc_gather_fused",
+    "lbc_advance",
     "lbc_get_R_history", "lbc_get_scgf", "lbc_set_field_maps",
     "lbc_get_field_maps", "lbc_get_macro", "lbc_sync", "lbc_step_kernel_ms",
     "lbc_gather_kernel_ms", "lbc_threads",
@@ -115,6 +116,7 @@ def lib() -> C.CDLL:
     L.lbc_get_logweights.argtypes = [vp, C.c_int, dp, dp]
     L.lbc_select.argtypes = [vp, C.c_int, dp, dp, ip, C.POINTER(Selection)]
     L.lbc_gather.argtypes = [vp]
+    L.lbc_gather_fused.argtypes = [vp]
     L.lbc_advance.argtypes = [vp, C.c_int]
     L.lbc_get_R_history.argtypes = [vp, C.c_int, dp, C.c_int,
                                     C.POINTER(C.c_int)]
@@ -431,6 +433,12 @@ class Sampler:
 
     def gather(self) -> None:
         self._check(lib().lbc_gather(self._h), "lbc_gather")
+
+    def gather_fused(self) -> None:
+        """The copies collided into the next interval's pass 0 (CHANNEL with
+        noise): no state may be read, replaced or reselected before the next
+        run_interval or advance."""
+        self._check(lib().lbc_gather_fused(self._h), "lbc_gather_fused")
 
     def advance(self, n_intervals: int) -> None:
         self._check(lib().lbc_advance(self._h, n_intervals), "lbc_advance")

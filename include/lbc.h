@@ -199,6 +199,19 @@ int lbc_select(lbc_ctx* ctx, int inst, const double* weights, const double* u,
  * loop of src/mpi_LBM_GK_pops_perturb.cpp:658-701. */
 int lbc_gather(lbc_ctx* ctx);
 
+/* The same copies, in CHANNEL mode with noise on the shifted storages taken
+ * straight into the next interval: each noisy copy is collided into that
+ * interval's collide-only pass 0 instead of being stored, which saves the
+ * copy's own read and write of the destination's populations (FP64, other
+ * modes and instances with macroscopic output copy as lbc_gather).  No counterpart in the reference; for drivers that run the
+ * interval, the selection and the copy one call at a time and read no
+ * states between a copy and the next interval (lbc_advance does the same
+ * internally).  Until the next lbc_run_interval or lbc_advance, the calls
+ * that read, replace or reselect states (lbc_store_states,
+ * lbc_load_states, lbc_init_equilibrium, lbc_select, lbc_gather,
+ * lbc_gather_fused) return LBC_E_STATE. */
+int lbc_gather_fused(lbc_ctx* ctx);
+
 /* n full cloning steps (interval, selection, copy) of every instance with
  * device Philox draws, replayed from one captured HIP graph. */
 int lbc_advance(lbc_ctx* ctx, int n_intervals);

@@ -534,6 +534,27 @@ void orc_box_muller4(const uint32_t r[4], double n[4]) {
   }
 }
 
+/* Bits [off, off + len) of a Philox block read as one 128-bit number, r[0]
+ * the least significant word. */
+static uint32_t draw_bits(const uint32_t r[4], int off, int len) {
+  const int i = off / 32, sh = off % 32;
+  const uint64_t lo = r[i], hi = i < 3 ? r[i + 1] : 0u;
+  const uint64_t two = (hi << 32) | lo;
+  return (uint32_t)(two >> sh) & ((1u << len) - 1u);
+}
+
+void orc_site_normals(const uint32_t r[4], double n[10]) {
+  const double twopi = 6.283185307179586;
+  for (int p = 0; p < 5; p++) {
+    double u1 = ((double)draw_bits(r, 25 * p, 13) + 1.0) * (1.0 / 8192.0);
+    double u2 = (double)draw_bits(r, 25 * p + 13, 12) * (1.0 / 4096.0);
+    double rad = sqrt(-2.0 * log(u1));
+    double ang = twopi * u2;
+    n[2 * p] = rad * cos(ang);
+    n[2 * p + 1] = rad * sin(ang);
+  }
+}
+
 void orc_noise(const orc_geom* g, double* f, double sigma, uint32_t seed_lo,
                uint32_t seed_hi, uint32_t clone, uint32_t interval,
                uint32_t instance) {
@@ -542,17 +563,12 @@ void orc_noise(const orc_geom* g, double* f, double sigma, uint32_t seed_lo,
     for (int y = 0; y < g->Ly; y++) {
       if (is_interior(g, x, y)) continue;
       uint32_t site = (uint32_t)(y * g->Lx + x);
-      for (int j = 0; j < 3; j++) {
-        uint32_t ctr[4] = {3u * site + (uint32_t)j, clone, interval, instance};
-        uint32_t r[4];
-        double n[4];
-        orc_philox4x32_10(ctr, key, r);
-        orc_box_muller4(r, n);
-        for (int i = 0; i < 4; i++) {
-          int q = 4 * j + i;
-          if (q < 9) f[IDX(g, x, y, q)] += sigma * n[i];
-        }
-      }
+      uint32_t ctr[4] = {site, clone, interval, instance};
+      uint32_t r[4];
+      double n[10];
+      orc_philox4x32_10(ctr, key, r);
+      orc_site_normals(r, n);
+      for (int q = 0; q < 9; q++) f[IDX(g, x, y, q)] += sigma * n[q];
     }
   }
 }

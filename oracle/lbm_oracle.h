@@ -119,9 +119,16 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
  * u1 = (r0+1)*2^-32, u2 = r1*2^-32 -> (n0, n1); same for (r2, r3). */
 void orc_box_muller4(const uint32_t r[4], double n[4]);
 
-/* CHANNEL population noise of one clone: every fluid population gets
- * sigma * N(0,1); counters (3*site + j, clone, interval, instance),
- * key (seed_lo, seed_hi).  site = y*Lx + x. */
+/* Nine normals (n[0..8]; n[9] unused) of one site's copy noise from one
+ * Philox block r, as five Box-Muller pairs: pair p takes bits
+ * [25p, 25p + 25) of r read as one 128-bit number (r[0] least significant):
+ * a 13-bit a, u1 = (a + 1) 2^-13, then a 12-bit b, u2 = b 2^-12;
+ * n[2p] = sqrt(-2 ln u1) cos(2 pi u2), n[2p+1] = ... sin(2 pi u2). */
+void orc_site_normals(const uint32_t r[4], double n[10]);
+
+/* CHANNEL population noise of one clone: every fluid population q of site
+ * y*Lx + x gets sigma * n[q] of orc_site_normals on the Philox block with
+ * counter (site, clone, interval, instance), key (seed_lo, seed_hi). */
 void orc_noise(const orc_geom* g, double* f, double sigma, uint32_t seed_lo,
                uint32_t seed_hi, uint32_t clone, uint32_t interval,
                uint32_t instance);

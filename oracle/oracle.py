@@ -105,6 +105,8 @@ def lib() -> C.CDLL:
         L.orc_philox4x32_10.restype = None
         L.orc_box_muller4.argtypes = [u32p, dp]
         L.orc_box_muller4.restype = None
+        L.orc_site_normals.argtypes = [u32p, dp]
+        L.orc_site_normals.restype = None
         L.orc_noise.argtypes = [gp, dp, C.c_double, C.c_uint32, C.c_uint32,
                                 C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_noise.restype = None
@@ -351,6 +353,15 @@ def box_muller4(r) -> np.ndarray:
     rr = np.ascontiguousarray(r, np.uint32)
     n = np.zeros(4)
     lib().orc_box_muller4(_u32p(rr), _dp(n))
+    return n
+
+
+def site_normals(r) -> np.ndarray:
+    """The ten normals orc_site_normals makes of one Philox block (nine used
+    as one site's copy noise)."""
+    rr = np.ascontiguousarray(r, np.uint32)
+    n = np.zeros(10)
+    lib().orc_site_normals(_u32p(rr), _dp(n))
     return n
 
 

@@ -166,6 +166,9 @@ static void generators(void) {
   double n[4];
   orc_box_muller4(out, n);
   CHECK(all_finite(n, 4));
+  double sn[10];
+  orc_site_normals(out, sn);
+  CHECK(all_finite(sn, 10));
   double z[64];
   orc_field_normals(1, 2, 3, 4, 5, 32, z);
   CHECK(all_finite(z, 64));

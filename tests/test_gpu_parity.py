@@ -437,6 +437,81 @@ def test_advance_graph_matches_stepwise(lbc):
         assert np.all(np.isfinite(Ra)) and np.all(Ra > 0)
 
 
+@pytest.mark.parametrize("storage,passes", [(0, 1), (0, 2), (2, 3), (1, 3),
+                                            (3, 2)])
+def test_advance_fused_copies_match_stepwise(lbc, storage, passes):
+    """Inside lbc_advance the noisy copies of every interval but the last
+    are collided straight into the next interval's pass 0 (k_gather with
+    fuse0), and that pass is skipped for them.  The states, drags and R
+    history equal the call-by-call sequence (stored copies, full pass 0) bit
+    for bit in every storage and sweep shape, copies happen in the fused
+    intervals, and a plain interval afterwards still matches (the labels
+    are restored)."""
+    spec = lbc.channel(96, 40, 8, 24, 20, 0.55, 0.05, 24, 12, k=4.0,
+                       sigma=1e-4)
+    print(f"case: advance with fused copies vs stepwise, CHANNEL storage "
+          f"{storage} passes {passes}", flush=True)
+    kw = dict(mode=lbc.MODE_CHANNEL, storage=storage, seed=21, threads=256,
+              nt=True, passes=passes)
+    with lbc.Sampler(spec, **kw) as a, lbc.Sampler(spec, **kw) as b:
+        a.init_equilibrium(0, 1e-3)
+        b.init_equilibrium(0, 1e-3)
+        a.advance(4)
+        copies = []
+        for _ in range(4):
+            b.run_interval()
+            copies.append(len(b.select(0).pairs))
+            b.gather()
+        assert sum(copies[:3]) > 0, copies
+        assert np.array_equal(a.store_states(0), b.store_states(0))
+        assert np.array_equal(a.drag(0), b.drag(0))
+        assert np.array_equal(a.R_history(0), b.R_history(0))
+        a.run_interval()
+        b.run_interval()
+        assert np.array_equal(a.store_states(0), b.store_states(0))
+        assert np.array_equal(a.logweights(0)[1], b.logweights(0)[1])
+
+
+@pytest.mark.parametrize("storage,passes", [(0, 1), (2, 3), (3, 2)])
+def test_gather_fused_matches_gather(lbc, storage, passes):
+    """lbc_gather_fused followed by an interval equals lbc_gather followed
+    by the same interval, bit for bit (states, drags, log-weights); in
+    between, the calls that read or reselect states are refused."""
+    spec = lbc.channel(96, 40, 8, 24, 20, 0.55, 0.05, 12, 10, k=0.0,
+                       sigma=1e-4)
+    print(f"case: gather_fused vs gather, CHANNEL storage {storage} passes "
+          f"{passes}", flush=True)
+    kw = dict(mode=lbc.MODE_CHANNEL, storage=storage, seed=5, threads=256,
+              nt=True, passes=passes)
+    w = np.where(np.arange(12) % 3 == 0, 1.0, 1e-9)
+    u = np.full(12, 0.5)
+    r = np.arange(12, dtype=np.int32)
+    with lbc.Sampler(spec, **kw) as a, lbc.Sampler(spec, **kw) as b:
+        for s in (a, b):
+            s.init_equilibrium(0, 1e-3)
+            s.run_interval()
+        sa = a.select(0, weights=w, u=u, rint=r)
+        sb = b.select(0, weights=w, u=u, rint=r)
+        assert np.array_equal(sa.labels, sb.labels) and len(sa.pairs) > 0
+        a.gather_fused()
+        b.gather()
+        with pytest.raises(lbc.LbcError, match=r"\(-4\)"):
+            a.store_states(0)
+        with pytest.raises(lbc.LbcError, match=r"\(-4\)"):
+            a.select(0, weights=w, u=u, rint=r)
+        a.run_interval()
+        b.run_interval()
+        assert np.array_equal(a.store_states(0), b.store_states(0))
+        assert np.array_equal(a.drag(0), b.drag(0))
+        assert np.array_equal(a.logweights(0)[1], b.logweights(0)[1])
+        # labels are restored: a plain cycle afterwards still agrees
+        for s in (a, b):
+            s.select(0, weights=w, u=u, rint=r)
+            s.gather()
+            s.run_interval()
+        assert np.array_equal(a.store_states(0), b.store_states(0))
+
+
 @pytest.mark.parametrize("storage,tol", [(1, 2e-6), (2, 1e-3), (3, 1e-2)])
 def test_low_precision_channel_interval(O, lbc, storage, tol):
     """FP32-collide on shifted storage vs the FP64 oracle from identical
