@@ -58,14 +58,24 @@ struct Traits<kF32> {
   __device__ static C rest(int) { return 0.0f; }
 };
 
+// FP32 value to half, rounded to FP32 first: the empty asm keeps the
+// compiler from folding the operation that produced v into one
+// mixed-precision instruction that rounds to half once, which it does in
+// some kernels and not in others (the collision's own half results are
+// spelled out in fma_store).
+__device__ inline _Float16 to_half(float v) {
+  asm("" : "+v"(v));
+  return (_Float16)v;
+}
+
 template <>
 struct Traits<kF16> {
   using S = _Float16;
   using C = float;
   static constexpr bool shifted = true;
   __device__ static C ld(const S* p) { return (float)*p; }
-  __device__ static void st(S* p, C v) { *p = (_Float16)v; }
-  __device__ static S pack(C v) { return (_Float16)v; }
+  __device__ static void st(S* p, C v) { *p = to_half(v); }
+  __device__ static S pack(C v) { return to_half(v); }
   __device__ static C rest(int) { return 0.0f; }
 };
 
@@ -624,7 +634,7 @@ struct BufIO<kF16> {
   }
   template <bool NT>
   __device__ static void st(__amdgpu_buffer_rsrc_t r, int vo, int so, float v) {
-    st_raw<NT>(r, vo, so, (_Float16)v);
+    st_raw<NT>(r, vo, so, to_half(v));
   }
 };
 
@@ -1319,16 +1329,18 @@ __device__ inline int square_mask_sel(int dx, int dy, int sx, int sy) {
   return (int)((t >> sh) & 0x1FFu);
 }
 
-// The boundary lanes of a channel wave on shifted storage (wall rows, the
-// square's box, x boundaries; Lx >= 64), computed for every lane with
-// selects instead of per-lane branches: bounce-back from the site's own
-// populations `own` (src/lbm_src/domain_noSlipWalls.cpp:10-21,
-// src/lbm_src/square.cpp:24-50), the ring nodes' drag terms (momentum
-// exchange, FP32 in direction order, as ring_drag), the inlet and outlet
-// folded into the collision, the solid interior stored at rest.  The same
-// results as the general path, bit for bit.
+// The boundary lanes of a wave on shifted storage (wall rows, the square's
+// box, and in channel mode x boundaries; Lx >= 64), computed for every lane
+// with selects instead of per-lane branches: bounce-back from the site's
+// own populations `own` (src/lbm_src/domain_noSlipWalls.cpp:10-21,
+// src/lbm_src/square.cpp:24-50), the ring nodes' drag terms as ring_drag
+// computes them (REF: the stress integral's face terms,
+// src/lbm_src/force.cpp:26-46; channel: momentum exchange, FP32 in
+// direction order), the inlet and outlet folded into the collision
+// (channel), the solid interior stored at rest.  The same results as the
+// general path, bit for bit.
 template <int MODE, int ST, bool NT, class SRC, class DST, class DV, class P>
-__device__ inline void channel_boundary_lanes(
+__device__ inline void boundary_lanes(
     const Geo& g, const WaveGeo& w, int lane, int site, const P (&f)[9],
     const P (&own)[9], const DST& dst, const PassConst<ST>& pc,
     double* __restrict__ ring, const DV& dv, double beta, double tau,
@@ -1348,7 +1360,21 @@ __device__ inline void channel_boundary_lanes(
 #pragma unroll
   for (int q = 1; q < 9; q++)
     fb[q] = pick((uint32_t)__builtin_amdgcn_sbfe(m, q, 1), own[OPP[q]], f[q]);
-  if (sm != 0) {
+  if (MODE == kModeRef && sm != 0) {
+    // ring node: its faces' stress terms (one face, two at a corner)
+    float dd[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) dd[q] = pop_c<ST>(fb[q]);
+    float dr, pxx, pxy;
+    node_stress_shifted(dd, dr, pxx, pxy);
+    const float cf = 1.0f - 0.5f / (float)tau;
+    const double tW = (double)(dr * (1.0f / 3.0f) + cf * pxx);
+    const double tN = (double)(-cf * pxy);
+    if (dx == 0) ring[dy] = tW;
+    if (dx == g.sx) ring[(g.sy + 1) + dy] = -tW;
+    if (dy == g.sy) ring[2 * (g.sy + 1) + dx] = tN;
+    if (dy == 0) ring[2 * (g.sy + 1) + (g.sx + 1) + dx] = -tN;
+  } else if (sm != 0) {
     // ring node: F = sum over its bounced directions of -2 cx f_q
     float tf = 0.0f;
 #pragma unroll
@@ -1364,7 +1390,8 @@ __device__ inline void channel_boundary_lanes(
                                             : 2 * (g.sy + 1) + (g.sx + 1) + dx));
     ring[slot] = (double)tf;
   }
-  if (w.l0 == (int)kWaveNoLane && w.l1 == (int)kWaveNoLane) {
+  if (MODE == kModeRef ||
+      (w.l0 == (int)kWaveNoLane && w.l1 == (int)kWaveNoLane)) {
     collide_store<MODE, ST, NT, SRC>(fb, solid, dst, pc, dv, beta, tau, site,
                                      g.plane, PH);
   } else {
@@ -1410,6 +1437,8 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
                                              site, g.plane, PH, bc);
       return;
     }
+  }
+  if constexpr (T::shifted) {
     if (PH != kCollide && w.kind == kWaveGeneric) {
       pop_t<ST> own[9];
       if constexpr (OP) {
@@ -1418,8 +1447,8 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
       } else {
         own_pop(src, own, site, pc);
       }
-      channel_boundary_lanes<MODE, ST, NT, SRC>(g, w, lane, site, f, own, dst,
-                                                pc, ring, dv, beta, tau, I, PH);
+      boundary_lanes<MODE, ST, NT, SRC>(g, w, lane, site, f, own, dst, pc,
+                                        ring, dv, beta, tau, I, PH);
       return;
     }
   }
@@ -1577,11 +1606,11 @@ __device__ inline void fused_sweep(const Geo& g, int tid, int nthr,
   for (int u = 0; u < PF; u++)
     pull_issue(g, wave_of(I, min(u, nch - 1) * bpc + wblk), lane,
                min(u, nch - 1) * CH + tid, src, pc, pre[u]);
-  // Channel boundary blocks on shifted storage take bounce-back values from
-  // the sites' own populations; for the HBM pass they are loaded one chunk
-  // step ahead (issued before that step's pulls, so waiting for them leaves
-  // the newest chunk's pulls in flight).
-  constexpr bool kOwnAhead = MODE == kModeChannel && Traits<ST>::shifted;
+  // Boundary blocks on shifted storage take bounce-back values from the
+  // sites' own populations; for the HBM pass they are loaded one chunk step
+  // ahead (issued before that step's pulls, so waiting for them leaves the
+  // newest chunk's pulls in flight).
+  constexpr bool kOwnAhead = Traits<ST>::shifted;
   R ownr[9];
   auto own_issue = [&](int cc) {
     if constexpr (kOwnAhead) {
