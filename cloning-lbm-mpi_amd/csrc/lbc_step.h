@@ -143,6 +143,14 @@ __device__ inline float pmul(float s, H16 a) {
       : "=v"(r) : "v"(s), "v"(a.v), "s"(-0.0f));
   return r;
 }
+// s * a + c on a population a (one rounding)
+__device__ inline float pfma(float s, float a, float c) { return fmaf(s, a, c); }
+__device__ inline float pfma(float s, H16 a, float c) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,1,0]"
+      : "=v"(r) : "v"(s), "v"(a.v), "v"(c));
+  return r;
+}
 
 // Register form of a pulled population: f16 in place for FP16 storage, the
 // compute type of the storage otherwise.
@@ -328,6 +336,20 @@ __device__ inline typename Traits<ST>::S fma_store(float a, float b, float c) {
   }
 }
 
+// s*d + c as a storage element of precision ST for a population d in
+// register form (f16 in place: read by the FMA itself).
+template <int ST>
+__device__ inline typename Traits<ST>::S fma_store_pop(float s, float d, float c) {
+  return fma_store<ST>(s, d, c);
+}
+template <int ST>
+__device__ inline _Float16 fma_store_pop(float s, H16 d, float c) {
+  unsigned r;
+  asm("v_fma_mixlo_f16 %0, %1, %2, %3 op_sel_hi:[0,1,0]"
+      : "=v"(r) : "v"(s), "v"(d.v), "v"(c));
+  return __builtin_bit_cast(_Float16, (unsigned short)r);
+}
+
 // Channel x boundary of one lane, folded into the shifted storages'
 // collision (channel mode, no reference counterpart): the inlet (kind 1,
 // x = 0) and the outlet (kind 2, x = Lx-1) set the site to the equilibrium
@@ -378,28 +400,33 @@ __device__ inline void collide_shifted(const P (&d)[9],
   }
   const float usq = fmaf(ux, ux, uy * uy);
   const float base = fmaf(-1.5f * rho, usq, drho);
-  const float r45 = 4.5f * rho, r3 = 3.0f * rho;
   const float om1 = 1.0f - om;
   const float o0 = om * (4.0f / 9.0f), o1 = om * (1.0f / 9.0f),
               o2 = om * (1.0f / 36.0f);
-  // eu of directions 1, 2, 5, 6 (their opposites 3, 4, 7, 8 have -eu)
-  const float e1 = ux, e2 = uy, e5 = ux + uy, e6 = uy - ux;
-  const float t1 = fmaf(r45 * e1, e1, base), s1 = r3 * e1;
-  const float t2 = fmaf(r45 * e2, e2, base), s2 = r3 * e2;
-  const float t5 = fmaf(r45 * e5, e5, base), s5 = r3 * e5;
-  const float t6 = fmaf(r45 * e6, e6, base), s6 = r3 * e6;
-  // d_q' = omega_q (feq_q - w_q) + (1 - omega) d_q, omega_q = omega w_q
-  const float oq[9] = {o0, o1, o1, o1, o1, o2, o2, o2, o2};
-  const float tq[9] = {base,    t1 + s1, t2 + s2, t1 - s1, t2 - s2,
-                       t5 + s5, t6 + s6, t5 - s5, t6 - s6};
+  // feq_q - w_q = w_q (base + 4.5 rho eu^2 +/- 3 rho eu) for eu = e.u of
+  // directions 1, 2, 5, 6 (their opposites 3, 4, 7, 8 have -eu); with
+  // omega_q = omega w_q folded into the coefficients of each weight class,
+  // omega_q (feq_q - w_q) = (k eu +/- m) eu + b, two FMAs per direction
+  const float k1 = (4.5f * o1) * rho, m1 = (3.0f * o1) * rho, b1 = o1 * base;
+  const float k2 = (4.5f * o2) * rho, m2 = (3.0f * o2) * rho, b2 = o2 * base;
+  const float e5 = ux + uy, e6 = uy - ux;
+  const float tq[9] = {o0 * base,
+                       fmaf(fmaf(k1, ux, m1), ux, b1),
+                       fmaf(fmaf(k1, uy, m1), uy, b1),
+                       fmaf(fmaf(k1, ux, -m1), ux, b1),
+                       fmaf(fmaf(k1, uy, -m1), uy, b1),
+                       fmaf(fmaf(k2, e5, m2), e5, b2),
+                       fmaf(fmaf(k2, e6, m2), e6, b2),
+                       fmaf(fmaf(k2, e5, -m2), e5, b2),
+                       fmaf(fmaf(k2, e6, -m2), e6, b2)};
+  // d_q' = (1 - omega) d_q + omega_q (feq_q - w_q)
   if (MODE != kModeRef) {
 #pragma unroll
-    for (int q = 0; q < 9; q++)
-      o[q] = fma_store<ST>(oq[q], tq[q], pmul(om1, d[q]));
+    for (int q = 0; q < 9; q++) o[q] = fma_store_pop<ST>(om1, d[q], tq[q]);
   } else {
     float n[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) n[q] = fmaf(oq[q], tq[q], pmul(om1, d[q]));
+    for (int q = 0; q < 9; q++) n[q] = pfma(om1, d[q], tq[q]);
     // Guo forcing w_q (1 - omega/2) beta (3 (cx - ux) + 9 cx eu)
     const float G = (1.0f - 0.5f * omega) * beta;
     const float g0 = (4.0f / 9.0f) * G, g1 = (1.0f / 9.0f) * G,
