@@ -370,24 +370,28 @@ __device__ inline void collide_shifted(const P (&d)[9],
   // contraction off), so the rounding is fixed by this source and every
   // kernel that inlines it produces the same bits, with populations given
   // as floats or as f16 in place.
-  float drho = paddf(d[8], (padd(d[0], d[1]) + padd(d[2], d[3])) +
-                               (padd(d[4], d[5]) + padd(d[6], d[7])));
-  const float jx = (psub(d[1], d[3]) + psub(d[5], d[6])) + psub(d[8], d[7]);
-  const float jy = (psub(d[2], d[4]) + padd(d[5], d[6])) - padd(d[7], d[8]);
+  // moments, the density sharing the x momentum's partial sums: A and B
+  // collect the directions with cx = +1 (1, 5, 8) and cx = -1 (3, 6, 7)
+  const float A = paddf(d[8], padd(d[1], d[5]));
+  const float B = paddf(d[7], padd(d[3], d[6]));
+  const float Z = paddf(d[0], padd(d[2], d[4]));  // cx = 0: 0, 2, 4
+  float drho = (A + B) + Z;
+  const float jx = A - B;
+  const float jy = paddf(d[6], padd(d[2], d[5])) - paddf(d[8], padd(d[4], d[7]));
   float rho = 1.0f + drho;
   const float inv = __builtin_amdgcn_rcpf(rho);
   float ux = jx * inv;
   if (MODE == kModeRef) ux = fmaf(0.5f * beta, inv, ux);
   float uy = jy * inv;
+  // the momentum rho u: (jx, jy), plus the half-step force in REF mode
+  float Jx = MODE == kModeRef ? jx + 0.5f * beta : jx, Jy = jy;
   float om = omega;
   if constexpr (BC) {
     if (bc.kind == 1) {
       // rho_in - 1 = (D + u_in) / (1 - u_in) from the populations that
-      // streamed in from the domain (the weights of the known directions
-      // sum to 1 - (w1 + w5 + w8) and the w-shifts cancel)
-      const float D = ((to_f(d[0]) + to_f(d[2])) + to_f(d[4])) +
-                      2.0f * ((to_f(d[3]) + to_f(d[6])) + to_f(d[7]));
-      drho = (D + bc.u_in) * bc.kin;
+      // streamed in from the domain, D = Z + 2 B (the weights of the known
+      // directions sum to 1 - (w1 + w5 + w8) and the w-shifts cancel)
+      drho = (fmaf(2.0f, B, Z) + bc.u_in) * bc.kin;
       ux = bc.u_in;
       uy = 0.0f;
     } else if (bc.kind == 2) {
@@ -396,29 +400,32 @@ __device__ inline void collide_shifted(const P (&d)[9],
     if (bc.kind != 0) {
       rho = 1.0f + drho;
       om = 1.0f;
+      Jx = rho * ux;
+      Jy = rho * uy;
     }
   }
-  const float usq = fmaf(ux, ux, uy * uy);
-  const float base = fmaf(-1.5f * rho, usq, drho);
+  // base = drho - 1.5 rho |u|^2
+  const float base = fmaf(-1.5f, fmaf(Jx, ux, Jy * uy), drho);
   const float om1 = 1.0f - om;
   const float o0 = om * (4.0f / 9.0f), o1 = om * (1.0f / 9.0f),
               o2 = om * (1.0f / 36.0f);
   // feq_q - w_q = w_q (base + 4.5 rho eu^2 +/- 3 rho eu) for eu = e.u of
   // directions 1, 2, 5, 6 (their opposites 3, 4, 7, 8 have -eu); with
-  // omega_q = omega w_q folded into the coefficients of each weight class,
-  // omega_q (feq_q - w_q) = (k eu +/- m) eu + b, two FMAs per direction
-  const float k1 = (4.5f * o1) * rho, m1 = (3.0f * o1) * rho, b1 = o1 * base;
-  const float k2 = (4.5f * o2) * rho, m2 = (3.0f * o2) * rho, b2 = o2 * base;
-  const float e5 = ux + uy, e6 = uy - ux;
+  // omega_q = omega w_q folded in and J = rho eu,
+  // omega_q (feq_q - w_q) = J (4.5 omega_q eu +/- 3 omega_q) + omega_q base,
+  // two FMAs per direction on coefficients that are constants of the pass
+  const float c1a = 4.5f * o1, c1b = 3.0f * o1, b1 = o1 * base;
+  const float c2a = 4.5f * o2, c2b = 3.0f * o2, b2 = o2 * base;
+  const float e5 = ux + uy, e6 = uy - ux, J5 = Jx + Jy, J6 = Jy - Jx;
   const float tq[9] = {o0 * base,
-                       fmaf(fmaf(k1, ux, m1), ux, b1),
-                       fmaf(fmaf(k1, uy, m1), uy, b1),
-                       fmaf(fmaf(k1, ux, -m1), ux, b1),
-                       fmaf(fmaf(k1, uy, -m1), uy, b1),
-                       fmaf(fmaf(k2, e5, m2), e5, b2),
-                       fmaf(fmaf(k2, e6, m2), e6, b2),
-                       fmaf(fmaf(k2, e5, -m2), e5, b2),
-                       fmaf(fmaf(k2, e6, -m2), e6, b2)};
+                       fmaf(Jx, fmaf(c1a, ux, c1b), b1),
+                       fmaf(Jy, fmaf(c1a, uy, c1b), b1),
+                       fmaf(Jx, fmaf(c1a, ux, -c1b), b1),
+                       fmaf(Jy, fmaf(c1a, uy, -c1b), b1),
+                       fmaf(J5, fmaf(c2a, e5, c2b), b2),
+                       fmaf(J6, fmaf(c2a, e6, c2b), b2),
+                       fmaf(J5, fmaf(c2a, e5, -c2b), b2),
+                       fmaf(J6, fmaf(c2a, e6, -c2b), b2)};
   // d_q' = (1 - omega) d_q + omega_q (feq_q - w_q)
   if (MODE != kModeRef) {
 #pragma unroll
@@ -720,10 +727,32 @@ struct LdsIO {
   static constexpr int BMASK = RCH * ES - 1;  // ring bytes per direction - 1
   static constexpr int PLANEB = (RCH + 2) * ES;  // one direction plane
   static constexpr int BYTES = (int)ring_bytes(RCH, ES);  // one ring
+  // A DS instruction's offset field holds 16 bits.  A ring whose planes
+  // reach past it (ring 1 of the 2-byte storages at 2048-site chunks) keeps
+  // a base OB in the address register instead: (b & BMASK) | OB, one
+  // v_and_or_b32 where the other rings take one v_and_b32, with OB a
+  // multiple of the ring bytes per direction (so | adds) chosen so that
+  // every plane's remaining offset fits the field.
+  static constexpr int OB = [] {
+    constexpr int last = RING * BYTES + 8 * PLANEB + 2 * ES;  // past plane 8
+    if (RING == 0 || last <= 65536) return 0;
+    const int ob = (RING * BYTES) & ~BMASK;
+    return last - ob <= 65536 ? ob : 0;
+  }();
   __device__ static LS* at_abs(int q, int b) {
     return (LS*)(size_t)(unsigned)(RING * BYTES + q * PLANEB + b);
   }
-  __device__ static LS* at(int q, int boff) { return at_abs(q, boff & BMASK); }
+  __device__ static LS* at(int q, int boff) {
+    if constexpr (OB != 0) {
+      unsigned v;
+      // (gfx9 VOP3 reads one scalar operand at most: OB stays in a VGPR)
+      asm("v_and_or_b32 %0, %1, %2, %3"
+          : "=v"(v) : "v"(boff), "s"(BMASK), "v"(OB));
+      return (LS*)(size_t)(v + (unsigned)(RING * BYTES - OB + q * PLANEB));
+    } else {
+      return at_abs(q, boff & BMASK);
+    }
+  }
   __device__ typename Traits<ST>::C ld_byte(int q, int boff) const {
     typename Traits<ST>::S e = *at(q, boff);
     return Traits<ST>::ld(&e);
@@ -1062,21 +1091,26 @@ __device__ inline void own9(const LdsIO<ST, RCH, RING>& io,
 
 template <int ST, bool NT>
 __device__ inline void put9(const HbmIO<ST>& io, const typename Traits<ST>::S (&o)[9],
-                            int site, const PassConst<ST>& pc) {
+                            int site, const PassConst<ST>& pc, int = -1) {
   constexpr int ES = HbmIO<ST>::ES;
 #pragma unroll
   for (int q = 0; q < 9; q++)
     BufIO<ST>::template st_raw<NT>(io.r, site * ES, pc.sown(q), o[q]);
 }
 
+// wblock: the wave's block index (sites 64 * wblock ..), when the wave's
+// 64 lanes hold consecutive sites; then only the first wave of every RCH
+// sites tests its lanes for the mirrored slots 0 and 1 (a scalar branch);
+// otherwise (-1) every lane tests its slot.
 template <int ST, bool NT, int RCH, int RING>
 __device__ inline void put9(const LdsIO<ST, RCH, RING>& io,
                             const typename Traits<ST>::S (&o)[9], int site,
-                            const PassConst<ST>&) {
+                            const PassConst<ST>&, int wblock = -1) {
   using IO = LdsIO<ST, RCH, RING>;
 #pragma unroll
   for (int q = 0; q < 9; q++) io.template st<NT>(q, site, o[q]);
   // slots 0 and 1 are also kept after the plane's last slot
+  if (wblock >= 0 && (wblock & (RCH / 64 - 1)) != 0) return;
   const int slot = site & (RCH - 1);
   if (slot < 2) {
 #pragma unroll
@@ -1125,6 +1159,7 @@ struct WaveGeo {
   bool special;  // holds a wall-row site or a site of the square's box
   int l0, l1;    // Lx >= 64: the lanes of the x = 0 and x = Lx-1 sites
                  // (kWaveNoLane: none)
+  int block;     // the block index: sites 64 * block .. 64 * block + 63
 };
 
 __device__ inline WaveGeo wave_of(const InstDev* __restrict__ I, int block) {
@@ -1140,6 +1175,7 @@ __device__ inline WaveGeo wave_of(const InstDev* __restrict__ I, int block) {
   g.special = (w >> kWaveSpecialBit) & 1u;
   g.l0 = (int)(wl & kWaveNoLane);
   g.l1 = (int)((wl >> 8) & kWaveNoLane);
+  g.block = block;
   return g;
 }
 
@@ -1303,7 +1339,8 @@ __device__ inline void collide_store(P (&f)[9], bool solid, const DST& dst,
                                      const PassConst<ST>& pc, const DV& dv,
                                      double beta, double tau, int site,
                                      int plane, int PH,
-                                     LaneBC bc = LaneBC{0, 0.0f, 0.0f}) {
+                                     LaneBC bc = LaneBC{0, 0.0f, 0.0f},
+                                     int wblock = -1) {
   using T = Traits<ST>;
   if constexpr (DV::kFrame) {
     if (dv.out) {
@@ -1327,7 +1364,7 @@ __device__ inline void collide_store(P (&f)[9], bool solid, const DST& dst,
 #pragma unroll
     for (int q = 0; q < 9; q++) o[q] = T::pack(T::rest(q));
   }
-  put9<ST, NT>(dst, o, site, pc);
+  put9<ST, NT>(dst, o, site, pc, wblock);
 }
 
 // x or y at bit-select precision: mk all ones picks a, zero picks b (one
@@ -1420,12 +1457,13 @@ __device__ inline void boundary_lanes(
   if (MODE == kModeRef ||
       (w.l0 == (int)kWaveNoLane && w.l1 == (int)kWaveNoLane)) {
     collide_store<MODE, ST, NT, SRC>(fb, solid, dst, pc, dv, beta, tau, site,
-                                     g.plane, PH);
+                                     g.plane, PH, LaneBC{0, 0.0f, 0.0f},
+                                     w.block);
   } else {
     const LaneBC bc{lane == w.l0 ? 1 : (lane == w.l1 ? 2 : 0),
                     (float)I->u_in, (float)I->inlet_k};
     collide_store<MODE, ST, NT, SRC, true>(fb, solid, dst, pc, dv, beta, tau,
-                                           site, g.plane, PH, bc);
+                                           site, g.plane, PH, bc, w.block);
   }
 }
 
@@ -1446,7 +1484,8 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
   if (w.kind == kWavePlain) {
     // 64 lattice sites with no boundary lane: the common path
     collide_store<MODE, ST, NT, SRC>(f, false, dst, pc, dv, beta, tau, site,
-                                     g.plane, PH);
+                                     g.plane, PH, LaneBC{0, 0.0f, 0.0f},
+                                     w.block);
     return;
   }
   if (site >= g.plane) return;  // a lane past the lattice's last site
@@ -1461,7 +1500,7 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
       const LaneBC bc{lane == w.l0 ? 1 : (lane == w.l1 ? 2 : 0),
                       (float)I->u_in, (float)I->inlet_k};
       collide_store<MODE, ST, NT, SRC, true>(f, false, dst, pc, dv, beta, tau,
-                                             site, g.plane, PH, bc);
+                                             site, g.plane, PH, bc, w.block);
       return;
     }
   }
@@ -1511,11 +1550,11 @@ __device__ inline void site_finish(const Geo& g, const WaveGeo& w, int lane,
     }
     if (in_box && !solid) ring_drag<MODE, ST>(g, x, y, fc, ring, tau);
     collide_store<MODE, ST, NT, SRC, kFold>(fc, solid, dst, pc, dv, beta, tau,
-                                            site, g.plane, PH, bc);
+                                            site, g.plane, PH, bc, w.block);
     return;
   }
   collide_store<MODE, ST, NT, SRC>(f, false, dst, pc, dv, beta, tau, site,
-                                   g.plane, PH);
+                                   g.plane, PH, LaneBC{0, 0.0f, 0.0f}, w.block);
 }
 
 // One site of one pass: the collide-only pass loads the site's own
