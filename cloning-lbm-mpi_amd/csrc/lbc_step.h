@@ -385,30 +385,35 @@ __device__ inline void collide_shifted(const P (&d)[9],
   float uy = jy * inv;
   // the momentum rho u: (jx, jy), plus the half-step force in REF mode
   float Jx = MODE == kModeRef ? jx + 0.5f * beta : jx, Jy = jy;
-  float om = omega;
+  float om1 = 1.0f - omega;
+  float base = fmaf(-1.5f, fmaf(Jx, ux, Jy * uy), drho);  // drho - 1.5 rho u^2
   if constexpr (BC) {
-    if (bc.kind == 1) {
-      // rho_in - 1 = (D + u_in) / (1 - u_in) from the populations that
-      // streamed in from the domain, D = Z + 2 B (the weights of the known
-      // directions sum to 1 - (w1 + w5 + w8) and the w-shifts cancel)
-      drho = (fmaf(2.0f, B, Z) + bc.u_in) * bc.kin;
-      ux = bc.u_in;
-      uy = 0.0f;
-    } else if (bc.kind == 2) {
-      drho = 0.0f;  // the outlet holds the reference density, own velocity
-    }
-    if (bc.kind != 0) {
-      rho = 1.0f + drho;
-      om = 1.0f;
-      Jx = rho * ux;
-      Jy = rho * uy;
-    }
+    // A boundary lane collides its boundary moments with omega = 1: its
+    // result is their equilibrium, w_q t_q.  The pass's coefficients below
+    // carry omega w_q, so the lane scales its momentum and base by
+    // 1 / omega and drops the (1 - omega) d_q term; selects, no branches.
+    const bool in = bc.kind == 1, edge = bc.kind != 0;
+    // inlet: rho_in - 1 = (D + u_in) / (1 - u_in) from the populations that
+    // streamed in from the domain, D = Z + 2 B (the weights of the known
+    // directions sum to 1 - (w1 + w5 + w8) and the w-shifts cancel); the
+    // outlet holds the reference density with its own velocity
+    const float dr_in = (fmaf(2.0f, B, Z) + bc.u_in) * bc.kin;
+    const float dr_b = in ? dr_in : 0.0f;
+    const float ux_b = in ? bc.u_in : ux, uy_b = in ? 0.0f : uy;
+    const float rho_b = 1.0f + dr_b;
+    const float sc = __builtin_amdgcn_rcpf(omega);
+    const float Jx_b = (rho_b * ux_b) * sc, Jy_b = (rho_b * uy_b) * sc;
+    const float base_b =
+        fmaf(-1.5f, fmaf(rho_b * ux_b, ux_b, (rho_b * uy_b) * uy_b), dr_b) * sc;
+    ux = edge ? ux_b : ux;
+    uy = edge ? uy_b : uy;
+    Jx = edge ? Jx_b : Jx;
+    Jy = edge ? Jy_b : Jy;
+    base = edge ? base_b : base;
+    om1 = edge ? 0.0f : om1;
   }
-  // base = drho - 1.5 rho |u|^2
-  const float base = fmaf(-1.5f, fmaf(Jx, ux, Jy * uy), drho);
-  const float om1 = 1.0f - om;
-  const float o0 = om * (4.0f / 9.0f), o1 = om * (1.0f / 9.0f),
-              o2 = om * (1.0f / 36.0f);
+  const float o0 = omega * (4.0f / 9.0f), o1 = omega * (1.0f / 9.0f),
+              o2 = omega * (1.0f / 36.0f);
   // feq_q - w_q = w_q (base + 4.5 rho eu^2 +/- 3 rho eu) for eu = e.u of
   // directions 1, 2, 5, 6 (their opposites 3, 4, 7, 8 have -eu); with
   // omega_q = omega w_q folded in and J = rho eu,
