@@ -199,11 +199,13 @@ def time_storage(storage: int, K: int, W: int, specs: list, mode: int,
     """W untimed warm-up intervals then K timed ones (graph replays), then
     G selections and copies of the last interval's clones with the copy
     kernel timed alone.  shape: (threads, sites per thread, non-temporal,
-    LBM passes per HBM sweep) or None (tuned at context creation)."""
+    LBM passes per HBM sweep[, clones per workgroup]) or None (tuned at
+    context creation)."""
     kw = {}
     if shape:
         kw = dict(threads=shape[0], vec=shape[1], nt=bool(shape[2]),
-                  passes=shape[3] if len(shape) > 3 else 1)
+                  passes=shape[3] if len(shape) > 3 else 1,
+                  cpw=shape[4] if len(shape) > 4 else 1)
     with lbc.Sampler(specs, mode=mode, storage=storage, seed=seed,
                      **kw) as s:
         prepare(s, specs)
@@ -220,6 +222,7 @@ def time_storage(storage: int, K: int, W: int, specs: list, mode: int,
         vec = s.vec
         nt = s.nt
         passes = s.passes
+        cpw = s.cpw
         # the copy kernel on its own, through the API (the kernels the graph
         # replays), each launch timed with events: G selections on the last
         # interval's weights, then G on weights that copy every other clone
@@ -296,6 +299,7 @@ def time_storage(storage: int, K: int, W: int, specs: list, mode: int,
         sites_per_thread=vec,
         nontemporal=nt,
         passes_per_sweep=passes,
+        clones_per_workgroup=cpw,
         logR_last=float(math.log(R[-1])) if len(R) else None,
         gather_workload=gather_stats(gather),
         gather=gather_stats(gather_half),
@@ -739,8 +743,8 @@ def main() -> int:
                          "HBM counters can be scaled against known bytes")
     ap.add_argument("--shape", default="",
                     help="threads,sites_per_thread,nontemporal,passes_per_sweep"
-                         " for every storage (skips the tuning at context "
-                         "creation)")
+                         "[,clones_per_workgroup] for every storage (skips "
+                         "the tuning at context creation)")
     ap.add_argument("--only", default="",
                     help="comma list of storages to time (f64,f32,f16,bf16); "
                          "default: the workload's list")

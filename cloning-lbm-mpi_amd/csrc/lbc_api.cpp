@@ -53,6 +53,8 @@ struct lbc_ctx {
   Counters* counters = nullptr;
   WorkItem* items = nullptr;
   int n_items = 0;
+  WorkItem* items2 = nullptr;         // the same clones in same-instance pairs
+  int n_items2 = 0;
   int* flip_mask = nullptr;
   double* scgf = nullptr;             // running estimate of one instance
   std::vector<int> parity;            // host mirror of counters->parity
@@ -64,6 +66,7 @@ struct lbc_ctx {
   int vec = 1;                        // sites per thread per iteration (step kernel)
   int nt = 0;                         // non-temporal population stores
   int passes = 1;                     // LBM passes per HBM sweep (LDS rings)
+  int cpw = 1;                        // clones per workgroup (one-pass kernel)
   int rch = 0;                        // LDS ring sites per direction
   int max_lx = 0;
   int ring_max = 0;
@@ -159,10 +162,15 @@ int validate(const lbc_config* cfg, std::string& why) {
   {
     const int v = cfg->flags & 7;
     const int ntm = (cfg->flags >> 3) & 3;
-    if ((cfg->flags & ~127) || (v != 0 && v != 1) || ntm == 3)
+    const int cpm = (cfg->flags >> 7) & 3;
+    const int pm = (cfg->flags >> 5) & 3;
+    if ((cfg->flags & ~511) || (v != 0 && v != 1) || ntm == 3 || cpm == 3 ||
+        (cpm == 2 && pm > 1))
       return why = "flags: bits 0-2 sites per thread (0 or 1), bits 3-4 "
                    "non-temporal stores (0 tuned, 1 off, 2 on), bits 5-6 "
-                   "LBM passes per HBM sweep (0 tuned, 1, 2 or 3)", -1;
+                   "LBM passes per HBM sweep (0 tuned, 1, 2 or 3), bits 7-8 "
+                   "clones per workgroup (0 tuned, 1, or 2 with one pass per "
+                   "sweep)", -1;
   }
   for (int i = 0; i < cfg->n_instances; i++) {
     const lbc_instance& s = cfg->instances[i];
@@ -249,6 +257,9 @@ lbc::Launch make_launch(lbc_ctx* c) {
   L.passes = c->passes;
   L.rch = c->rch;
   L.forced = c->forced ? 1 : 0;
+  L.cpw = c->cpw;
+  L.items2 = c->items2;
+  L.n_items2 = c->n_items2;
   L.stream = c->stream;
   return L;
 }
@@ -494,19 +505,27 @@ int autotune(lbc_ctx* c) {
   const int vfixed = c->cfg.flags & 7;
   const int ntmode = (c->cfg.flags >> 3) & 3;  // 0 tuned, 1 off, 2 on
   const int pmode = (c->cfg.flags >> 5) & 3;   // 0 tuned, else passes per sweep
+  const int cpmode = (c->cfg.flags >> 7) & 3;  // 0 tuned, else clones per group
   float best = 1e30f;
-  int best_t = 256, best_v = 1, best_nt = 0, best_p = 1;
+  int best_t = 256, best_v = 1, best_nt = 0, best_p = 1, best_cpw = 1;
   hipEvent_t e0, e1;
   HIPC(c, hipEventCreate(&e0));
   HIPC(c, hipEventCreate(&e1));
   for (int passes = 1; passes <= 3; passes++)
   for (int nt = 0; nt < 2; nt++)
-  for (int v : {1}) {
+  for (int v : {1})
+  for (int cpw = 1; cpw <= 2; cpw++) {
     if (vfixed && v != vfixed) continue;
     if ((ntmode == 1 && nt) || (ntmode == 2 && !nt)) continue;
     if (pmode && passes != pmode) continue;
+    // two clones per workgroup: the one-pass kernel only (the multi-pass
+    // kernels' LDS rings are per clone, so two clones need twice the LDS
+    // of one and no more waves fit per CU)
+    if (cpw == 2 && passes != 1) continue;
+    if (cpmode && cpw != cpmode) continue;
     for (int t : tcands) {
       if (c->cfg.threads && t != c->cfg.threads) continue;
+      if (cpw * t > 1024) continue;
       const int rch = lbc::fused_ring_sites(t, c->max_lx, c->cfg.mode);
       if (passes > 1 &&
           !lbc::fused_lds_bytes(c->cfg.storage, c->ring_max, rch, passes))
@@ -517,6 +536,7 @@ int autotune(lbc_ctx* c) {
       L.nt = nt;
       L.passes = passes;
       L.rch = rch;
+      L.cpw = cpw;
       // a shape the device refuses to launch (a configuration error,
       // reported at launch) is skipped; the others are still timed
       if (lbc::launch_interval(L) != hipSuccess) {  // warm-up
@@ -535,6 +555,7 @@ int autotune(lbc_ctx* c) {
         best_v = v;
         best_nt = nt;
         best_p = passes;
+        best_cpw = cpw;
       }
     }
   }
@@ -546,6 +567,7 @@ int autotune(lbc_ctx* c) {
   c->vec = best_v;
   c->nt = best_nt;
   c->passes = best_p;
+  c->cpw = best_cpw;
   c->rch = lbc::fused_ring_sites(best_t, c->max_lx, c->cfg.mode);
   HIPC(c, hipMemcpyAsync(c->idd, c->idh.data(), sizeof(InstDev) * c->idh.size(),
                          hipMemcpyHostToDevice, c->stream));
@@ -569,7 +591,8 @@ int lbc_sites_per_thread(const lbc_ctx* ctx) { return ctx ? ctx->vec : 0; }
 
 int lbc_step_shape(const lbc_ctx* ctx) {
   if (!ctx) return 0;
-  return ctx->vec | ((ctx->nt ? 2 : 1) << 3) | (ctx->passes << 5);
+  return ctx->vec | ((ctx->nt ? 2 : 1) << 3) | (ctx->passes << 5) |
+         (ctx->cpw << 7);
 }
 
 void lbc_destroy(lbc_ctx* c) {
@@ -742,10 +765,24 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     for (int j = 0; j < c->inst[i].n_clones; j++) items.push_back({i, j});
   c->n_items = (int)items.size();
   if (dalloc(c, &c->items, sizeof(WorkItem) * items.size())) return bail(LBC_E_NOMEM);
+  // the same clones in pairs of one instance for two clones per workgroup
+  // (both halves then make the same passes); an odd clone count leaves the
+  // last clone with a padding partner {inst, -1}
+  std::vector<WorkItem> items2;
+  for (int i : order) {
+    const int n = c->inst[i].n_clones;
+    for (int j = 0; j < n; j++) items2.push_back({i, j});
+    if (n & 1) items2.push_back({i, -1});
+  }
+  c->n_items2 = (int)items2.size();
+  if (dalloc(c, &c->items2, sizeof(WorkItem) * items2.size()))
+    return bail(LBC_E_NOMEM);
   Counters zero{};
   if (hipMemcpy(c->idd, c->idh.data(), sizeof(InstDev) * ni,
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->items, items.data(), sizeof(WorkItem) * items.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->items2, items2.data(), sizeof(WorkItem) * items2.size(),
                 hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->counters, &zero, sizeof zero, hipMemcpyHostToDevice) !=
           hipSuccess)
@@ -772,6 +809,11 @@ int lbc_create(const lbc_config* cfg, lbc_ctx** out) {
     c->vec = vfixed;
     c->nt = ntmode == 2;
     c->passes = pmode;
+    c->cpw = ((cfg->flags >> 7) & 3) == 2 ? 2 : 1;
+    if (c->cpw == 2 && 2 * c->threads > 1024)
+      return bail(fail(c, LBC_E_ARG,
+                       "step kernel: two clones per workgroup take at most "
+                       "512 threads each"));
     c->rch = lbc::fused_ring_sites(c->threads, c->max_lx, c->cfg.mode);
     if (c->passes > 1 &&
         !lbc::fused_lds_bytes(cfg->storage, c->ring_max, c->rch, c->passes))

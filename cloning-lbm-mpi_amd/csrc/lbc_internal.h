@@ -93,6 +93,12 @@ struct Launch {
   int passes;        // LBM passes per HBM sweep (1, 2 or 3)
   int rch;           // LDS ring sites per direction (fused)
   int forced;        // some instance has forced steps (random-field force)
+  // clones per workgroup of the one-pass step kernel (1 or 2); with 2 it
+  // runs over items2, pairs of clones of one instance (a lone clone's
+  // partner is {inst, -1}: it runs the barriers only)
+  int cpw;
+  const WorkItem* items2;
+  int n_items2;
   hipStream_t stream;
 };
 

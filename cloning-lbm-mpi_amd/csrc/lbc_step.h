@@ -1860,35 +1860,46 @@ __device__ inline void finish_clone(const InstDev* __restrict__ I,
 // S_M in buf[parity ^ ((M+1)&1)].  Drag F_n is taken on S_n in pass n.
 // Pass n collides step n; with FRC the passes n < forced_steps collide with
 // the clone's random-field force (src/mpi_LBM_GK_forcing.cpp:335-364).
-template <int MODE, int ST, bool NT, bool FRC>
+// CPW clones per workgroup: with 2, items holds pairs of clones of one
+// instance (so both halves make the same passes and meet the same
+// barriers), each half of the workgroup runs one of them with its own drag
+// slots, and a padding item {inst, -1} only meets the barriers.
+template <int MODE, int ST, bool NT, bool FRC, int CPW>
 __global__ void __launch_bounds__(1024)
     k_interval(const InstDev* __restrict__ insts,
                const WorkItem* __restrict__ items,
                const Counters* __restrict__ counters, int ring_max) {
   extern __shared__ double ring_lds[];
-  const WorkItem wi = items[blockIdx.x];
+  const int nthr = (int)blockDim.x / CPW;
+  // the half of the workgroup, uniform over each wave (nthr is a multiple
+  // of 64)
+  const int sub =
+      CPW == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / nthr);
+  const int tid = (int)threadIdx.x - sub * nthr;
+  WorkItem wi = items[blockIdx.x * CPW + sub];
+  const bool live = wi.clone >= 0;
+  if (!live) wi.clone = 0;  // set up on clone 0 of the instance, never written
   const InstDev* __restrict__ I = insts + wi.inst;
   const KernelSetup k = setup_clone<ST>(I, wi, counters->parity[wi.inst]);
-  const int nthr = blockDim.x, tid = threadIdx.x;
   const HbmIO<ST> ioA{clone_rsrc(k.A, k.clone_bytes), k.g.plane};
   const HbmIO<ST> ioB{clone_rsrc(k.B, k.clone_bytes), k.g.plane};
   const int M1 = FRC ? I->forced_steps : 0;
   // a copy collided by k_gather (label -1 - parent) has had its pass 0
-  const bool done0 = I->labels[wi.clone] < 0;
+  const bool done0 = live && I->labels[wi.clone] < 0;
   double acc = 0.0;
   for (int n = 0; n <= k.M; ++n) {
     const HbmIO<ST>& src = (n & 1) ? ioB : ioA;
     const HbmIO<ST>& dst = (n & 1) ? ioA : ioB;
-    double* ring = ring_lds + (n & 1) * ring_max;
+    double* ring = ring_lds + (2 * sub + (n & 1)) * ring_max;
     const int ph = (n == 0) ? kCollide : (n == k.M ? kStream : kBoth);
     const LonePass<DriveOf<FRC, ST>> lp{drive_of<FRC, ST>(I, wi.clone, n < M1),
                                   macro_out(I, wi.clone, n, k.M)};
-    if (n > 0 || !done0)
+    if (live && (n > 0 || !done0))
       site_pass<MODE, ST, NT>(k.g, tid, nthr, src, dst, ring, lp, k.tau, I, ph);
     __syncthreads();
-    if (n >= 1 && tid == 0) take_drag<MODE>(k, ring, n, acc);
+    if (live && n >= 1 && tid == 0) take_drag<MODE>(k, ring, n, acc);
   }
-  if (tid == 0) {
+  if (live && tid == 0) {
     finish_clone<MODE>(I, wi, acc);
     if (done0) I->labels[wi.clone] = -1 - I->labels[wi.clone];
   }
@@ -2032,8 +2043,16 @@ template <int MODE, int ST, bool NT, bool FRC>
 hipError_t launch_interval_k(const Launch& L) {
   if (L.passes == 2) return launch_fused_r<MODE, ST, NT, 2, FRC>(L);
   if (L.passes == 3) return launch_fused_r<MODE, ST, NT, 3, FRC>(L);
+  if (L.cpw == 2) {
+    if (2 * L.threads > 1024) return hipErrorInvalidValue;
+    const size_t lds = sizeof(double) * 4 * (size_t)L.ring_max;
+    hipLaunchKernelGGL((k_interval<MODE, ST, NT, FRC, 2>), dim3(L.n_items2 / 2),
+                       dim3(2 * L.threads), lds, L.stream, L.insts, L.items2,
+                       L.counters, L.ring_max);
+    return hipGetLastError();
+  }
   const size_t lds = sizeof(double) * 2 * (size_t)L.ring_max;
-  hipLaunchKernelGGL((k_interval<MODE, ST, NT, FRC>), dim3(L.n_items),
+  hipLaunchKernelGGL((k_interval<MODE, ST, NT, FRC, 1>), dim3(L.n_items),
                      dim3(L.threads), lds, L.stream, L.insts, L.items,
                      L.counters, L.ring_max);
   return hipGetLastError();

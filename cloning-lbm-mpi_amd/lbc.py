@@ -267,11 +267,13 @@ class Sampler:
     def __init__(self, specs, mode: int = MODE_CHANNEL,
                  storage: int = STORE_F64, seed: int = 0, threads: int = 0,
                  device: int = 0, vec: int = 0, nt: int | None = None,
-                 fused: bool | None = None, passes: int | None = None):
+                 fused: bool | None = None, passes: int | None = None,
+                 cpw: int = 0):
         """threads = 0 tunes the step kernel's shape at creation; with
         threads given, unspecified nt / passes default to off / one pass.
         passes: LBM passes per HBM sweep (1, 2 or 3); fused=True is
-        passes=2, fused=False passes=1."""
+        passes=2, fused=False passes=1.  cpw: clones per workgroup (1, or 2
+        with one pass per sweep; 0 = tuned where the tuner runs, else 1)."""
         if isinstance(specs, InstanceSpec):
             specs = [specs]
         self.specs = list(specs)
@@ -289,7 +291,7 @@ class Sampler:
         if passes is None:
             passes = 1 if threads else 0
         flags = (vec | ((0 if nt is None else (2 if nt else 1)) << 3)
-                 | (passes << 5))
+                 | (passes << 5) | (cpw << 7))
         cfg = Config(mode, storage, len(self.specs), device, seed, threads,
                      flags, C.cast(arr, C.POINTER(Instance)))
         h = C.c_void_p()
@@ -339,6 +341,11 @@ class Sampler:
     def passes(self) -> int:
         """LBM passes per HBM sweep of the step kernel (LDS rings between)."""
         return (lib().lbc_step_shape(self._h) >> 5) & 3
+
+    @property
+    def cpw(self) -> int:
+        """Clones per workgroup of the step kernel (1 or 2)."""
+        return (lib().lbc_step_shape(self._h) >> 7) & 3
 
     @property
     def fused(self) -> bool:

@@ -110,8 +110,10 @@ typedef struct lbc_config {
                                  bits 3-4 non-temporal population stores
                                  (1 off, 2 on), bits 5-6 LBM passes per HBM
                                  sweep, chained through LDS rings (1, 2 or
-                                 3); other bits 0.  Every shape gives the
-                                 same results. */
+                                 3), bits 7-8 clones per workgroup (1, or 2
+                                 with one pass per sweep; 0: tuned where
+                                 the tuner runs, else 1); other bits 0.
+                                 Every shape gives the same results. */
   const lbc_instance* instances;
 } lbc_config;
 
@@ -273,7 +275,8 @@ int lbc_sites_per_thread(const lbc_ctx* ctx);
 /* The step-kernel shape the context settled on (tuned or pinned), in the
  * encoding of lbc_config.flags with every field resolved: sites per thread
  * in bits 0-2, non-temporal stores in bits 3-4 (1 off, 2 on), LBM passes
- * per HBM sweep in bits 5-6 (1, 2 or 3).  0 for a null context. */
+ * per HBM sweep in bits 5-6 (1, 2 or 3), clones per workgroup in bits 7-8
+ * (1 or 2).  0 for a null context. */
 int lbc_step_shape(const lbc_ctx* ctx);
 
 /* Largest population of lbc_select_static. */

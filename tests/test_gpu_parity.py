@@ -607,6 +607,41 @@ def test_multi_pass_sweeps_equal_single_passes(O, lbc, storage, mode):
                 f"{v[tuple(bad[0])]!r})")
 
 
+@pytest.mark.parametrize("storage", [0, 2])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_two_clones_per_workgroup_equal_one(lbc, storage, mode):
+    """Two clones per workgroup of the one-pass step kernel (pairs of clones
+    of one instance, a padding partner for an odd count) give bit-identical
+    states, drag and weights to one clone per workgroup, for two instances
+    of different lattices and odd clone counts."""
+    if mode == 0:
+        specs = [lbc.ref_box(8, 0.55, 0.05, 5, 9, alpha=2.0),
+                 lbc.ref_box(4, 0.55, 0.05, 3, 9, alpha=2.0)]
+    else:
+        specs = [lbc.channel(300, 40, 8, 75, 20, 0.55, 0.05, 5, 9, k=0.5),
+                 lbc.channel(120, 30, 6, 30, 15, 0.55, 0.05, 3, 9, k=0.5)]
+    print(f"case: two clones per workgroup vs one, storage={storage} "
+          f"mode={mode}", flush=True)
+    out = []
+    for cpw in (1, 2):
+        with lbc.Sampler(specs, mode=mode, storage=storage, seed=5,
+                         threads=256, passes=1, cpw=cpw) as s:
+            assert s.cpw == cpw
+            for i in range(len(specs)):
+                s.init_equilibrium(i, 1e-3)
+            s.run_interval()
+            s.run_interval()
+            out.append([(s.store_states(i), s.drag(i), s.logweights(i)[1])
+                        for i in range(len(specs))])
+    for i in range(len(specs)):
+        for name, u, v in zip(("states", "drag", "logw"), out[0][i],
+                              out[1][i]):
+            bad = np.argwhere(u != v)
+            assert bad.size == 0, (
+                f"instance {i}, {name}: {len(bad)} of {u.size} differ, first "
+                f"at {bad[0].tolist()}")
+
+
 def test_scgf_estimate_matches_oracle(O, lbc):
     """The device's running large-deviation estimate equals the oracle's
     restatement of read_rvalue.py on the sampler's own mean weights; with
