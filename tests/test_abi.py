@@ -36,10 +36,12 @@ def _bad(lbc, **kw):
     storage = kw.pop("storage", lbc.STORE_F64)
     threads = kw.pop("threads", 0)
     vec = kw.pop("vec", 0)
+    shape = {k: kw.pop(k) for k in ("passes", "cpw") if k in kw}
     for k, v in kw.items():
         setattr(spec, k, v)
     with pytest.raises(lbc.LbcError) as e:
-        lbc.Sampler(spec, mode=mode, storage=storage, threads=threads, vec=vec)
+        lbc.Sampler(spec, mode=mode, storage=storage, threads=threads, vec=vec,
+                    **shape)
     assert "(-1)" in str(e.value)
     return str(e.value)
 
@@ -60,6 +62,9 @@ def test_create_rejects_invalid_configurations(lbc):
     assert "storage" in _bad(lbc, storage=9)
     assert "threads" in _bad(lbc, threads=100)
     assert "sites per thread" in _bad(lbc, threads=256, vec=2)
+    # two clones per workgroup: the one-pass kernel only; 3 is no count
+    assert "clones per workgroup" in _bad(lbc, threads=256, passes=2, cpw=2)
+    assert "clones per workgroup" in _bad(lbc, threads=256, passes=1, cpw=3)
     assert "non-finite" in _bad(lbc, bias=float("nan"))
     # forcing mode and macroscopic output
     assert "forced_steps" in _bad(lbc, forced_steps=11)
