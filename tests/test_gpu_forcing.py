@@ -172,6 +172,31 @@ def test_forced_multi_pass_sweeps_equal_single_passes(O, lbc, storage, M1):
         assert rel < (1e-2 if storage == 2 else 1e-4), (j, rel)
 
 
+@pytest.mark.parametrize("storage", [0, 2])
+def test_forced_two_clones_per_workgroup_equal_one(O, lbc, storage):
+    """The one-pass step kernel with the random-field force gives the same
+    bits with two clones per workgroup (three clones: one pair and a padded
+    one) as with one."""
+    spec = lbc.ref_box(40, 0.55, 0.05, 3, 9, alpha=2.0, zero_mean=False)
+    spec.forced_steps = 5
+    spec.field_eps = 0.5
+    g = O.ref_box_geom(40, 0.55, 0.05)
+    st = _states(O, g, 3, 1)
+    maps = np.random.default_rng(2).standard_normal((3, g.Lx, g.Ly))
+    print(f"case: forced two clones per workgroup storage={storage}", flush=True)
+    out = []
+    for cpw in (1, 2):
+        with lbc.Sampler(spec, mode=lbc.MODE_REF, storage=storage, threads=256,
+                         passes=1, cpw=cpw) as s:
+            assert s.cpw == cpw
+            s.load_states(0, 0, st)
+            s.set_field_maps(0, maps)
+            s.run_interval()
+            out.append((s.store_states(0), s.drag(0), s.logweights(0)[1]))
+    for name, u, v in zip(("states", "drag", "logw"), out[0], out[1]):
+        assert np.array_equal(u, v), f"two clones per workgroup: {name} differ"
+
+
 @pytest.mark.parametrize("N", [7, 1])
 @pytest.mark.parametrize("storage", [0, 1, 2, 3])
 def test_fields_of_every_storage_match_oracle(O, lbc, storage, N):
